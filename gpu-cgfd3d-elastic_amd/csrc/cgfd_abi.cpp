@@ -1,0 +1,1045 @@
+// cgfd_abi.cpp -- session management and the C ABI of the gfx950 el_iso path.
+//
+// The session plays the role of the device half of drv_rk_curv_col_allstep
+// (src/forward/drv_rk_curv_col.cu:28-567): it owns the device copies of the
+// metric, medium, free-surface matrices, PML profiles and memory variables,
+// source tables and receivers (init_*_device, src/forward/alloc.cu:9-419),
+// and runs the RK4 time loop with the reference's step/stage semantics.
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/cgfd_el_iso.h"
+#include "el_iso_device.h"
+#include "el_iso_launch.h"
+
+using namespace cgfd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess) return fail(CGFD_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+// RK4 coefficients (src/forward/fd_t.cu:36-49) and operator pair flags (fd_t.cu:184-194)
+const double kRkA[3] = {0.5, 0.5, 1.0};
+const double kRkB[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+const int kFlags[8][3] = {{0, 0, 0}, {1, 1, 0}, {1, 1, 1}, {0, 0, 1},
+                          {0, 1, 0}, {1, 0, 0}, {1, 0, 1}, {0, 1, 1}};
+
+// host float -> half / bfloat16, round to nearest even
+uint16_t f2h(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  uint32_t sign = (x >> 16) & 0x8000u;
+  uint32_t mant = x & 0x7fffffu;
+  int exp = (int)((x >> 23) & 0xff);
+  if (exp == 0xff) return (uint16_t)(sign | 0x7c00u | (mant ? 0x200u : 0));
+  int e = exp - 127 + 15;
+  if (e >= 31) return (uint16_t)(sign | 0x7c00u);
+  if (e <= 0) {
+    if (e < -10) return (uint16_t)sign;
+    mant |= 0x800000u;
+    int shift = 14 - e;
+    uint32_t half = mant >> shift;
+    uint32_t rem = mant & ((1u << shift) - 1), mid = 1u << (shift - 1);
+    if (rem > mid || (rem == mid && (half & 1))) half++;
+    return (uint16_t)(sign | half);
+  }
+  uint32_t half = (uint32_t)(e << 10) | (mant >> 13);
+  uint32_t rem = mant & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (half & 1))) half++;
+  return (uint16_t)(sign | half);
+}
+float h2f(uint16_t h) {
+  uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  int exp = (h >> 10) & 0x1f;
+  uint32_t mant = h & 0x3ffu;
+  uint32_t x;
+  if (exp == 0) {
+    if (mant == 0) x = sign;
+    else {
+      exp = 1;
+      while (!(mant & 0x400u)) { mant <<= 1; exp--; }
+      mant &= 0x3ffu;
+      x = sign | (uint32_t)(exp + 127 - 15) << 23 | (mant << 13);
+    }
+  } else if (exp == 31) x = sign | 0x7f800000u | (mant << 13);
+  else x = sign | (uint32_t)(exp + 127 - 15) << 23 | (mant << 13);
+  float f;
+  std::memcpy(&f, &x, 4);
+  return f;
+}
+uint16_t f2bf(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  if ((x & 0x7f800000u) == 0x7f800000u && (x & 0x7fffffu)) return (uint16_t)((x >> 16) | 0x40);
+  uint32_t lsb = (x >> 16) & 1u;
+  x += 0x7fffu + lsb;
+  return (uint16_t)(x >> 16);
+}
+float bf2f(uint16_t b) {
+  uint32_t x = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &x, 4);
+  return f;
+}
+
+double rd(int dtype, const void *p, size_t i) {
+  return dtype == CGFD_DT_F64 ? ((const double *)p)[i] : (double)((const float *)p)[i];
+}
+
+struct FaceHost {
+  bool on = false;
+  int nlay = 0;
+  int i1 = 0, i2 = -1, j1 = 0, j2 = -1, k1 = 0, k2 = -1;
+  long fsize = 0;
+  void *coef = nullptr;   // A,B,D (model type)
+  void *aux = nullptr;    // nshots x 5 levels x 9 x fsize (storage type)
+};
+
+struct SrcHost {
+  int nsrc = 0, force_on = 0, moment_on = 0, max_nt = 0, max_ext = 0;
+  std::vector<int> itb, ite, ext_num;
+  std::vector<long long> ext_indx;
+  std::vector<double> ext_coef, F, M;
+};
+
+struct RecvHost {
+  int nrecv = 0, nt = 0;
+  std::vector<long long> indx8;
+  std::vector<double> frac;
+};
+
+}  // namespace
+
+struct cgfd_session {
+  cgfd_desc d{};
+  int dev = 0;
+  hipStream_t st = nullptr;
+  Layout L{};
+  size_t es = 4, ms = 4;   // storage / model element sizes
+  long level_elems = 0;    // 9 * cs
+  long shot_elems = 0;     // 4 * level_elems
+  void *wav = nullptr;     // nshots x 4 levels
+  void *rhs = nullptr;     // nshots x 1 level (lazy)
+  void *met = nullptr, *med = nullptr, *mat = nullptr;
+  bool have_met = false, have_med = false;
+  int free_top = 0;
+  FaceHost face[6];
+  int slot[4] = {0, 1, 2, 3};  // physical level slot of PRE, TMPA, TMPB, END
+  float sc[9], isc[9];
+  std::vector<SrcHost> src;
+  std::vector<RecvHost> rec;
+  // device source CSR
+  int s_npt = 0;
+  void *s_buf = nullptr;
+  size_t s_bytes = 0;
+  int *s_pt_shot = nullptr, *s_pt_beg = nullptr, *s_ct_src = nullptr, *s_itb = nullptr, *s_ite = nullptr,
+      *s_kind = nullptr;
+  long *s_pt_off = nullptr, *s_tab = nullptr;
+  void *s_ct_coef = nullptr, *s_F = nullptr, *s_M = nullptr;
+  long s_ntab = 0;
+  bool src_dirty = false;
+  // device receivers
+  int r_n = 0;
+  int *r_shot = nullptr, *r_nt = nullptr;
+  long *r_off = nullptr, *r_out = nullptr;
+  void *r_frac = nullptr, *seismo = nullptr;
+  long seismo_elems = 0;
+  std::vector<long> seismo_off;  // per shot
+  bool rec_dirty = false;
+  int *clock = nullptr, *shot_start = nullptr, *shot_active = nullptr;
+  // timing
+  bool timing = true;
+  std::vector<hipEvent_t> ev;
+  hipEvent_t ev_run0 = nullptr, ev_run1 = nullptr;
+  double last_run_ms = 0, last_int_avg_ms = 0;
+  int last_int_n = 0;
+  std::vector<void *> allocs;
+};
+
+namespace {
+
+int dalloc(cgfd_session *s, void **p, size_t bytes) {
+  if (bytes == 0) { *p = nullptr; return CGFD_OK; }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(CGFD_E_NOMEM, "insufficient HBM: hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  }
+  s->allocs.push_back(*p);
+  return CGFD_OK;
+}
+
+void dfree(cgfd_session *s, void *&p) {
+  if (!p) return;
+  auto it = std::find(s->allocs.begin(), s->allocs.end(), p);
+  if (it != s->allocs.end()) s->allocs.erase(it);
+  (void)hipFree(p);
+  p = nullptr;
+}
+
+Layout make_layout(int ni, int nj, int nk, size_t es) {
+  Layout L;
+  L.ni = ni; L.nj = nj; L.nk = nk;
+  const int align = (int)(128 / es);  // elements per 128 bytes
+  L.xa = align;                        // i = 0 at a 128-byte boundary, ghosts before it
+  long need = (long)L.xa + ni + kNG;
+  L.px = (need + align - 1) / align * align;
+  L.pz = L.px * (nj + 2 * kNG);
+  L.cs = L.pz * (nk + 2 * kNG);
+  return L;
+}
+
+long ref_to_dev(const Layout &L, long long ref) {
+  const long nx = L.ni + 2 * kNG, ny = L.nj + 2 * kNG;
+  long ir = (long)(ref % nx);
+  long t = (long)(ref / nx);
+  long jr = t % ny, kr = t / ny;
+  return kr * L.pz + jr * L.px + (ir - kNG + L.xa);
+}
+
+void face_box(const cgfd_session *s, int f, int nlay, FaceHost &F) {
+  const int ni = s->d.ni, nj = s->d.nj, nk = s->d.nk;
+  F.i1 = 0; F.i2 = ni - 1; F.j1 = 0; F.j2 = nj - 1; F.k1 = 0; F.k2 = nk - 1;
+  switch (f) {
+    case 0: F.i2 = nlay; break;
+    case 1: F.i1 = ni - 1 - nlay; break;
+    case 2: F.j2 = nlay; break;
+    case 3: F.j1 = nj - 1 - nlay; break;
+    case 4: F.k2 = nlay; break;
+    default: F.k1 = nk - 1 - nlay; break;
+  }
+  F.fsize = (long)(F.i2 - F.i1 + 1) * (F.j2 - F.j1 + 1) * (F.k2 - F.k1 + 1);
+}
+
+// copy host model values (any dtype) into a device buffer of the model type
+int put_model(cgfd_session *s, void *dst, const std::vector<double> &v) {
+  if (s->ms == 8) {
+    HIPCHK(hipMemcpy(dst, v.data(), v.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(v.size());
+    for (size_t i = 0; i < v.size(); i++) f[i] = (float)v[i];
+    HIPCHK(hipMemcpy(dst, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  }
+  return CGFD_OK;
+}
+
+// reference-layout component -> padded device-layout host buffer (double)
+void ref_to_dev_comp(const Layout &L, int dtype, const void *src, size_t comp, std::vector<double> &out,
+                     size_t out_base) {
+  const long nx = L.ni + 2 * kNG, ny = L.nj + 2 * kNG, nz = L.nk + 2 * kNG;
+  const size_t sc = (size_t)nx * ny * nz;
+  for (long k = 0; k < nz; k++)
+    for (long j = 0; j < ny; j++)
+      for (long i = 0; i < nx; i++)
+        out[out_base + k * L.pz + j * L.px + (i - kNG + L.xa)] = rd(dtype, src, comp * sc + (k * ny + j) * nx + i);
+}
+
+// storage-typed device write of n values given as double (scaled by inverse scale per component)
+int put_storage(cgfd_session *s, void *dst, const std::vector<double> &v, long comp_stride) {
+  const size_t n = v.size();
+  if (s->d.storage == CGFD_STORE_F64) {
+    HIPCHK(hipMemcpy(dst, v.data(), n * 8, hipMemcpyHostToDevice));
+  } else if (s->d.storage == CGFD_STORE_F32) {
+    std::vector<float> f(n);
+    for (size_t i = 0; i < n; i++) f[i] = (float)v[i];
+    HIPCHK(hipMemcpy(dst, f.data(), n * 4, hipMemcpyHostToDevice));
+  } else {
+    std::vector<uint16_t> h(n);
+    const bool bf = s->d.storage == CGFD_STORE_BF16;
+    for (size_t i = 0; i < n; i++) {
+      int c = comp_stride > 0 ? (int)(i / comp_stride) % 9 : 0;
+      float x = (float)v[i] * s->isc[c];
+      h[i] = bf ? f2bf(x) : f2h(x);
+    }
+    HIPCHK(hipMemcpy(dst, h.data(), n * 2, hipMemcpyHostToDevice));
+  }
+  return CGFD_OK;
+}
+
+int get_storage(cgfd_session *s, const void *src, size_t n, long comp_stride, std::vector<double> &v) {
+  v.resize(n);
+  if (s->d.storage == CGFD_STORE_F64) {
+    HIPCHK(hipMemcpy(v.data(), src, n * 8, hipMemcpyDeviceToHost));
+  } else if (s->d.storage == CGFD_STORE_F32) {
+    std::vector<float> f(n);
+    HIPCHK(hipMemcpy(f.data(), src, n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) v[i] = f[i];
+  } else {
+    std::vector<uint16_t> h(n);
+    HIPCHK(hipMemcpy(h.data(), src, n * 2, hipMemcpyDeviceToHost));
+    const bool bf = s->d.storage == CGFD_STORE_BF16;
+    for (size_t i = 0; i < n; i++) {
+      int c = comp_stride > 0 ? (int)(i / comp_stride) % 9 : 0;
+      v[i] = (double)(bf ? bf2f(h[i]) : h2f(h[i])) * s->sc[c];
+    }
+  }
+  return CGFD_OK;
+}
+
+void *wav_level(cgfd_session *s, int shot, int slot) {
+  return (char *)s->wav + ((size_t)shot * s->shot_elems + (size_t)slot * s->level_elems) * s->es;
+}
+
+int fill_zero(cgfd_session *s, void *p, size_t elems, size_t esz) {
+  hipError_t e;
+  if (esz == 8) e = launch_fill<double>((double *)p, (long)elems, 0.0, s->st);
+  else if (esz == 4) e = launch_fill<float>((float *)p, (long)elems, 0.0f, s->st);
+  else e = launch_fill<uint16_t>((uint16_t *)p, (long)elems, (uint16_t)0, s->st);
+  if (e != hipSuccess) return fail(CGFD_E_HIP, "fill kernel: %s", hipGetErrorString(e));
+  return CGFD_OK;
+}
+
+// ---- rebuild device source CSR over all shots ----
+template <typename TM> int build_sources(cgfd_session *s) {
+  for (void *p : {(void *)s->s_pt_shot, (void *)s->s_pt_beg, (void *)s->s_ct_src, (void *)s->s_itb,
+                  (void *)s->s_ite, (void *)s->s_kind, (void *)s->s_pt_off, (void *)s->s_tab, s->s_ct_coef, s->s_F,
+                  s->s_M}) {
+    if (p) { void *q = p; dfree(s, q); }
+  }
+  s->s_pt_shot = s->s_pt_beg = s->s_ct_src = s->s_itb = s->s_ite = s->s_kind = nullptr;
+  s->s_pt_off = s->s_tab = nullptr;
+  s->s_ct_coef = s->s_F = s->s_M = nullptr;
+  s->s_npt = 0;
+  std::vector<int> itb, ite, kind;
+  std::vector<long> tab;
+  std::vector<TM> F[3], M[6];
+  std::map<std::pair<int, long>, std::vector<std::pair<int, double>>> pts;
+  long ntab = 0;
+  int gsrc = 0;
+  for (int shot = 0; shot < (int)s->src.size(); shot++) {
+    const SrcHost &h = s->src[shot];
+    for (int is = 0; is < h.nsrc; is++, gsrc++) {
+      itb.push_back(h.itb[is]);
+      ite.push_back(h.ite[is]);
+      kind.push_back((h.force_on ? 1 : 0) | (h.moment_on ? 2 : 0));
+      tab.push_back(ntab);
+      const size_t per = (size_t)h.max_nt * 4;
+      for (int c = 0; c < 3; c++)
+        for (size_t q = 0; q < per; q++)
+          F[c].push_back(h.force_on ? (TM)h.F[(size_t)c * h.nsrc * per + (size_t)is * per + q] : TM(0));
+      for (int c = 0; c < 6; c++)
+        for (size_t q = 0; q < per; q++)
+          M[c].push_back(h.moment_on ? (TM)h.M[(size_t)c * h.nsrc * per + (size_t)is * per + q] : TM(0));
+      ntab += (long)per;
+      for (int e = 0; e < h.ext_num[is]; e++) {
+        long long ref = h.ext_indx[(size_t)is * h.max_ext + e];
+        long off = ref_to_dev(s->L, ref);
+        pts[{shot, off}].push_back({gsrc, h.ext_coef[(size_t)is * h.max_ext + e]});
+      }
+    }
+  }
+  const int npt = (int)pts.size();
+  s->s_npt = npt;
+  s->s_ntab = ntab;
+  if (npt == 0) { s->src_dirty = false; return CGFD_OK; }
+  std::vector<int> pt_shot, pt_beg, ct_src;
+  std::vector<long> pt_off;
+  std::vector<TM> ct_coef;
+  for (auto &kv : pts) {
+    pt_shot.push_back(kv.first.first);
+    pt_off.push_back(kv.first.second);
+    pt_beg.push_back((int)ct_src.size());
+    for (auto &c : kv.second) { ct_src.push_back(c.first); ct_coef.push_back((TM)c.second); }
+  }
+  pt_beg.push_back((int)ct_src.size());
+  auto up = [&](void **dst, const void *src, size_t bytes) -> int {
+    int r = dalloc(s, dst, bytes);
+    if (r) return r;
+    if (bytes) HIPCHK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return CGFD_OK;
+  };
+  int r;
+  if ((r = up((void **)&s->s_pt_shot, pt_shot.data(), pt_shot.size() * 4))) return r;
+  if ((r = up((void **)&s->s_pt_off, pt_off.data(), pt_off.size() * 8))) return r;
+  if ((r = up((void **)&s->s_pt_beg, pt_beg.data(), pt_beg.size() * 4))) return r;
+  if ((r = up((void **)&s->s_ct_src, ct_src.data(), ct_src.size() * 4))) return r;
+  if ((r = up(&s->s_ct_coef, ct_coef.data(), ct_coef.size() * sizeof(TM)))) return r;
+  if ((r = up((void **)&s->s_itb, itb.data(), itb.size() * 4))) return r;
+  if ((r = up((void **)&s->s_ite, ite.data(), ite.size() * 4))) return r;
+  if ((r = up((void **)&s->s_kind, kind.data(), kind.size() * 4))) return r;
+  if ((r = up((void **)&s->s_tab, tab.data(), tab.size() * 8))) return r;
+  std::vector<TM> Fall, Mall;
+  for (int c = 0; c < 3; c++) Fall.insert(Fall.end(), F[c].begin(), F[c].end());
+  for (int c = 0; c < 6; c++) Mall.insert(Mall.end(), M[c].begin(), M[c].end());
+  if ((r = up(&s->s_F, Fall.data(), Fall.size() * sizeof(TM)))) return r;
+  if ((r = up(&s->s_M, Mall.data(), Mall.size() * sizeof(TM)))) return r;
+  s->src_dirty = false;
+  return CGFD_OK;
+}
+
+template <typename TC> int build_receivers(cgfd_session *s) {
+  for (void *p : {(void *)s->r_shot, (void *)s->r_nt, (void *)s->r_off, (void *)s->r_out, s->r_frac, s->seismo}) {
+    if (p) { void *q = p; dfree(s, q); }
+  }
+  s->r_shot = s->r_nt = nullptr;
+  s->r_off = s->r_out = nullptr;
+  s->r_frac = s->seismo = nullptr;
+  std::vector<int> shot_of, nt_of;
+  std::vector<long> off, out;
+  std::vector<TC> frac;
+  long total = 0;
+  s->seismo_off.assign(s->rec.size(), 0);
+  for (int shot = 0; shot < (int)s->rec.size(); shot++) {
+    const RecvHost &h = s->rec[shot];
+    s->seismo_off[shot] = total;
+    for (int r = 0; r < h.nrecv; r++) {
+      shot_of.push_back(shot);
+      nt_of.push_back(h.nt);
+      for (int q = 0; q < 8; q++) off.push_back(ref_to_dev(s->L, h.indx8[(size_t)r * 8 + q]));
+      for (int q = 0; q < 3; q++) frac.push_back((TC)h.frac[(size_t)r * 3 + q]);
+      out.push_back(total);
+      total += (long)kNC * h.nt;
+    }
+  }
+  s->r_n = (int)shot_of.size();
+  s->seismo_elems = total;
+  if (s->r_n == 0) { s->rec_dirty = false; return CGFD_OK; }
+  auto up = [&](void **dst, const void *src, size_t bytes) -> int {
+    int r = dalloc(s, dst, bytes);
+    if (r) return r;
+    HIPCHK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return CGFD_OK;
+  };
+  int r;
+  if ((r = up((void **)&s->r_shot, shot_of.data(), shot_of.size() * 4))) return r;
+  if ((r = up((void **)&s->r_nt, nt_of.data(), nt_of.size() * 4))) return r;
+  if ((r = up((void **)&s->r_off, off.data(), off.size() * 8))) return r;
+  if ((r = up((void **)&s->r_out, out.data(), out.size() * 8))) return r;
+  if ((r = up(&s->r_frac, frac.data(), frac.size() * sizeof(TC)))) return r;
+  if ((r = dalloc(s, &s->seismo, (size_t)total * sizeof(TC)))) return r;
+  HIPCHK(hipMemset(s->seismo, 0, (size_t)total * sizeof(TC)));
+  s->rec_dirty = false;
+  return CGFD_OK;
+}
+
+// ---- stage argument assembly ----
+template <typename TS, typename TC>
+StageArgs<TS, TC> stage_args(cgfd_session *s, int slot_in, int slot_pre, int slot_out, int slot_end, bool rhs_mode) {
+  StageArgs<TS, TC> a{};
+  a.L = s->L;
+  a.met = (const TC *)s->met;
+  a.med = (const TC *)s->med;
+  a.mat = (const TC *)s->mat;
+  a.free_top = s->free_top;
+  const long aux_lvl_shot = 5;
+  for (int f = 0; f < 6; f++) {
+    FaceHost &F = s->face[f];
+    FaceArgs<TS, TC> &A = a.f[f];
+    A.on = F.on ? 1 : 0;
+    A.dim = f / 2;
+    if (!F.on) continue;
+    A.i1 = F.i1; A.i2 = F.i2; A.j1 = F.j1; A.j2 = F.j2; A.k1 = F.k1; A.k2 = F.k2;
+    A.bi = F.i2 - F.i1 + 1;
+    A.bj = F.j2 - F.j1 + 1;
+    A.fsize = F.fsize;
+    A.shot_stride = aux_lvl_shot * kNC * F.fsize;
+    const TC *cf = (const TC *)F.coef;
+    A.A = cf;
+    A.B = cf + (F.nlay + 1);
+    A.D = cf + 2 * (F.nlay + 1);
+    TS *base = (TS *)F.aux;
+    auto lv = [&](int sl) { return base + (long)sl * kNC * F.fsize; };
+    A.a_in = lv(slot_in);
+    A.a_pre = lv(slot_pre);
+    A.a_out = rhs_mode ? lv(4) : lv(slot_out);
+    A.a_end = lv(slot_end);
+  }
+  TS *w = (TS *)s->wav;
+  a.w_in = w + (long)slot_in * s->level_elems;
+  a.w_pre = w + (long)slot_pre * s->level_elems;
+  a.w_out = rhs_mode ? (TS *)s->rhs : w + (long)slot_out * s->level_elems;
+  a.w_end = w + (long)slot_end * s->level_elems;
+  a.shot_stride = rhs_mode ? 0 : s->shot_elems;
+  for (int c = 0; c < kNC; c++) { a.sc[c] = s->sc[c]; a.isc[c] = s->isc[c]; }
+  a.shot_active = s->shot_active;
+  return a;
+}
+
+struct Shell {
+  InteriorBox ib;
+  bool has_interior;
+  int nbox;
+  int box[6][6];
+  long box_end[6];
+};
+
+Shell make_shell(cgfd_session *s) {
+  Shell sh{};
+  const int ni = s->d.ni, nj = s->d.nj, nk = s->d.nk;
+  auto lay = [&](int f) { return s->face[f].on ? s->face[f].nlay + 1 : 0; };
+  int ib0 = lay(0), ib1 = ni - 1 - lay(1);
+  int jb0 = lay(2), jb1 = nj - 1 - lay(3);
+  int kb0 = lay(4), kb1 = nk - 1 - std::max(s->free_top ? 4 : 0, lay(5));
+  const bool plain = (s->d.flags & CGFD_FLAG_PLAIN) != 0;
+  sh.has_interior = !plain && ib1 >= ib0 && jb1 >= jb0 && kb1 >= kb0;
+  std::vector<std::array<int, 6>> boxes;
+  if (!sh.has_interior) {
+    boxes.push_back({0, ni - 1, 0, nj - 1, 0, nk - 1});
+  } else {
+    int zc = 48;
+    sh.ib = InteriorBox{ib0, ib1, jb0, jb1, kb0, kb1, zc};
+    auto add = [&](int a0, int a1, int b0, int b1, int c0, int c1) {
+      if (a1 >= a0 && b1 >= b0 && c1 >= c0) boxes.push_back({a0, a1, b0, b1, c0, c1});
+    };
+    add(0, ib0 - 1, 0, nj - 1, 0, nk - 1);
+    add(ib1 + 1, ni - 1, 0, nj - 1, 0, nk - 1);
+    add(ib0, ib1, 0, jb0 - 1, 0, nk - 1);
+    add(ib0, ib1, jb1 + 1, nj - 1, 0, nk - 1);
+    add(ib0, ib1, jb0, jb1, 0, kb0 - 1);
+    add(ib0, ib1, jb0, jb1, kb1 + 1, nk - 1);
+  }
+  sh.nbox = (int)boxes.size();
+  long acc = 0;
+  for (int b = 0; b < sh.nbox; b++) {
+    for (int q = 0; q < 6; q++) sh.box[b][q] = boxes[b][q];
+    acc += (long)(boxes[b][1] - boxes[b][0] + 1) * (boxes[b][3] - boxes[b][2] + 1) * (boxes[b][5] - boxes[b][4] + 1);
+    sh.box_end[b] = acc;
+  }
+  return sh;
+}
+
+template <typename TS, typename TC> int run_t(cgfd_session *s, int it0, int nsteps, double dt_in) {
+  if (s->src_dirty) { int r = build_sources<TC>(s); if (r) return r; }
+  if (s->rec_dirty) { int r = build_receivers<TC>(s); if (r) return r; }
+  const Shell sh = make_shell(s);
+  const TC dt = (TC)dt_in;
+  HIPCHK(hipMemcpyAsync(s->clock, &it0, sizeof(int), hipMemcpyHostToDevice, s->st));
+  const size_t nev_need = (size_t)(sh.has_interior && s->timing ? 2 * 4 * nsteps : 0);
+  if (s->timing && s->ev.size() < nev_need) {
+    size_t old = s->ev.size();
+    s->ev.resize(nev_need);
+    for (size_t e = old; e < nev_need; e++) HIPCHK(hipEventCreate(&s->ev[e]));
+  }
+  if (!s->ev_run0) { HIPCHK(hipEventCreate(&s->ev_run0)); HIPCHK(hipEventCreate(&s->ev_run1)); }
+  HIPCHK(hipEventRecord(s->ev_run0, s->st));
+  size_t ne = 0;
+  for (int n = 0; n < nsteps; n++) {
+    const int it = it0 + n;
+    // receivers read W^n before the step (drv_rk_curv_col.cu:195-216)
+    if (s->r_n > 0) {
+      RecvArgs<TS, TC> ra{};
+      ra.nrec = s->r_n; ra.r_shot = s->r_shot; ra.r_off = s->r_off; ra.r_frac = (const TC *)s->r_frac;
+      ra.r_out = s->r_out; ra.r_nt = s->r_nt; ra.seismo = (TC *)s->seismo;
+      ra.w_pre = (const TS *)s->wav + (long)s->slot[0] * s->level_elems;
+      ra.shot_stride = s->shot_elems; ra.cs = s->L.cs;
+      ra.clock = s->clock; ra.shot_start = s->shot_start; ra.shot_active = s->shot_active;
+      for (int c = 0; c < kNC; c++) ra.sc[c] = s->sc[c];
+      hipError_t e = launch_recv<TS, TC>(ra, s->st);
+      if (e != hipSuccess) return fail(CGFD_E_HIP, "receiver kernel: %s", hipGetErrorString(e));
+    }
+    const int ipair = it % 8;
+    const int PRE = s->slot[0], TA = s->slot[1], TB = s->slot[2], END = s->slot[3];
+    for (int istage = 0; istage < 4; istage++) {
+      const int dx = (kFlags[ipair][0] + istage) % 2;
+      const int dy = (kFlags[ipair][1] + istage) % 2;
+      const int dz = (kFlags[ipair][2] + istage) % 2;
+      int in, out, mode;
+      if (istage == 0) { in = PRE; out = TA; mode = MODE_S0; }
+      else if (istage == 1) { in = TA; out = TB; mode = MODE_S12; }
+      else if (istage == 2) { in = TB; out = TA; mode = MODE_S12; }
+      else { in = TA; out = TB; mode = MODE_S3; }
+      StageArgs<TS, TC> a = stage_args<TS, TC>(s, in, PRE, out, END, false);
+      a.mode = mode;
+      // coef = rk_a[istage]*dt, both float in the reference (drv_rk_curv_col.cu:341-342)
+      a.ca = (istage < 3) ? (TC)kRkA[istage] * dt : TC(0);
+      a.cb = (TC)kRkB[istage] * dt;
+      if (sh.has_interior) {
+        if (s->timing) HIPCHK(hipEventRecord(s->ev[ne++], s->st));
+        hipError_t e = launch_interior<TS, TC>(a, sh.ib, dx, dy, dz, s->d.nshots, s->st);
+        if (e != hipSuccess) return fail(CGFD_E_HIP, "interior kernel: %s", hipGetErrorString(e));
+        if (s->timing) HIPCHK(hipEventRecord(s->ev[ne++], s->st));
+      }
+      a.nbox = sh.nbox;
+      for (int b = 0; b < sh.nbox; b++) {
+        for (int q = 0; q < 6; q++) a.box[b][q] = sh.box[b][q];
+        a.box_end[b] = sh.box_end[b];
+      }
+      hipError_t e = launch_point<TS, TC>(a, dx, dy, dz, s->d.nshots, s->st);
+      if (e != hipSuccess) return fail(CGFD_E_HIP, "point kernel: %s", hipGetErrorString(e));
+      if (s->s_npt > 0) {
+        SrcArgs<TS, TC> sa{};
+        sa.L = s->L; sa.npt = s->s_npt; sa.pt_shot = s->s_pt_shot; sa.pt_off = s->s_pt_off; sa.pt_beg = s->s_pt_beg;
+        sa.ct_src = s->s_ct_src; sa.ct_coef = (const TC *)s->s_ct_coef; sa.s_itb = s->s_itb; sa.s_ite = s->s_ite;
+        sa.s_tab = s->s_tab; sa.s_kind = s->s_kind; sa.F = (const TC *)s->s_F; sa.M = (const TC *)s->s_M;
+        sa.ntab = s->s_ntab; sa.met = (const TC *)s->met; sa.med = (const TC *)s->med;
+        sa.w_out = a.w_out; sa.w_end = a.w_end; sa.shot_stride = s->shot_elems;
+        sa.mode = mode; sa.istage = istage; sa.ca = a.ca; sa.cb = a.cb;
+        sa.clock = s->clock; sa.shot_start = s->shot_start; sa.shot_active = s->shot_active;
+        for (int c = 0; c < kNC; c++) sa.isc[c] = s->isc[c];
+        e = launch_source<TS, TC>(sa, s->st);
+        if (e != hipSuccess) return fail(CGFD_E_HIP, "source kernel: %s", hipGetErrorString(e));
+      }
+    }
+    hipError_t e = launch_clock_advance(s->clock, s->st);
+    if (e != hipSuccess) return fail(CGFD_E_HIP, "clock kernel: %s", hipGetErrorString(e));
+    std::swap(s->slot[0], s->slot[3]);  // pre <-> end (drv_rk_curv_col.cu:529-541)
+  }
+  HIPCHK(hipEventRecord(s->ev_run1, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, s->ev_run0, s->ev_run1));
+  s->last_run_ms = ms;
+  double tot = 0;
+  for (size_t q = 0; q + 1 < ne; q += 2) {
+    float m = 0;
+    HIPCHK(hipEventElapsedTime(&m, s->ev[q], s->ev[q + 1]));
+    tot += m;
+  }
+  s->last_int_n = (int)(ne / 2);
+  s->last_int_avg_ms = s->last_int_n ? tot / s->last_int_n : 0.0;
+  return CGFD_OK;
+}
+
+template <typename TS, typename TC> int onestage_t(cgfd_session *s, int shot, int in_level, int ipair, int istage, int it) {
+  if (s->src_dirty) { int r = build_sources<TC>(s); if (r) return r; }
+  if (!s->rhs) {
+    int r = dalloc(s, &s->rhs, (size_t)s->level_elems * s->es);
+    if (r) return r;
+    r = fill_zero(s, s->rhs, s->level_elems, s->es);
+    if (r) return r;
+  }
+  const int dx = (kFlags[ipair][0] + istage) % 2;
+  const int dy = (kFlags[ipair][1] + istage) % 2;
+  const int dz = (kFlags[ipair][2] + istage) % 2;
+  const int slot_in = s->slot[in_level];
+  StageArgs<TS, TC> a = stage_args<TS, TC>(s, slot_in, s->slot[0], slot_in, s->slot[3], true);
+  // select the shot by offsetting the bases (RHS level holds a single shot)
+  const long so = (long)shot * s->shot_elems;
+  a.w_in += so; a.w_pre += so; a.w_end += so;
+  for (int f = 0; f < 6; f++) {
+    if (!a.f[f].on) continue;
+    const long fo = (long)shot * a.f[f].shot_stride;
+    a.f[f].a_in += fo; a.f[f].a_pre += fo; a.f[f].a_out += fo; a.f[f].a_end += fo;
+    a.f[f].shot_stride = 0;
+  }
+  a.shot_stride = 0;
+  a.shot_active = nullptr;
+  a.mode = MODE_RHS;
+  a.ca = TC(0); a.cb = TC(0);
+  const Shell sh = make_shell(s);
+  if (sh.has_interior) {
+    hipError_t e = launch_interior<TS, TC>(a, sh.ib, dx, dy, dz, 1, s->st);
+    if (e != hipSuccess) return fail(CGFD_E_HIP, "interior kernel: %s", hipGetErrorString(e));
+  }
+  a.nbox = sh.nbox;
+  for (int b = 0; b < sh.nbox; b++) {
+    for (int q = 0; q < 6; q++) a.box[b][q] = sh.box[b][q];
+    a.box_end[b] = sh.box_end[b];
+  }
+  hipError_t e = launch_point<TS, TC>(a, dx, dy, dz, 1, s->st);
+  if (e != hipSuccess) return fail(CGFD_E_HIP, "point kernel: %s", hipGetErrorString(e));
+  if (s->s_npt > 0) {
+    // the source kernel scans all shots' points; keep only this shot's via the active mask
+    std::vector<int> act(s->d.nshots, 0);
+    act[shot] = 1;
+    HIPCHK(hipMemcpyAsync(s->shot_active, act.data(), act.size() * 4, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(s->clock, &it, sizeof(int), hipMemcpyHostToDevice, s->st));
+    std::vector<int> zero(s->d.nshots, 0);
+    HIPCHK(hipMemcpyAsync(s->shot_start, zero.data(), zero.size() * 4, hipMemcpyHostToDevice, s->st));
+    SrcArgs<TS, TC> sa{};
+    sa.L = s->L; sa.npt = s->s_npt; sa.pt_shot = s->s_pt_shot; sa.pt_off = s->s_pt_off; sa.pt_beg = s->s_pt_beg;
+    sa.ct_src = s->s_ct_src; sa.ct_coef = (const TC *)s->s_ct_coef; sa.s_itb = s->s_itb; sa.s_ite = s->s_ite;
+    sa.s_tab = s->s_tab; sa.s_kind = s->s_kind; sa.F = (const TC *)s->s_F; sa.M = (const TC *)s->s_M;
+    sa.ntab = s->s_ntab; sa.met = (const TC *)s->met; sa.med = (const TC *)s->med;
+    sa.w_out = (TS *)s->rhs - (long)shot * 0; sa.w_end = (TS *)s->rhs;
+    sa.shot_stride = 0;
+    sa.mode = MODE_RHS; sa.istage = istage; sa.ca = TC(0); sa.cb = TC(0);
+    sa.clock = s->clock; sa.shot_start = s->shot_start; sa.shot_active = s->shot_active;
+    for (int c = 0; c < kNC; c++) sa.isc[c] = s->isc[c];
+    e = launch_source<TS, TC>(sa, s->st);
+    if (e != hipSuccess) return fail(CGFD_E_HIP, "source kernel: %s", hipGetErrorString(e));
+    std::vector<int> one(s->d.nshots, 1);
+    HIPCHK(hipMemcpyAsync(s->shot_active, one.data(), one.size() * 4, hipMemcpyHostToDevice, s->st));
+  }
+  HIPCHK(hipStreamSynchronize(s->st));
+  return CGFD_OK;
+}
+
+}  // namespace
+
+// =====================================================================================
+extern "C" {
+
+const char *cgfd_last_error(void) { return g_err.c_str(); }
+
+long long cgfd_footprint_bytes(const cgfd_desc *d, int nrecv, int nt_total) {
+  if (!d || d->ni <= 0 || d->nj <= 0 || d->nk <= 0) return -1;
+  size_t es = d->storage == CGFD_STORE_F64 ? 8 : d->storage == CGFD_STORE_F32 ? 4 : 2;
+  size_t ms = d->storage == CGFD_STORE_F64 ? 8 : 4;
+  Layout L = make_layout(d->ni, d->nj, d->nk, es);
+  Layout Lm = make_layout(d->ni, d->nj, d->nk, ms);
+  long long ns = d->nshots > 0 ? d->nshots : 1;
+  long long wav = ns * 4LL * kNC * L.cs * (long long)es;
+  long long model = (long long)(NMET + 3) * Lm.cs * (long long)ms + 18LL * d->ni * d->nj * (long long)ms;
+  // PML memory: 5 faces of 11 layers at most is bounded by 5 levels over the shell
+  long long shell = 2LL * 11 * d->nj * d->nk + 2LL * 11 * d->ni * d->nk + 2LL * 11 * d->ni * d->nj;
+  long long aux = ns * 5LL * kNC * shell * (long long)es;
+  long long rec = ns * (long long)nrecv * kNC * (long long)(nt_total > 0 ? nt_total : 0) * (long long)ms;
+  return wav + model + aux + rec;
+}
+
+int cgfd_create(const cgfd_desc *d, cgfd_session **out) {
+  if (!d || !out) return fail(CGFD_E_ARG, "null descriptor or output pointer");
+  *out = nullptr;
+  if (d->ni < 1 || d->nj < 1 || d->nk < 1) return fail(CGFD_E_ARG, "grid %d x %d x %d invalid", d->ni, d->nj, d->nk);
+  if (d->storage < 0 || d->storage > 3) return fail(CGFD_E_ARG, "storage %d invalid", d->storage);
+  if (d->nshots < 1) return fail(CGFD_E_ARG, "nshots %d invalid", d->nshots);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    (void)hipGetLastError();
+    return fail(CGFD_E_NODEV, "no HIP device available");
+  }
+  cgfd_session *s = new cgfd_session();
+  s->d = *d;
+  HIPCHK(hipGetDevice(&s->dev));
+  s->es = d->storage == CGFD_STORE_F64 ? 8 : d->storage == CGFD_STORE_F32 ? 4 : 2;
+  s->ms = d->storage == CGFD_STORE_F64 ? 8 : 4;
+  // wavefield and model share one geometry; aligning i = 0 to 128 bytes in
+  // storage elements also aligns the (equal or wider) model elements
+  s->L = make_layout(d->ni, d->nj, d->nk, s->es);
+  s->level_elems = (long)kNC * s->L.cs;
+  s->shot_elems = 4 * s->level_elems;
+  for (int c = 0; c < 9; c++) { s->sc[c] = 1.0f; s->isc[c] = 1.0f; }
+  auto cleanup = [&](int r) {
+    cgfd_destroy(s);
+    return r;
+  };
+  if (hipStreamCreate(&s->st) != hipSuccess) return cleanup(fail(CGFD_E_HIP, "hipStreamCreate failed"));
+  int r;
+  if ((r = dalloc(s, &s->wav, (size_t)d->nshots * s->shot_elems * s->es))) return cleanup(r);
+  if ((r = dalloc(s, &s->met, (size_t)NMET * s->L.cs * s->ms))) return cleanup(r);
+  if ((r = dalloc(s, &s->med, (size_t)3 * s->L.cs * s->ms))) return cleanup(r);
+  if ((r = dalloc(s, &s->mat, (size_t)18 * d->ni * d->nj * s->ms))) return cleanup(r);
+  if ((r = dalloc(s, (void **)&s->clock, sizeof(int)))) return cleanup(r);
+  if ((r = dalloc(s, (void **)&s->shot_start, sizeof(int) * d->nshots))) return cleanup(r);
+  if ((r = dalloc(s, (void **)&s->shot_active, sizeof(int) * d->nshots))) return cleanup(r);
+  if ((r = fill_zero(s, s->wav, (size_t)d->nshots * s->shot_elems, s->es))) return cleanup(r);
+  if ((r = fill_zero(s, s->met, (size_t)NMET * s->L.cs, s->ms))) return cleanup(r);
+  if ((r = fill_zero(s, s->med, (size_t)3 * s->L.cs, s->ms))) return cleanup(r);
+  if ((r = fill_zero(s, s->mat, (size_t)18 * d->ni * d->nj, s->ms))) return cleanup(r);
+  std::vector<int> zero(d->nshots, 0), one(d->nshots, 1);
+  if (hipMemcpy(s->shot_start, zero.data(), zero.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->shot_active, one.data(), one.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->clock, zero.data(), 4, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(CGFD_E_HIP, "initial copies failed"));
+  s->src.resize(d->nshots);
+  s->rec.resize(d->nshots);
+  if (hipStreamSynchronize(s->st) != hipSuccess) return cleanup(fail(CGFD_E_HIP, "stream sync failed"));
+  *out = s;
+  return CGFD_OK;
+}
+
+void cgfd_destroy(cgfd_session *s) {
+  if (!s) return;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  for (void *p : s->allocs) (void)hipFree(p);
+  s->allocs.clear();
+  for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+  if (s->ev_run0) (void)hipEventDestroy(s->ev_run0);
+  if (s->ev_run1) (void)hipEventDestroy(s->ev_run1);
+  if (s->st) (void)hipStreamDestroy(s->st);
+  delete s;
+}
+
+int cgfd_upload_metric(cgfd_session *s, int dtype, const void *metric) {
+  if (!s || !metric) return fail(CGFD_E_ARG, "null argument");
+  std::vector<double> v((size_t)NMET * s->L.cs, 0.0);
+  for (int c = 0; c < NMET; c++) ref_to_dev_comp(s->L, dtype, metric, c, v, (size_t)c * s->L.cs);
+  int r = put_model(s, s->met, v);
+  if (r) return r;
+  s->have_met = true;
+  return CGFD_OK;
+}
+
+int cgfd_upload_medium(cgfd_session *s, int dtype, const void *lam, const void *mu, const void *slw) {
+  if (!s || !lam || !mu || !slw) return fail(CGFD_E_ARG, "null argument");
+  std::vector<double> v((size_t)3 * s->L.cs, 0.0);
+  ref_to_dev_comp(s->L, dtype, lam, 0, v, 0);
+  ref_to_dev_comp(s->L, dtype, mu, 0, v, s->L.cs);
+  ref_to_dev_comp(s->L, dtype, slw, 0, v, 2 * s->L.cs);
+  int r = put_model(s, s->med, v);
+  if (r) return r;
+  s->have_med = true;
+  return CGFD_OK;
+}
+
+int cgfd_upload_free_surface(cgfd_session *s, int free_top, int dtype, const void *mvx, const void *mvy) {
+  if (!s) return fail(CGFD_E_ARG, "null session");
+  s->free_top = free_top ? 1 : 0;
+  if (!free_top) return CGFD_OK;
+  if (!mvx || !mvy) return fail(CGFD_E_ARG, "free surface enabled without matrices");
+  const int ni = s->d.ni, nj = s->d.nj;
+  const long nx = ni + 2 * kNG;
+  const size_t pl = (size_t)ni * nj;
+  std::vector<double> v(18 * pl);
+  for (int j = 0; j < nj; j++)
+    for (int i = 0; i < ni; i++) {
+      size_t ij = (size_t)((j + kNG) * nx + (i + kNG)) * 9;
+      for (int q = 0; q < 9; q++) {
+        v[(size_t)q * pl + (size_t)j * ni + i] = rd(dtype, mvx, ij + q);
+        v[(size_t)(9 + q) * pl + (size_t)j * ni + i] = rd(dtype, mvy, ij + q);
+      }
+    }
+  return put_model(s, s->mat, v);
+}
+
+int cgfd_upload_pml(cgfd_session *s, int face, int nlay, int dtype, const void *A, const void *B, const void *D) {
+  if (!s || face < 0 || face > 5) return fail(CGFD_E_ARG, "bad session or face %d", face);
+  FaceHost &F = s->face[face];
+  if (F.coef) dfree(s, F.coef);
+  if (F.aux) dfree(s, F.aux);
+  F.on = false;
+  F.nlay = 0;
+  if (nlay <= 0) return CGFD_OK;
+  const int axis_n = face < 2 ? s->d.ni : face < 4 ? s->d.nj : s->d.nk;
+  if (nlay + 1 > axis_n) return fail(CGFD_E_ARG, "face %d: %d layers exceed the grid", face, nlay);
+  if (!A || !B || !D) return fail(CGFD_E_ARG, "face %d: null profile", face);
+  face_box(s, face, nlay, F);
+  F.nlay = nlay;
+  std::vector<double> v(3 * (size_t)(nlay + 1));
+  for (int n = 0; n <= nlay; n++) {
+    v[n] = rd(dtype, A, n);
+    v[(nlay + 1) + n] = rd(dtype, B, n);
+    v[2 * (nlay + 1) + n] = rd(dtype, D, n);
+  }
+  int r;
+  if ((r = dalloc(s, &F.coef, v.size() * s->ms))) return r;
+  if ((r = put_model(s, F.coef, v))) return r;
+  const size_t aux_elems = (size_t)s->d.nshots * 5 * kNC * F.fsize;
+  if ((r = dalloc(s, &F.aux, aux_elems * s->es))) return r;
+  if ((r = fill_zero(s, F.aux, aux_elems, s->es))) return r;
+  HIPCHK(hipStreamSynchronize(s->st));
+  F.on = true;
+  return CGFD_OK;
+}
+
+int cgfd_set_storage_scale(cgfd_session *s, const int exp2[9]) {
+  if (!s || !exp2) return fail(CGFD_E_ARG, "null argument");
+  if (s->d.storage != CGFD_STORE_F16 && s->d.storage != CGFD_STORE_BF16)
+    return fail(CGFD_E_ARG, "storage scales apply to F16/BF16 sessions only");
+  for (int c = 0; c < 9; c++) {
+    if (exp2[c] < -126 || exp2[c] > 127) return fail(CGFD_E_ARG, "scale exponent %d out of range", exp2[c]);
+    s->sc[c] = std::ldexp(1.0f, exp2[c]);
+    s->isc[c] = std::ldexp(1.0f, -exp2[c]);
+  }
+  return CGFD_OK;
+}
+
+int cgfd_upload_source(cgfd_session *s, int shot, const cgfd_source_desc *src) {
+  if (!s || shot < 0 || shot >= s->d.nshots) return fail(CGFD_E_ARG, "bad session or shot %d", shot);
+  SrcHost h;
+  if (src && src->nsrc > 0) {
+    if (src->max_ext < 1 || src->max_ext > 343 || src->max_nt < 1)
+      return fail(CGFD_E_ARG, "source tables: max_ext %d max_nt %d", src->max_ext, src->max_nt);
+    if (!src->it_begin || !src->it_end || !src->ext_num || !src->ext_indx || !src->ext_coef)
+      return fail(CGFD_E_ARG, "source tables: null array");
+    if ((src->force_on && !src->F) || (src->moment_on && !src->M))
+      return fail(CGFD_E_ARG, "source tables: missing F or M");
+    h.nsrc = src->nsrc; h.force_on = src->force_on; h.moment_on = src->moment_on;
+    h.max_nt = src->max_nt; h.max_ext = src->max_ext;
+    h.itb.assign(src->it_begin, src->it_begin + h.nsrc);
+    h.ite.assign(src->it_end, src->it_end + h.nsrc);
+    h.ext_num.assign(src->ext_num, src->ext_num + h.nsrc);
+    const size_t ne = (size_t)h.nsrc * h.max_ext;
+    h.ext_indx.assign(src->ext_indx, src->ext_indx + ne);
+    const long long ntot = (long long)(s->d.ni + 6) * (s->d.nj + 6) * (s->d.nk + 6);
+    for (int is = 0; is < h.nsrc; is++) {
+      if (h.ext_num[is] < 0 || h.ext_num[is] > h.max_ext) return fail(CGFD_E_ARG, "source %d: ext_num", is);
+      if (h.ite[is] - h.itb[is] + 1 > h.max_nt || h.ite[is] < h.itb[is])
+        return fail(CGFD_E_ARG, "source %d: steps %d..%d exceed its %d-sample table", is, h.itb[is], h.ite[is], h.max_nt);
+      for (int e = 0; e < h.ext_num[is]; e++) {
+        long long v = h.ext_indx[(size_t)is * h.max_ext + e];
+        if (v < 0 || v >= ntot) return fail(CGFD_E_ARG, "source %d: index %lld outside the grid", is, v);
+      }
+    }
+    h.ext_coef.resize(ne);
+    for (size_t q = 0; q < ne; q++) h.ext_coef[q] = rd(src->dtype, src->ext_coef, q);
+    const size_t nt4 = (size_t)h.nsrc * h.max_nt * 4;
+    if (h.force_on) { h.F.resize(3 * nt4); for (size_t q = 0; q < 3 * nt4; q++) h.F[q] = rd(src->dtype, src->F, q); }
+    if (h.moment_on) { h.M.resize(6 * nt4); for (size_t q = 0; q < 6 * nt4; q++) h.M[q] = rd(src->dtype, src->M, q); }
+  }
+  s->src[shot] = std::move(h);
+  s->src_dirty = true;
+  return CGFD_OK;
+}
+
+int cgfd_upload_receivers(cgfd_session *s, int shot, int nrecv, const long long *indx8, int dtype, const void *frac3,
+                          int nt_total) {
+  if (!s || shot < 0 || shot >= s->d.nshots) return fail(CGFD_E_ARG, "bad session or shot %d", shot);
+  if (nrecv < 0 || (nrecv > 0 && (!indx8 || !frac3 || nt_total < 1))) return fail(CGFD_E_ARG, "receiver arguments");
+  RecvHost h;
+  h.nrecv = nrecv;
+  h.nt = nt_total;
+  h.indx8.assign(indx8, indx8 + (size_t)nrecv * 8);
+  const long long ntot = (long long)(s->d.ni + 6) * (s->d.nj + 6) * (s->d.nk + 6);
+  for (long long v : h.indx8)
+    if (v < 0 || v >= ntot) return fail(CGFD_E_ARG, "receiver index %lld outside the grid", v);
+  h.frac.resize((size_t)nrecv * 3);
+  for (size_t q = 0; q < h.frac.size(); q++) h.frac[q] = rd(dtype, frac3, q);
+  s->rec[shot] = std::move(h);
+  s->rec_dirty = true;
+  return CGFD_OK;
+}
+
+int cgfd_set_level(cgfd_session *s, int shot, int level, int dtype, const void *w9) {
+  if (!s || shot < 0 || shot >= s->d.nshots || level < 0 || level > 3 || !w9)
+    return fail(CGFD_E_ARG, "bad set_level arguments");
+  std::vector<double> v((size_t)s->level_elems, 0.0);
+  for (int c = 0; c < kNC; c++) ref_to_dev_comp(s->L, dtype, w9, c, v, (size_t)c * s->L.cs);
+  return put_storage(s, wav_level(s, shot, s->slot[level]), v, s->L.cs);
+}
+
+int cgfd_reset_shot(cgfd_session *s, int shot) {
+  if (!s || shot < 0 || shot >= s->d.nshots) return fail(CGFD_E_ARG, "bad shot");
+  int r = fill_zero(s, wav_level(s, shot, 0), (size_t)s->shot_elems, s->es);
+  if (r) return r;
+  for (int f = 0; f < 6; f++) {
+    FaceHost &F = s->face[f];
+    if (!F.on) continue;
+    r = fill_zero(s, (char *)F.aux + (size_t)shot * 5 * kNC * F.fsize * s->es, (size_t)5 * kNC * F.fsize, s->es);
+    if (r) return r;
+  }
+  HIPCHK(hipStreamSynchronize(s->st));
+  return CGFD_OK;
+}
+
+int cgfd_run(cgfd_session *s, int it0, int nsteps, double dt) {
+  if (!s) return fail(CGFD_E_ARG, "null session");
+  if (!s->have_met || !s->have_med) return fail(CGFD_E_STATE, "metric and medium must be uploaded before running");
+  if (nsteps < 0 || it0 < 0) return fail(CGFD_E_ARG, "bad step range");
+  if (!(dt > 0.0)) return fail(CGFD_E_ARG, "dt must be positive");
+  switch (s->d.storage) {
+    case CGFD_STORE_F32: return run_t<float, float>(s, it0, nsteps, dt);
+    case CGFD_STORE_F64: return run_t<double, double>(s, it0, nsteps, dt);
+    case CGFD_STORE_F16: return run_t<__half, float>(s, it0, nsteps, dt);
+    default: return run_t<__hip_bfloat16, float>(s, it0, nsteps, dt);
+  }
+}
+
+int cgfd_el_iso_onestage(cgfd_session *s, int shot, int in_level, int ipair, int istage, int it) {
+  if (!s || shot < 0 || shot >= s->d.nshots) return fail(CGFD_E_ARG, "bad session or shot");
+  if (in_level < 0 || in_level > 3 || ipair < 0 || ipair > 7 || istage < 0 || istage > 3)
+    return fail(CGFD_E_ARG, "bad level/pair/stage");
+  if (!s->have_met || !s->have_med) return fail(CGFD_E_STATE, "metric and medium must be uploaded first");
+  switch (s->d.storage) {
+    case CGFD_STORE_F32: return onestage_t<float, float>(s, shot, in_level, ipair, istage, it);
+    case CGFD_STORE_F64: return onestage_t<double, double>(s, shot, in_level, ipair, istage, it);
+    case CGFD_STORE_F16: return onestage_t<__half, float>(s, shot, in_level, ipair, istage, it);
+    default: return onestage_t<__hip_bfloat16, float>(s, shot, in_level, ipair, istage, it);
+  }
+}
+
+int cgfd_read_seismo(cgfd_session *s, int shot, int dtype, void *out) {
+  if (!s || shot < 0 || shot >= s->d.nshots || !out) return fail(CGFD_E_ARG, "bad read_seismo arguments");
+  if (s->rec_dirty) return fail(CGFD_E_STATE, "receivers uploaded but not yet run");
+  const RecvHost &h = s->rec[shot];
+  const size_t n = (size_t)h.nrecv * kNC * h.nt;
+  if (n == 0) return CGFD_OK;
+  const size_t off = (size_t)s->seismo_off[shot];
+  if (s->ms == 8) {
+    std::vector<double> v(n);
+    HIPCHK(hipMemcpy(v.data(), (double *)s->seismo + off, n * 8, hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < n; q++) {
+      if (dtype == CGFD_DT_F64) ((double *)out)[q] = v[q];
+      else ((float *)out)[q] = (float)v[q];
+    }
+  } else {
+    std::vector<float> v(n);
+    HIPCHK(hipMemcpy(v.data(), (float *)s->seismo + off, n * 4, hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < n; q++) {
+      if (dtype == CGFD_DT_F64) ((double *)out)[q] = v[q];
+      else ((float *)out)[q] = v[q];
+    }
+  }
+  return CGFD_OK;
+}
+
+int cgfd_fetch_level(cgfd_session *s, int shot, int level, int dtype, void *w9) {
+  if (!s || shot < 0 || shot >= s->d.nshots || level < 0 || level > 4 || !w9)
+    return fail(CGFD_E_ARG, "bad fetch_level arguments");
+  const void *src;
+  if (level == CGFD_LEVEL_RHS) {
+    if (!s->rhs) return fail(CGFD_E_STATE, "no RHS level: call cgfd_el_iso_onestage first");
+    src = s->rhs;
+  } else {
+    src = wav_level(s, shot, s->slot[level]);
+  }
+  HIPCHK(hipStreamSynchronize(s->st));
+  std::vector<double> v;
+  int r = get_storage(s, src, (size_t)s->level_elems, s->L.cs, v);
+  if (r) return r;
+  const long nx = s->d.ni + 6, ny = s->d.nj + 6, nz = s->d.nk + 6;
+  const size_t sc = (size_t)nx * ny * nz;
+  for (int c = 0; c < kNC; c++)
+    for (long k = 0; k < nz; k++)
+      for (long j = 0; j < ny; j++)
+        for (long i = 0; i < nx; i++) {
+          double x = v[(size_t)c * s->L.cs + k * s->L.pz + j * s->L.px + (i - kNG + s->L.xa)];
+          size_t q = (size_t)c * sc + (k * ny + j) * nx + i;
+          if (dtype == CGFD_DT_F64) ((double *)w9)[q] = x;
+          else ((float *)w9)[q] = (float)x;
+        }
+  return CGFD_OK;
+}
+
+int cgfd_fetch_pml_aux(cgfd_session *s, int shot, int face, int level, int dtype, void *out) {
+  if (!s || shot < 0 || shot >= s->d.nshots || face < 0 || face > 5 || level < 0 || level > 4 || !out)
+    return fail(CGFD_E_ARG, "bad fetch_pml_aux arguments");
+  FaceHost &F = s->face[face];
+  if (!F.on) return fail(CGFD_E_STATE, "face %d has no PML", face);
+  const int slot = level == CGFD_LEVEL_RHS ? 4 : s->slot[level];
+  const char *src = (const char *)F.aux + ((size_t)shot * 5 + slot) * kNC * F.fsize * s->es;
+  HIPCHK(hipStreamSynchronize(s->st));
+  std::vector<double> v;
+  int r = get_storage(s, src, (size_t)kNC * F.fsize, F.fsize, v);
+  if (r) return r;
+  for (size_t q = 0; q < v.size(); q++) {
+    if (dtype == CGFD_DT_F64) ((double *)out)[q] = v[q];
+    else ((float *)out)[q] = (float)v[q];
+  }
+  return CGFD_OK;
+}
+
+int cgfd_set_timing(cgfd_session *s, int on) {
+  if (!s) return fail(CGFD_E_ARG, "null session");
+  s->timing = on != 0;
+  return CGFD_OK;
+}
+
+int cgfd_last_run_timing(cgfd_session *s, double *run_ms, double *interior_avg_ms, int *interior_launches) {
+  if (!s) return fail(CGFD_E_ARG, "null session");
+  if (run_ms) *run_ms = s->last_run_ms;
+  if (interior_avg_ms) *interior_avg_ms = s->last_int_avg_ms;
+  if (interior_launches) *interior_launches = s->last_int_n;
+  return CGFD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,144 @@
+// el_iso_device.h -- device-side building blocks of the gfx950 el_iso path:
+// storage conversion, MacCormack/DRP operator constants, the padded HBM
+// layout and the by-value argument blocks of the stage kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <cstdint>
+#include <type_traits>
+
+namespace cgfd {
+
+constexpr int kNG = 3;    // ghost points per side (src/forward/fd_t.cu:64-66)
+constexpr int kNC = 9;    // wavefield components
+enum Cmp { VX = 0, VY, VZ, TXX, TYY, TZZ, TYZ, TXZ, TXY };
+enum Met { JAC = 0, XIX, XIY, XIZ, ETX, ETY, ETZ, ZTX, ZTY, ZTZ, NMET };
+enum Mode { MODE_RHS = 0, MODE_S0 = 1, MODE_S12 = 2, MODE_S3 = 3 };
+
+// ---- storage traits ---------------------------------------------------------
+template <typename TS> struct St;
+template <> struct St<float> {
+  static constexpr bool scaled = false;
+  __device__ __forceinline__ static float ld(const float *p) { return *p; }
+  __device__ __forceinline__ static void st(float *p, float v) { *p = v; }
+};
+template <> struct St<double> {
+  static constexpr bool scaled = false;
+  __device__ __forceinline__ static double ld(const double *p) { return *p; }
+  __device__ __forceinline__ static void st(double *p, double v) { *p = v; }
+};
+template <> struct St<__half> {
+  static constexpr bool scaled = true;
+  __device__ __forceinline__ static float ld(const __half *p) { return __half2float(*p); }
+  __device__ __forceinline__ static void st(__half *p, float v) { *p = __float2half(v); }
+};
+template <> struct St<__hip_bfloat16> {
+  static constexpr bool scaled = true;
+  __device__ __forceinline__ static float ld(const __hip_bfloat16 *p) { return __bfloat162float(*p); }
+  __device__ __forceinline__ static void st(__hip_bfloat16 *p, float v) { *p = __float2bfloat16(v); }
+};
+
+// ---- operator constants (src/forward/fd_t.cu:76-148) ----------------------
+// DIR 0 = backward {-3,-2,-1,0,1}, DIR 1 = forward {-1,0,1,2,3}.
+template <typename TC, int DIR> struct Drp;
+template <typename TC> struct Drp<TC, 0> {
+  static constexpr int lo = -3;
+  __device__ __forceinline__ static TC c(int n) {
+    return n == 0 ? TC(-0.041686285) : n == 1 ? TC(0.333411808) : n == 2 ? TC(-1.250117713)
+         : n == 3 ? TC(0.666745142) : TC(0.291647047);
+  }
+};
+template <typename TC> struct Drp<TC, 1> {
+  static constexpr int lo = -1;
+  __device__ __forceinline__ static TC c(int n) {
+    return n == 0 ? TC(-0.291647047) : n == 1 ? TC(-0.666745142) : n == 2 ? TC(1.250117713)
+         : n == 3 ? TC(-0.333411808) : TC(0.041686285);
+  }
+};
+
+// ---- padded HBM layout --------------------------------------------------------
+// Physical point (i,j,k), 0-based, lives at (k+3)*pz + (j+3)*px + (i+xa) within a
+// component; xa >= 3 aligns i = 0 to 128 bytes, px is the padded row pitch.
+struct Layout {
+  int ni, nj, nk;
+  int xa;
+  long px, pz, cs;
+  __host__ __device__ __forceinline__ long off(int i, int j, int k) const {
+    return (long)(k + kNG) * pz + (long)(j + kNG) * px + (long)(i + xa);
+  }
+};
+
+// ---- kernel argument blocks ------------------------------------------------------
+template <typename TS, typename TM> struct FaceArgs {
+  int on, dim;
+  int i1, i2, j1, j2, k1, k2;   // physical inclusive box
+  int bi, bj;                   // box extents along i and j
+  long fsize;                   // points in the box
+  long shot_stride;             // elements between shots of this face's buffer
+  const TM *A, *B, *D;          // nlay+1 profile values
+  TS *a_in, *a_pre, *a_out, *a_end;  // shot-0 level bases, component stride fsize
+};
+
+template <typename TS, typename TM> struct StageArgs {
+  Layout L;
+  const TM *met;   // NMET components, stride L.cs
+  const TM *med;   // lam, mu, 1/rho, stride L.cs
+  const TM *mat;   // 18 components [c][j][i] on the physical top plane
+  int free_top;
+  FaceArgs<TS, TM> f[6];
+  TS *w_in, *w_pre, *w_out, *w_end;  // shot-0 level bases
+  long shot_stride;
+  int mode;
+  TM ca, cb;
+  float sc[kNC], isc[kNC];  // storage scale (value = stored*sc) and its inverse
+  int nbox;
+  int box[6][6];            // i1,i2,j1,j2,k1,k2 inclusive, physical
+  long box_end[6];          // cumulative point counts
+  const int *shot_active;   // per-shot 0/1, or null
+};
+
+// Source contributions grouped by grid point (CSR), all shots of a session.
+template <typename TS, typename TM> struct SrcArgs {
+  Layout L;
+  int npt;
+  const int *pt_shot;       // [npt]
+  const long *pt_off;       // [npt] layout offset
+  const int *pt_beg;        // [npt+1] into contributions
+  const int *ct_src;        // [nct] global source id
+  const TM *ct_coef;        // [nct]
+  const int *s_itb, *s_ite; // [nsrc_total]
+  const long *s_tab;        // [nsrc_total] offset of the source's (it,stage) table
+  const int *s_kind;        // [nsrc_total] bit0 force, bit1 moment
+  const TM *F;              // [3][ntab]
+  const TM *M;              // [6][ntab]
+  long ntab;
+  const TM *met;            // for jac
+  const TM *med;            // for 1/rho
+  TS *w_out, *w_end;
+  long shot_stride;
+  int mode, istage;
+  TM ca, cb;
+  const int *clock;         // current global step
+  const int *shot_start;    // per-shot start step
+  const int *shot_active;
+  float isc[kNC];
+};
+
+template <typename TS, typename TC> struct RecvArgs {
+  int nrec;                 // receivers over all shots
+  const int *r_shot;
+  const long *r_off;        // [nrec][8] layout offsets
+  const TC *r_frac;         // [nrec][3]
+  const long *r_out;        // [nrec] offset of the receiver's [9][nt] block in seismo
+  const int *r_nt;          // [nrec] samples kept
+  TC *seismo;
+  const TS *w_pre;
+  long shot_stride, cs;
+  const int *clock;
+  const int *shot_start;
+  const int *shot_active;
+  float sc[kNC];
+};
+
+}  // namespace cgfd

@@ -1,0 +1,416 @@
+// el_iso_point.hip -- per-point stage kernel of the el_iso RK4 path.
+//
+// One thread updates one grid point of one shot: 27 one-sided DRP
+// derivatives, the momentum/Hooke right-hand side (inner form, or the
+// traction-image / vlow forms on the free-surface rows), the CFS-PML ADE
+// corrections of every face box containing the point, then the fused RK
+// update of the wavefield and of the PML memory.  It covers any list of boxes,
+// so it serves both as the boundary-shell kernel next to the z-marching
+// interior kernel and as the whole-domain kernel of the plain method.
+//
+// Semantics follow, in order, sv_curv_col_el_iso_rhs_inner_gpu
+// (src/forward/sv_curv_col_el_iso_gpu.cu:291-452),
+// sv_curv_col_el_iso_rhs_timg_z2_gpu (src/forward/sv_curv_col_el_gpu.cu:19-221),
+// sv_curv_col_el_iso_rhs_vlow_z2_gpu (sv_curv_col_el_iso_gpu.cu:462-603),
+// sv_curv_col_el_iso_rhs_cfspml_gpu (sv_curv_col_el_iso_gpu.cu:680-1147) and the
+// RK axpys wav_update / wav_update_end (src/forward/wav_t.cu:362-378) as driven by
+// drv_rk_curv_col.cu:339-491.
+#include "el_iso_device.h"
+#include "el_iso_launch.h"
+
+namespace cgfd {
+
+template <int DIR, typename TC, typename F>
+__device__ __forceinline__ TC drp5(F f, long s) {
+  constexpr int lo = Drp<TC, DIR>::lo;
+  return Drp<TC, DIR>::c(0) * f((lo + 0) * s) + Drp<TC, DIR>::c(1) * f((lo + 1) * s) +
+         Drp<TC, DIR>::c(2) * f((lo + 2) * s) + Drp<TC, DIR>::c(3) * f((lo + 3) * s) +
+         Drp<TC, DIR>::c(4) * f((lo + 4) * s);
+}
+
+template <int DX, int DY, int DZ, typename TS, typename TC>
+__device__ __forceinline__ void point_update(const StageArgs<TS, TC> &a, int shot, int i, int j, int k) {
+  const Layout &L = a.L;
+  const long o = L.off(i, j, k);
+  const long so = (long)shot * a.shot_stride;
+  const TS *win = a.w_in + so;
+  const long cs = L.cs;
+  auto W = [&](int c, long d) -> TC { return (TC)St<TS>::ld(win + c * cs + o + d); };
+  auto S = [&](int c) -> TC { return St<TS>::scaled ? (TC)a.sc[c] : TC(1); };
+
+  TC Dx[kNC], Dy[kNC], Dz[kNC];
+#pragma unroll
+  for (int c = 0; c < kNC; c++) {
+    auto f = [&](long d) { return W(c, d); };
+    Dx[c] = drp5<DX, TC>(f, 1);
+    Dy[c] = drp5<DY, TC>(f, L.px);
+    Dz[c] = drp5<DZ, TC>(f, L.pz);
+    if (St<TS>::scaled) { Dx[c] *= S(c); Dy[c] *= S(c); Dz[c] *= S(c); }
+  }
+  const TC *met = a.met;
+  auto Mt = [&](int m, long d) -> TC { return met[m * cs + o + d]; };
+  const TC xix = Mt(XIX, 0), xiy = Mt(XIY, 0), xiz = Mt(XIZ, 0);
+  const TC etx = Mt(ETX, 0), ety = Mt(ETY, 0), etz = Mt(ETZ, 0);
+  const TC ztx = Mt(ZTX, 0), zty = Mt(ZTY, 0), ztz = Mt(ZTZ, 0);
+  const TC lam = a.med[o], mu = a.med[cs + o], slw = a.med[2 * cs + o];
+  // lam + 2.0*mu: the double-promoted sum of the reference rounds to the same
+  // float as the float sum whenever lam and 2mu are within 2^29 of each other.
+  const TC lam2mu = lam + TC(2) * mu;
+
+  TC h[kNC];
+  const bool top = a.free_top != 0;
+  const int ktop = L.nk - 1;
+  constexpr int zlast = (DZ == 0) ? 1 : 3;
+  if (top && k >= ktop - zlast) {
+    // traction image: conservative form with the odd extension of J*zeta.T
+    constexpr int zlo = (DZ == 0) ? -3 : -1;
+    const int n_free = ktop - k - zlo;
+    const TC slwjac = slw / Mt(JAC, 0);
+    constexpr int T1[3] = {TXX, TXY, TXZ}, T2[3] = {TXY, TYY, TYZ}, T3[3] = {TXZ, TYZ, TZZ};
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) {
+      auto vec = [&](long d, int e1, int e2, int e3) -> TC {
+        return Mt(JAC, d) * (Mt(e1, d) * (W(T1[cc], d) * S(T1[cc])) + Mt(e2, d) * (W(T2[cc], d) * S(T2[cc])) +
+                             Mt(e3, d) * (W(T3[cc], d) * S(T3[cc])));
+      };
+      constexpr int xlo = (DX == 0) ? -3 : -1, ylo = (DY == 0) ? -3 : -1;
+      TC DxTx = TC(0), DyTy = TC(0), DzTz = TC(0);
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        DxTx += Drp<TC, DX>::c(n) * vec((long)(xlo + n), XIX, XIY, XIZ);
+        DyTy += Drp<TC, DY>::c(n) * vec((long)(ylo + n) * L.px, ETX, ETY, ETZ);
+      }
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        TC v;
+        if (n < n_free) v = vec((long)(zlo + n) * L.pz, ZTX, ZTY, ZTZ);
+        else if (n == n_free) v = TC(0);
+        else v = -vec((long)(zlo + n - 2 * (n - n_free)) * L.pz, ZTX, ZTY, ZTZ);
+        DzTz += Drp<TC, DZ>::c(n) * v;
+      }
+      h[VX + cc] = (DxTx + DyTy + DzTz) * slwjac;
+    }
+  } else {
+    h[VX] = slw * (xix * Dx[TXX] + xiy * Dx[TXY] + xiz * Dx[TXZ] + etx * Dy[TXX] + ety * Dy[TXY] + etz * Dy[TXZ] +
+                   ztx * Dz[TXX] + zty * Dz[TXY] + ztz * Dz[TXZ]);
+    h[VY] = slw * (xix * Dx[TXY] + xiy * Dx[TYY] + xiz * Dx[TYZ] + etx * Dy[TXY] + ety * Dy[TYY] + etz * Dy[TYZ] +
+                   ztx * Dz[TXY] + zty * Dz[TYY] + ztz * Dz[TYZ]);
+    h[VZ] = slw * (xix * Dx[TXZ] + xiy * Dx[TYZ] + xiz * Dx[TZZ] + etx * Dy[TXZ] + ety * Dy[TYZ] + etz * Dy[TZZ] +
+                   ztx * Dz[TXZ] + zty * Dz[TYZ] + ztz * Dz[TZZ]);
+  }
+
+  TC DzVx = Dz[VX], DzVy = Dz[VY], DzVz = Dz[VZ];
+  if (top && k >= ktop - 2) {
+    const int n = ktop - k;
+    if (n == 0) {
+      const long pl = (long)L.ni * L.nj, q = (long)j * L.ni + i;
+      const TC *A = a.mat + q, *B = a.mat + 9 * pl + q;
+      DzVx = A[0] * Dx[VX] + A[pl] * Dx[VY] + A[2 * pl] * Dx[VZ] + B[0] * Dy[VX] + B[pl] * Dy[VY] + B[2 * pl] * Dy[VZ];
+      DzVy = A[3 * pl] * Dx[VX] + A[4 * pl] * Dx[VY] + A[5 * pl] * Dx[VZ] + B[3 * pl] * Dy[VX] + B[4 * pl] * Dy[VY] +
+             B[5 * pl] * Dy[VZ];
+      DzVz = A[6 * pl] * Dx[VX] + A[7 * pl] * Dx[VY] + A[8 * pl] * Dx[VZ] + B[6 * pl] * Dy[VX] + B[7 * pl] * Dy[VY] +
+             B[8 * pl] * Dy[VZ];
+    } else if (n == 1) {
+      // 2nd-order op, layer 1 (fd_t.cu:82-85, 103-106)
+      const long z = L.pz;
+      if (DZ == 0) {
+        DzVx = -W(VX, -z) + W(VX, 0); DzVy = -W(VY, -z) + W(VY, 0); DzVz = -W(VZ, -z) + W(VZ, 0);
+      } else {
+        DzVx = -W(VX, 0) + W(VX, z); DzVy = -W(VY, 0) + W(VY, z); DzVz = -W(VZ, 0) + W(VZ, z);
+      }
+      DzVx *= S(VX); DzVy *= S(VY); DzVz *= S(VZ);
+    } else {
+      // 4th-order op, layer 2 (fd_t.cu:86-89, 107-110)
+      const long z = L.pz;
+      const TC c0 = (DZ == 0) ? TC(1.0 / 6.0) : TC(-7.0 / 6.0);
+      const TC c1 = (DZ == 0) ? TC(-8.0 / 6.0) : TC(8.0 / 6.0);
+      const TC c2 = (DZ == 0) ? TC(7.0 / 6.0) : TC(-1.0 / 6.0);
+      const long s0 = (DZ == 0) ? -2 * z : 0, s1 = (DZ == 0) ? -z : z, s2 = (DZ == 0) ? 0 : 2 * z;
+      DzVx = (c0 * W(VX, s0) + c1 * W(VX, s1) + c2 * W(VX, s2)) * S(VX);
+      DzVy = (c0 * W(VY, s0) + c1 * W(VY, s1) + c2 * W(VY, s2)) * S(VY);
+      DzVz = (c0 * W(VZ, s0) + c1 * W(VZ, s1) + c2 * W(VZ, s2)) * S(VZ);
+    }
+  }
+  h[TXX] = lam2mu * (xix * Dx[VX] + etx * Dy[VX] + ztx * DzVx) +
+           lam * (xiy * Dx[VY] + ety * Dy[VY] + zty * DzVy + xiz * Dx[VZ] + etz * Dy[VZ] + ztz * DzVz);
+  h[TYY] = lam2mu * (xiy * Dx[VY] + ety * Dy[VY] + zty * DzVy) +
+           lam * (xix * Dx[VX] + etx * Dy[VX] + ztx * DzVx + xiz * Dx[VZ] + etz * Dy[VZ] + ztz * DzVz);
+  h[TZZ] = lam2mu * (xiz * Dx[VZ] + etz * Dy[VZ] + ztz * DzVz) +
+           lam * (xix * Dx[VX] + etx * Dy[VX] + ztx * DzVx + xiy * Dx[VY] + ety * Dy[VY] + zty * DzVy);
+  h[TXY] = mu * (xiy * Dx[VX] + xix * Dx[VY] + ety * Dy[VX] + etx * Dy[VY] + zty * DzVx + ztx * DzVy);
+  h[TXZ] = mu * (xiz * Dx[VX] + xix * Dx[VZ] + etz * Dy[VX] + etx * Dy[VZ] + ztz * DzVx + ztx * DzVz);
+  h[TYZ] = mu * (xiz * Dx[VY] + xiy * Dx[VZ] + etz * Dy[VY] + ety * Dy[VZ] + ztz * DzVy + zty * DzVz);
+
+  // ---- CFS-PML faces, in the reference's face order ----
+#pragma unroll
+  for (int fi = 0; fi < 6; fi++) {
+    const FaceArgs<TS, TC> &F = a.f[fi];
+    if (!F.on) continue;
+    if (i < F.i1 || i > F.i2 || j < F.j1 || j > F.j2 || k < F.k1 || k > F.k2) continue;
+    const int dim = fi >> 1;
+    const int ic = (dim == 0) ? i - F.i1 : (dim == 1) ? j - F.j1 : k - F.k1;
+    const long ia = ((long)(k - F.k1) * F.bj + (j - F.j1)) * F.bi + (i - F.i1);
+    const TC cD = F.D[ic], cA = F.A[ic], cB = F.B[ic];
+    const TC Bm1 = cB - TC(1);
+    const TC *Dd = (dim == 0) ? Dx : (dim == 1) ? Dy : Dz;
+    const TC e1 = (dim == 0) ? xix : (dim == 1) ? etx : ztx;
+    const TC e2 = (dim == 0) ? xiy : (dim == 1) ? ety : zty;
+    const TC e3 = (dim == 0) ? xiz : (dim == 1) ? etz : ztz;
+    TC R[kNC];
+    R[VX] = slw * (e1 * Dd[TXX] + e2 * Dd[TXY] + e3 * Dd[TXZ]);
+    R[VY] = slw * (e1 * Dd[TXY] + e2 * Dd[TYY] + e3 * Dd[TYZ]);
+    R[VZ] = slw * (e1 * Dd[TXZ] + e2 * Dd[TYZ] + e3 * Dd[TZZ]);
+    R[TXX] = lam2mu * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam * e3 * Dd[VZ];
+    R[TYY] = lam * e1 * Dd[VX] + lam2mu * e2 * Dd[VY] + lam * e3 * Dd[VZ];
+    R[TZZ] = lam * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam2mu * e3 * Dd[VZ];
+    R[TXY] = mu * (e2 * Dd[VX] + e1 * Dd[VY]);
+    R[TXZ] = mu * (e3 * Dd[VX] + e1 * Dd[VZ]);
+    R[TYZ] = mu * (e3 * Dd[VY] + e2 * Dd[VZ]);
+    const long fso = (long)shot * F.shot_stride;
+    const TS *Pin = F.a_in + fso;
+    TC P[kNC], Ph[kNC];
+#pragma unroll
+    for (int c = 0; c < kNC; c++) {
+      P[c] = (TC)St<TS>::ld(Pin + c * F.fsize + ia) * S(c);
+      h[c] += Bm1 * R[c] - cB * P[c];
+      Ph[c] = cD * R[c] - cA * P[c];
+    }
+    if (top && dim < 2 && k == ktop) {
+      const long pl = (long)L.ni * L.nj, q = (long)j * L.ni + i;
+      const TC *A = a.mat + (dim == 0 ? 0 : 9 * pl) + q;
+      const TC dvx = A[0] * Dd[VX] + A[pl] * Dd[VY] + A[2 * pl] * Dd[VZ];
+      const TC dvy = A[3 * pl] * Dd[VX] + A[4 * pl] * Dd[VY] + A[5 * pl] * Dd[VZ];
+      const TC dvz = A[6 * pl] * Dd[VX] + A[7 * pl] * Dd[VY] + A[8 * pl] * Dd[VZ];
+      TC t[kNC];
+      t[TXX] = lam2mu * (ztx * dvx) + lam * (zty * dvy + ztz * dvz);
+      t[TYY] = lam2mu * (zty * dvy) + lam * (ztx * dvx + ztz * dvz);
+      t[TZZ] = lam2mu * (ztz * dvz) + lam * (ztx * dvx + zty * dvy);
+      t[TXY] = mu * (zty * dvx + ztx * dvy);
+      t[TXZ] = mu * (ztz * dvx + ztx * dvz);
+      t[TYZ] = mu * (ztz * dvy + zty * dvz);
+      // (coef_B - 1.0) * rhs accumulated in double (sv_curv_col_el_iso_gpu.cu:900-905)
+      const double bm1 = (double)cB - 1.0;
+#pragma unroll
+      for (int c = TXX; c < kNC; c++) {
+        h[c] = (TC)((double)h[c] + bm1 * (double)t[c]);
+        Ph[c] += cD * t[c];
+      }
+    }
+    // RK update of this face's memory variables
+    TS *Pout = F.a_out + fso, *Pend = F.a_end + fso;
+    const TS *Ppre = F.a_pre + fso;
+#pragma unroll
+    for (int c = 0; c < kNC; c++) {
+      const long q = c * F.fsize + ia;
+      const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
+      if (a.mode == MODE_RHS) {
+        St<TS>::st(Pout + q, Ph[c] * is);
+      } else if (a.mode == MODE_S0) {
+        St<TS>::st(Pout + q, (P[c] + a.ca * Ph[c]) * is);
+        St<TS>::st(Pend + q, (P[c] + a.cb * Ph[c]) * is);
+      } else if (a.mode == MODE_S12) {
+        const TC pre = (TC)St<TS>::ld(Ppre + q) * S(c);
+        const TC en = (TC)St<TS>::ld(Pend + q) * S(c);
+        St<TS>::st(Pout + q, (pre + a.ca * Ph[c]) * is);
+        St<TS>::st(Pend + q, (en + a.cb * Ph[c]) * is);
+      } else {
+        const TC en = (TC)St<TS>::ld(Pend + q) * S(c);
+        St<TS>::st(Pend + q, (en + a.cb * Ph[c]) * is);
+      }
+    }
+  }
+
+  // ---- RK update of the wavefield ----
+  TS *wout = a.w_out + so, *wend = a.w_end + so;
+  const TS *wpre = a.w_pre + so;
+#pragma unroll
+  for (int c = 0; c < kNC; c++) {
+    const long q = c * cs + o;
+    const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
+    if (a.mode == MODE_RHS) {
+      St<TS>::st(wout + q, h[c] * is);
+    } else if (a.mode == MODE_S0) {
+      const TC v = W(c, 0) * S(c);
+      St<TS>::st(wout + q, (v + a.ca * h[c]) * is);
+      St<TS>::st(wend + q, (v + a.cb * h[c]) * is);
+    } else if (a.mode == MODE_S12) {
+      const TC pre = (TC)St<TS>::ld(wpre + q) * S(c);
+      const TC en = (TC)St<TS>::ld(wend + q) * S(c);
+      St<TS>::st(wout + q, (pre + a.ca * h[c]) * is);
+      St<TS>::st(wend + q, (en + a.cb * h[c]) * is);
+    } else {
+      const TC en = (TC)St<TS>::ld(wend + q) * S(c);
+      St<TS>::st(wend + q, (en + a.cb * h[c]) * is);
+    }
+  }
+}
+
+template <int DX, int DY, int DZ, typename TS, typename TC>
+__global__ __launch_bounds__(256) void k_point(StageArgs<TS, TC> a) {
+  const int shot = blockIdx.y;
+  if (a.shot_active && !a.shot_active[shot]) return;
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int b = 0;
+  while (b < a.nbox && gid >= a.box_end[b]) b++;
+  if (b >= a.nbox) return;
+  const long loc = gid - (b ? a.box_end[b - 1] : 0);
+  const int bx = a.box[b][1] - a.box[b][0] + 1, by = a.box[b][3] - a.box[b][2] + 1;
+  const int i = a.box[b][0] + (int)(loc % bx);
+  const long t = loc / bx;
+  const int j = a.box[b][2] + (int)(t % by);
+  const int k = a.box[b][4] + (int)(t / by);
+  point_update<DX, DY, DZ, TS, TC>(a, shot, i, j, k);
+}
+
+// ---- source injection (sv_curv_col_el_iso_rhs_src_gpu, sv_curv_col_el_gpu.cu:227-301) ----
+// One thread per distinct grid point; contributions of the shot's sources are
+// summed in source order and applied to the RK outputs of this stage.
+template <typename TS, typename TC>
+__global__ __launch_bounds__(256) void k_source(SrcArgs<TS, TC> a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.npt) return;
+  const int shot = a.pt_shot[p];
+  if (a.shot_active && !a.shot_active[shot]) return;
+  const int it = *a.clock - a.shot_start[shot];
+  const long o = a.pt_off[p];
+  const long cs = a.L.cs;
+  const TC jac = a.met[JAC * cs + o], slw = a.med[2 * cs + o];
+  TC v[kNC];
+#pragma unroll
+  for (int c = 0; c < kNC; c++) v[c] = TC(0);
+  bool any = false;
+  for (int q = a.pt_beg[p]; q < a.pt_beg[p + 1]; q++) {
+    const int s = a.ct_src[q];
+    if (it < a.s_itb[s] || it > a.s_ite[s]) continue;
+    any = true;
+    const long ist = a.s_tab[s] + (long)(it - a.s_itb[s]) * 4 + a.istage;
+    const TC coef = a.ct_coef[q];
+    if (a.s_kind[s] & 1) {
+      const TC V = coef * slw / jac;
+      v[VX] += a.F[ist] * V;
+      v[VY] += a.F[a.ntab + ist] * V;
+      v[VZ] += a.F[2 * a.ntab + ist] * V;
+    }
+    if (a.s_kind[s] & 2) {
+      const TC rjac = coef / jac;
+      v[TXX] += -a.M[ist] * rjac;
+      v[TYY] += -a.M[a.ntab + ist] * rjac;
+      v[TZZ] += -a.M[2 * a.ntab + ist] * rjac;
+      v[TYZ] += -a.M[3 * a.ntab + ist] * rjac;
+      v[TXZ] += -a.M[4 * a.ntab + ist] * rjac;
+      v[TXY] += -a.M[5 * a.ntab + ist] * rjac;
+    }
+  }
+  if (!any) return;
+  const long so = (long)shot * a.shot_stride;
+#pragma unroll
+  for (int c = 0; c < kNC; c++) {
+    if (v[c] == TC(0)) continue;
+    const long q = so + c * cs + o;
+    const TC sc = St<TS>::scaled ? TC(1) / (TC)a.isc[c] : TC(1);
+    const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
+    if (a.mode == MODE_RHS) {
+      St<TS>::st(a.w_out + q, ((TC)St<TS>::ld(a.w_out + q) * sc + v[c]) * is);
+    } else if (a.mode == MODE_S0 || a.mode == MODE_S12) {
+      St<TS>::st(a.w_out + q, ((TC)St<TS>::ld(a.w_out + q) * sc + a.ca * v[c]) * is);
+      St<TS>::st(a.w_end + q, ((TC)St<TS>::ld(a.w_end + q) * sc + a.cb * v[c]) * is);
+    } else {
+      St<TS>::st(a.w_end + q, ((TC)St<TS>::ld(a.w_end + q) * sc + a.cb * v[c]) * is);
+    }
+  }
+}
+
+// ---- receivers (io_recv_keep + io_recv_line_interp_pack_buff, io_funcs.cu:1624-1666, 1747-1763) ----
+// Corner 5 carries both (x1,y1,z2) and (x2,y1,z2) weights and corner 4 is
+// unused, as in the reference; with on-node receivers the quirk vanishes.
+template <typename TS, typename TC>
+__global__ __launch_bounds__(256) void k_recv(RecvArgs<TS, TC> a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nrec * kNC) return;
+  const int r = t / kNC, c = t % kNC;
+  const int shot = a.r_shot[r];
+  if (a.shot_active && !a.shot_active[shot]) return;
+  const int it = *a.clock - a.shot_start[shot];
+  if (it < 0 || it >= a.r_nt[r]) return;
+  const TS *w = a.w_pre + (long)shot * a.shot_stride + c * a.cs;
+  const long *ix = a.r_off + (long)r * 8;
+  const TC sc = St<TS>::scaled ? (TC)a.sc[c] : TC(1);
+  TC b[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) b[q] = (TC)St<TS>::ld(w + ix[q]) * sc;
+  const TC Lx2 = a.r_frac[r * 3 + 0], Ly2 = a.r_frac[r * 3 + 1], Lz2 = a.r_frac[r * 3 + 2];
+  const TC Lx1 = TC(1) - Lx2, Ly1 = TC(1) - Ly2, Lz1 = TC(1) - Lz2;
+  const TC s = b[0] * Lx1 * Ly1 * Lz1 + b[1] * Lx2 * Ly1 * Lz1 + b[2] * Lx1 * Ly2 * Lz1 + b[3] * Lx2 * Ly2 * Lz1 +
+               b[5] * Lx1 * Ly1 * Lz2 + b[5] * Lx2 * Ly1 * Lz2 + b[6] * Lx1 * Ly2 * Lz2 + b[7] * Lx2 * Ly2 * Lz2;
+  a.seismo[a.r_out[r] + (long)c * a.r_nt[r] + it] = s;
+}
+
+__global__ void k_clock_advance(int *clock) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *clock += 1;
+}
+
+template <typename T> __global__ __launch_bounds__(256) void k_fill(T *p, long n, T v) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+// ---- launchers ------------------------------------------------------------------
+template <typename TS, typename TC>
+hipError_t launch_point(const StageArgs<TS, TC> &a, int dx, int dy, int dz, int nshots, hipStream_t st) {
+  const long n = a.nbox ? a.box_end[a.nbox - 1] : 0;
+  if (n == 0) return hipSuccess;
+  dim3 grid((unsigned)((n + 255) / 256), (unsigned)nshots), block(256);
+  const int sel = dx * 4 + dy * 2 + dz;
+  switch (sel) {
+    case 0: hipLaunchKernelGGL((k_point<0, 0, 0, TS, TC>), grid, block, 0, st, a); break;
+    case 1: hipLaunchKernelGGL((k_point<0, 0, 1, TS, TC>), grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_point<0, 1, 0, TS, TC>), grid, block, 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_point<0, 1, 1, TS, TC>), grid, block, 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_point<1, 0, 0, TS, TC>), grid, block, 0, st, a); break;
+    case 5: hipLaunchKernelGGL((k_point<1, 0, 1, TS, TC>), grid, block, 0, st, a); break;
+    case 6: hipLaunchKernelGGL((k_point<1, 1, 0, TS, TC>), grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL((k_point<1, 1, 1, TS, TC>), grid, block, 0, st, a); break;
+  }
+  return hipGetLastError();
+}
+
+template <typename TS, typename TC> hipError_t launch_source(const SrcArgs<TS, TC> &a, hipStream_t st) {
+  if (a.npt == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_source<TS, TC>), dim3((a.npt + 255) / 256), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+template <typename TS, typename TC> hipError_t launch_recv(const RecvArgs<TS, TC> &a, hipStream_t st) {
+  if (a.nrec == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_recv<TS, TC>), dim3((a.nrec * kNC + 255) / 256), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_clock_advance(int *clock, hipStream_t st) {
+  hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, st, clock);
+  return hipGetLastError();
+}
+
+template <typename T> hipError_t launch_fill(T *p, long n, T v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL((k_fill<T>), dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+#define CGFD_INST(TS, TC)                                                                                   \
+  template hipError_t launch_point<TS, TC>(const StageArgs<TS, TC> &, int, int, int, int, hipStream_t);    \
+  template hipError_t launch_source<TS, TC>(const SrcArgs<TS, TC> &, hipStream_t);                          \
+  template hipError_t launch_recv<TS, TC>(const RecvArgs<TS, TC> &, hipStream_t);
+CGFD_INST(float, float)
+CGFD_INST(double, double)
+CGFD_INST(__half, float)
+CGFD_INST(__hip_bfloat16, float)
+#undef CGFD_INST
+template hipError_t launch_fill<float>(float *, long, float, hipStream_t);
+template hipError_t launch_fill<double>(double *, long, double, hipStream_t);
+template hipError_t launch_fill<uint16_t>(uint16_t *, long, uint16_t, hipStream_t);
+
+}  // namespace cgfd

@@ -1,0 +1,185 @@
+/*
+ * cgfd_el_iso.h -- C ABI of the MI355X (gfx950) elastic-isotropic RK4
+ * time-stepping path of CGFD3D.
+ *
+ * The library owns every device buffer; callers pass plain host arrays in the
+ * reference's host layout and get results copied back on request.  No torch
+ * or HIP types appear in these signatures.
+ *
+ * Host array layout ("reference layout"): one component is nx*ny*nz values,
+ * nx = ni+6, ny = nj+6, nz = nk+6 (3 ghost points per side), i fastest, then
+ * j, then k; component c of a multi-component array starts at c*nx*ny*nz.
+ * This is the layout of gd_t / gd_metric_t / md_t / wav_t in the reference
+ * (src/forward/gd_t.cu:1985-2021, wav_t.cu:35-40).
+ *
+ * Wavefield component order: Vx,Vy,Vz,Txx,Tyy,Tzz,Tyz,Txz,Txy
+ * (src/forward/wav_t.cu:100-140).  Metric order: jac, xi_x, xi_y, xi_z,
+ * eta_x, eta_y, eta_z, zeta_x, zeta_y, zeta_z (src/forward/gd_t.cu:115-200).
+ *
+ * Error convention: every int-returning call returns CGFD_OK (0) or a
+ * negative CGFD_E_* code; cgfd_last_error() gives a thread-local message.
+ * The library never aborts the process.
+ */
+#ifndef CGFD_EL_ISO_H
+#define CGFD_EL_ISO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGFD_OK 0
+#define CGFD_E_ARG (-1)      /* invalid argument */
+#define CGFD_E_HIP (-2)      /* HIP runtime error */
+#define CGFD_E_NOMEM (-3)    /* insufficient HBM for the request */
+#define CGFD_E_STATE (-4)    /* call out of order (e.g. run before upload) */
+#define CGFD_E_NODEV (-5)    /* no usable gfx950 device */
+
+/* wavefield / PML-memory storage type; arithmetic is FP32 except F64 */
+#define CGFD_STORE_F32 0
+#define CGFD_STORE_F16 1
+#define CGFD_STORE_BF16 2
+#define CGFD_STORE_F64 3
+
+/* dtype of host arrays handed to upload/fetch calls */
+#define CGFD_DT_F32 0
+#define CGFD_DT_F64 1
+
+/* wavefield levels of one shot */
+#define CGFD_LEVEL_PRE 0   /* W^n: the level receivers and snapshots read */
+#define CGFD_LEVEL_TMPA 1  /* RK intermediate (ping) */
+#define CGFD_LEVEL_TMPB 2  /* RK intermediate (pong) */
+#define CGFD_LEVEL_END 3   /* RK accumulator of W^{n+1} */
+#define CGFD_LEVEL_RHS 4   /* right-hand side written by cgfd_el_iso_onestage */
+
+/* PML faces */
+#define CGFD_FACE_X1 0
+#define CGFD_FACE_X2 1
+#define CGFD_FACE_Y1 2
+#define CGFD_FACE_Y2 3
+#define CGFD_FACE_Z1 4
+#define CGFD_FACE_Z2 5
+
+typedef struct cgfd_session cgfd_session;
+
+typedef struct {
+  int ni, nj, nk;   /* physical points per axis, PML layers included (gd_t ni/nj/nk) */
+  int storage;      /* CGFD_STORE_* */
+  int nshots;       /* independent shots sharing the model; 1 for a plain run */
+  int flags;        /* CGFD_FLAG_* */
+} cgfd_desc;
+
+/* use the plain per-point kernel everywhere (no z-marching interior kernel) */
+#define CGFD_FLAG_PLAIN 1
+
+/* Point-source tables of one shot: the device mirror of src_t
+ * (src/forward/src_t.h:14-53, filled by src_read_locate_file src_t.cu:143-790). */
+typedef struct {
+  int nsrc;                  /* point sources of this shot */
+  int force_on, moment_on;   /* which tables are present */
+  int max_nt;                /* STF samples per source */
+  int max_ext;               /* spreading points per source (<= 343) */
+  int dtype;                 /* CGFD_DT_* of ext_coef/F/M */
+  const int *it_begin;       /* [nsrc] first time step the source acts */
+  const int *it_end;         /* [nsrc] last time step */
+  const int *ext_num;        /* [nsrc] valid spreading points */
+  const long long *ext_indx; /* [nsrc*max_ext] flat index in reference layout */
+  const void *ext_coef;      /* [nsrc*max_ext] spreading weights */
+  const void *F;             /* [3][nsrc][max_nt][4] Fx,Fy,Fz per RK stage, or NULL */
+  const void *M;             /* [6][nsrc][max_nt][4] Mxx,Myy,Mzz,Myz,Mxz,Mxy, or NULL */
+} cgfd_source_desc;
+
+/* ---- session ---------------------------------------------------------- */
+
+/* Allocate a session on the current HIP device (device memory for nshots
+ * wavefields of 4 levels each plus the shared model).  Returns CGFD_E_NOMEM
+ * when free HBM is insufficient.  Replaces the device setup of
+ * drv_rk_curv_col_allstep (src/forward/drv_rk_curv_col.cu:98-133). */
+int cgfd_create(const cgfd_desc *desc, cgfd_session **out);
+
+/* Free every device buffer (src/forward/drv_rk_curv_col.cu:544-556). */
+void cgfd_destroy(cgfd_session *s);
+
+/* Bytes of device memory a session with this descriptor would allocate. */
+long long cgfd_footprint_bytes(const cgfd_desc *desc, int nrecv, int nt_total);
+
+/* Thread-local description of the last error. */
+const char *cgfd_last_error(void);
+
+/* ---- model upload (shared by all shots) -------------------------------- */
+
+/* 10 metric components in reference layout (init_metric_device, alloc.cu). */
+int cgfd_upload_metric(cgfd_session *s, int dtype, const void *metric);
+
+/* lambda, mu and 1/rho (after md_rho_to_slow, src/forward/md_t.cu:622-637). */
+int cgfd_upload_medium(cgfd_session *s, int dtype, const void *lam, const void *mu, const void *slw);
+
+/* Free-surface velocity-gradient conversion matrices at the top (k = nk2):
+ * matVx2Vz, matVy2Vz of nx*ny*9 values [j][i][row][col]
+ * (sv_curv_col_el_iso_dvh2dvz, src/forward/sv_curv_col_el_iso_gpu.cu:1152-1261).
+ * free_top = 0 disables the free surface. */
+int cgfd_upload_free_surface(cgfd_session *s, int free_top, int dtype, const void *matVx2Vz, const void *matVy2Vz);
+
+/* CFS-PML profile of one face: A, B, D of nlay+1 values ordered along the
+ * axis from low to high index (bdry_pml_set, src/forward/bdry_t.cu:85-257).
+ * nlay = 0 disables the face. */
+int cgfd_upload_pml(cgfd_session *s, int face, int nlay, int dtype, const void *A, const void *B, const void *D);
+
+/* Per-component power-of-two storage scales for F16/BF16 sessions:
+ * stored = value * 2^-exp2[c] for wavefield component c (and its PML memory). */
+int cgfd_set_storage_scale(cgfd_session *s, const int exp2[9]);
+
+/* ---- per shot ----------------------------------------------------------- */
+
+int cgfd_upload_source(cgfd_session *s, int shot, const cgfd_source_desc *src);
+
+/* Receivers of one shot: 8 trilinear corner indices (flat, reference layout,
+ * order of iorecv_one_t.indx1d, src/forward/io_funcs.cu:227-234) and the
+ * fractions di,dj,dk; nt_total samples are kept per component. */
+int cgfd_upload_receivers(cgfd_session *s, int shot, int nrecv, const long long *indx8, int dtype,
+                          const void *frac3, int nt_total);
+
+/* Set one level of one shot from a host array (9 components, reference
+ * layout; ghosts must be zero).  Levels not set are zero. */
+int cgfd_set_level(cgfd_session *s, int shot, int level, int dtype, const void *w9);
+
+/* Zero every level and PML memory of a shot and rewind its step counter. */
+int cgfd_reset_shot(cgfd_session *s, int shot);
+
+/* ---- time stepping --------------------------------------------------------- */
+
+/* Advance every shot by nsteps full RK4 steps starting at step it0, with
+ * receivers sampled from W^n before each step, as drv_rk_curv_col_allstep
+ * (src/forward/drv_rk_curv_col.cu:193-542): step it uses operator pair
+ * it % 8, stage times {0,1/2,1/2,1}*dt, and W^{n+1} becomes the PRE level.
+ * dt is rounded to float for FP32/F16/BF16 sessions, as the reference keeps it. */
+int cgfd_run(cgfd_session *s, int it0, int nsteps, double dt);
+
+/* One RK-stage right-hand side into the RHS level (and the PML memory RHS):
+ * the operator sv_curv_col_el_iso_onestage
+ * (src/forward/sv_curv_col_el_iso_gpu.cu:23-285) reading level in_level. */
+int cgfd_el_iso_onestage(cgfd_session *s, int shot, int in_level, int ipair, int istage, int it);
+
+/* ---- results ---------------------------------------------------------------- */
+
+/* Seismograms of one shot: [nrecv][9][nt_total] (iorecv_one_t.seismo). */
+int cgfd_read_seismo(cgfd_session *s, int shot, int dtype, void *out);
+
+/* One level of one shot back to a host array in reference layout (ghosts zero). */
+int cgfd_fetch_level(cgfd_session *s, int shot, int level, int dtype, void *w9);
+
+/* PML memory of one face: level = PRE/TMPA/TMPB/END or RHS; [9][face box]
+ * with the face box indexed iz*(nj*ni)+iy*ni+ix (sv_curv_col_el_iso_gpu.cu:775). */
+int cgfd_fetch_pml_aux(cgfd_session *s, int shot, int face, int level, int dtype, void *out);
+
+/* Enable (default) or disable HIP-event timing of interior kernel launches. */
+int cgfd_set_timing(cgfd_session *s, int on);
+
+/* Milliseconds of device time spent in the last cgfd_run, and the average
+ * duration of its z-marching interior kernel launches (events on the
+ * session stream). */
+int cgfd_last_run_timing(cgfd_session *s, double *run_ms, double *interior_avg_ms, int *interior_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,79 @@
+"""ctypes access to the CPU oracle (oracle/liborc_el_iso.so) -- the checker.
+
+Only tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORC_PATH = os.path.join(ROOT, "oracle", "liborc_el_iso.so")
+
+DP = C.POINTER(C.c_double)
+
+
+class OrcProblem(C.Structure):
+    _fields_ = [
+        ("ni", C.c_int), ("nj", C.c_int), ("nk", C.c_int),
+        ("metric", C.c_void_p), ("lam", C.c_void_p), ("mu", C.c_void_p), ("slw", C.c_void_p),
+        ("free_top", C.c_int), ("matVx2Vz", C.c_void_p), ("matVy2Vz", C.c_void_p),
+        ("pml_on", C.c_int * 6), ("pml_nlay", C.c_int * 6),
+        ("pml_A", C.c_void_p * 6), ("pml_B", C.c_void_p * 6), ("pml_D", C.c_void_p * 6),
+        ("nsrc", C.c_int), ("force_on", C.c_int), ("moment_on", C.c_int), ("max_nt", C.c_int),
+        ("max_ext", C.c_int),
+        ("src_it_begin", C.c_void_p), ("src_it_end", C.c_void_p), ("src_ext_num", C.c_void_p),
+        ("src_ext_indx", C.c_void_p), ("src_ext_coef", C.c_void_p),
+        ("src_F", C.c_void_p), ("src_M", C.c_void_p),
+        ("nrecv", C.c_int), ("recv_indx", C.c_void_p), ("recv_frac", C.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORC_PATH):
+            raise RuntimeError(f"{ORC_PATH} not built; run make -C oracle")
+        L = C.CDLL(ORC_PATH)
+        vp, i, d = C.c_void_p, C.c_int, C.c_double
+        L.orc_run.argtypes = [C.POINTER(OrcProblem), i, d, i, vp, vp, vp, vp, vp]
+        L.orc_fd_op.argtypes = [i, i, vp, vp]
+        L.orc_fdc.argtypes = [vp]
+        L.orc_pair_dirs.argtypes = [i, i, vp]
+        L.orc_rk.argtypes = [vp, vp, vp]
+        L.orc_onestage.argtypes = [C.POINTER(OrcProblem), i, i, i, i, vp, vp, vp, vp]
+        L.orc_face_sizes.argtypes = [C.POINTER(OrcProblem), vp]
+        L.orc_metric_cal.argtypes = [i, i, i, i, vp, vp, vp, vp]
+        L.orc_free_matrices.argtypes = [i, i, i, i, vp, vp, vp, vp, vp]
+        L.orc_gen_cart.argtypes = [i, i, i, d, d, d, d, d, d, vp, vp, vp]
+        L.orc_abl_len_dh.argtypes = [i, i, i, vp, vp, vp, i, i, i, i, i, i, i, DP, DP]
+        L.orc_pml_coefs.argtypes = [i, i, d, d, d, d, d, vp, vp, vp]
+        L.orc_ricker.argtypes = [d, d, d]
+        L.orc_ricker.restype = d
+        L.orc_ricker_deriv.argtypes = [d, d, d]
+        L.orc_ricker_deriv.restype = d
+        L.orc_gauss.argtypes = [d, d, d]
+        L.orc_gauss.restype = d
+        L.orc_norm_delt3d.argtypes = [d, d, d, d, d, d, i, vp]
+        L.orc_angle2moment.argtypes = [d, d, d, vp]
+        L.orc_src_point.argtypes = [i, i, i, d, d, d, i, d, d, d, i, d, d, d,
+                                    vp, vp, vp, vp, vp, vp, vp]
+        L.orc_recv_index.argtypes = [i, i, i, d, d, d, i, vp, vp]
+        L.orc_dt_esti.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, d]
+        L.orc_dt_esti.restype = d
+        L.orc_rho_to_slow.argtypes = [C.c_size_t, vp, vp]
+        L.orc_set_threads.argtypes = [i]
+        L.orc_max_threads.restype = i
+        _lib = L
+    return _lib
+
+
+def p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
