@@ -26,7 +26,24 @@ EXPORTS = [
     "cgfd_upload_receivers", "cgfd_set_level", "cgfd_reset_shot", "cgfd_run",
     "cgfd_el_iso_onestage", "cgfd_read_seismo", "cgfd_fetch_level",
     "cgfd_fetch_pml_aux", "cgfd_set_timing", "cgfd_last_run_timing",
+    "cgfd_grid_cartesian", "cgfd_grid_topography", "cgfd_metric_calc", "cgfd_dt_estimate",
+    "cgfd_media_layered", "cgfd_media_homogeneous", "cgfd_rho_to_slow", "cgfd_pml_len_dh",
+    "cgfd_pml_profile", "cgfd_free_surface_matrices", "cgfd_wavelet", "cgfd_angle2moment",
+    "cgfd_source_point", "cgfd_receiver_index",
 ]
+
+
+class TopoDesc(C.Structure):
+    _fields_ = [("ni", C.c_int), ("nj", C.c_int), ("nk", C.c_int), ("dh", C.c_double), ("stretch", C.c_double),
+                ("nhill", C.c_int), ("amp_min", C.c_double), ("amp_max", C.c_double), ("sig_min", C.c_double),
+                ("sig_max", C.c_double), ("rough_rms", C.c_double), ("min_wavelength", C.c_double),
+                ("seed", C.c_ulonglong)]
+
+
+class LayerDesc(C.Structure):
+    _fields_ = [("nlayer", C.c_int), ("z_interface", C.c_void_p), ("vp", C.c_void_p), ("vpvs", C.c_void_p),
+                ("rho", C.c_void_p), ("pert_std", C.c_double), ("pert_clip", C.c_double),
+                ("corr_len", C.c_double), ("seed", C.c_ulonglong)]
 
 
 class Desc(C.Structure):
@@ -80,6 +97,23 @@ def load(path=LIB_PATH):
         "cgfd_set_timing": ([vp, i], i),
         "cgfd_last_run_timing": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)], i),
     }
+    d = C.c_double
+    sig.update({
+        "cgfd_grid_cartesian": ([i, i, i, d, d, d, d, d, d, vp, vp, vp], i),
+        "cgfd_grid_topography": ([C.POINTER(TopoDesc), vp, vp, vp], i),
+        "cgfd_metric_calc": ([i, i, i, vp, vp, vp, vp], i),
+        "cgfd_dt_estimate": ([i, i, i, vp, vp, vp, vp, vp, vp, d], d),
+        "cgfd_media_layered": ([i, i, i, vp, vp, vp, C.POINTER(LayerDesc), vp, vp, vp], i),
+        "cgfd_media_homogeneous": ([i, i, i, d, d, d, vp, vp, vp], i),
+        "cgfd_rho_to_slow": ([ll, vp, vp], i),
+        "cgfd_pml_len_dh": ([i, i, i, vp, vp, vp, i, i, C.POINTER(d), C.POINTER(d)], i),
+        "cgfd_pml_profile": ([i, i, d, d, d, d, d, vp, vp, vp], i),
+        "cgfd_free_surface_matrices": ([i, i, i, vp, vp, vp, vp, vp], i),
+        "cgfd_wavelet": ([i, d, d, d], d),
+        "cgfd_angle2moment": ([d, d, d, vp], i),
+        "cgfd_source_point": ([i, i, i, d, d, d, i, d, d, d, i, d, d, d, vp, vp, vp, vp, vp, vp, vp], i),
+        "cgfd_receiver_index": ([i, i, i, d, d, d, i, vp, vp], i),
+    })
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
         f.argtypes = args
@@ -229,3 +263,128 @@ class Session:
         a, b, n = C.c_double(), C.c_double(), C.c_int()
         _check(_lib.cgfd_last_run_timing(self.h, C.byref(a), C.byref(b), C.byref(n)))
         return a.value, b.value, n.value
+
+
+# ---------------------------------------------------------------- host setup
+def _z(shape, dtype=np.float64):
+    return np.zeros(shape, dtype=dtype)
+
+
+def grid_cartesian(ni, nj, nk, dx, x0, dy, y0, dz, z0):
+    load()
+    sh = (nk + 6, nj + 6, ni + 6)
+    x, y, z = _z(sh), _z(sh), _z(sh)
+    _check(_lib.cgfd_grid_cartesian(ni, nj, nk, dx, x0, dy, y0, dz, z0, _ptr(x), _ptr(y), _ptr(z)))
+    return x, y, z
+
+
+def grid_topography(ni, nj, nk, dh, stretch, nhill, amp, sig, rough_rms, min_wavelength, seed):
+    load()
+    sh = (nk + 6, nj + 6, ni + 6)
+    x, y, z = _z(sh), _z(sh), _z(sh)
+    d = TopoDesc(ni, nj, nk, dh, stretch, nhill, amp[0], amp[1], sig[0], sig[1], rough_rms, min_wavelength, seed)
+    _check(_lib.cgfd_grid_topography(C.byref(d), _ptr(x), _ptr(y), _ptr(z)))
+    return x, y, z
+
+
+def metric_calc(ni, nj, nk, x, y, z):
+    load()
+    m = _z((10,) + x.shape)
+    _check(_lib.cgfd_metric_calc(ni, nj, nk, _ptr(_arr(x, np.float64)), _ptr(_arr(y, np.float64)),
+                                 _ptr(_arr(z, np.float64)), _ptr(m)))
+    return m
+
+
+def dt_estimate(ni, nj, nk, x, y, z, lam, mu, rho, cfl=1.3):
+    load()
+    a = [_arr(v, np.float64) for v in (x, y, z, lam, mu, rho)]
+    r = _lib.cgfd_dt_estimate(ni, nj, nk, *[_ptr(v) for v in a], cfl)
+    if r <= 0:
+        raise CgfdError("dt estimate failed")
+    return r
+
+
+def media_homogeneous(ni, nj, nk, vp, vs, rho):
+    load()
+    sh = (nk + 6, nj + 6, ni + 6)
+    l, m, r = _z(sh), _z(sh), _z(sh)
+    _check(_lib.cgfd_media_homogeneous(ni, nj, nk, vp, vs, rho, _ptr(l), _ptr(m), _ptr(r)))
+    return l, m, r
+
+
+def media_layered(ni, nj, nk, x, y, z, z_interface, vp, vpvs, rho, pert_std, pert_clip, corr_len, seed):
+    load()
+    sh = (nk + 6, nj + 6, ni + 6)
+    l, m, r = _z(sh), _z(sh), _z(sh)
+    zi, v, vv, rr = [_arr(a, np.float64) for a in (z_interface, vp, vpvs, rho)]
+    d = LayerDesc(len(v), _ptr(zi), _ptr(v), _ptr(vv), _ptr(rr), pert_std, pert_clip, corr_len, seed)
+    _check(_lib.cgfd_media_layered(ni, nj, nk, _ptr(_arr(x, np.float64)), _ptr(_arr(y, np.float64)),
+                                   _ptr(_arr(z, np.float64)), C.byref(d), _ptr(l), _ptr(m), _ptr(r)))
+    return l, m, r
+
+
+def rho_to_slow(rho):
+    load()
+    r = _arr(rho, np.float64)
+    s = np.zeros_like(r)
+    _check(_lib.cgfd_rho_to_slow(r.size, _ptr(r), _ptr(s)))
+    return s
+
+
+def pml_len_dh(ni, nj, nk, x, y, z, face, nlay):
+    load()
+    L0, dh = C.c_double(), C.c_double()
+    _check(_lib.cgfd_pml_len_dh(ni, nj, nk, _ptr(_arr(x, np.float64)), _ptr(_arr(y, np.float64)),
+                                _ptr(_arr(z, np.float64)), face, nlay, C.byref(L0), C.byref(dh)))
+    return L0.value, dh.value
+
+
+def pml_profile(nlay, face, L0, dh, vel, amax, bmax):
+    load()
+    A, B, D = _z(nlay + 1), _z(nlay + 1), _z(nlay + 1)
+    _check(_lib.cgfd_pml_profile(nlay, face, L0, dh, vel, amax, bmax, _ptr(A), _ptr(B), _ptr(D)))
+    return A, B, D
+
+
+def free_surface_matrices(ni, nj, nk, metric, lam, mu):
+    load()
+    mx = _z((nj + 6, ni + 6, 9))
+    my = _z((nj + 6, ni + 6, 9))
+    _check(_lib.cgfd_free_surface_matrices(ni, nj, nk, _ptr(_arr(metric, np.float64)), _ptr(_arr(lam, np.float64)),
+                                           _ptr(_arr(mu, np.float64)), _ptr(mx), _ptr(my)))
+    return mx, my
+
+
+def wavelet(kind, t, c0, c1):
+    load()
+    return _lib.cgfd_wavelet(kind, t, c0, c1)
+
+
+def angle2moment(strike, dip, rake):
+    load()
+    m = _z(6)
+    _check(_lib.cgfd_angle2moment(strike, dip, rake, _ptr(m)))
+    return m
+
+
+def source_point(ni, nj, nk, sx, sy, sz, depth_flag, t0, dt, wavelet_tstart, kind, c0, c1, stf_length):
+    """Returns dict(it_begin, it_end, ext_num, ext_indx[343], ext_coef[343], max_nt, stf[max_nt,4])."""
+    load()
+    itb, ite, en, mnt = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    ei = np.zeros(343, dtype=np.int64)
+    ec = _z(343)
+    args = (ni, nj, nk, sx, sy, sz, depth_flag, t0, dt, wavelet_tstart, kind, c0, c1, stf_length,
+            C.byref(itb), C.byref(ite), C.byref(en), _ptr(ei), _ptr(ec), C.byref(mnt))
+    _check(_lib.cgfd_source_point(*args, None))
+    stf = _z((mnt.value, 4))
+    _check(_lib.cgfd_source_point(*args, _ptr(stf)))
+    return dict(it_begin=itb.value, it_end=ite.value, ext_num=en.value, ext_indx=ei, ext_coef=ec,
+                max_nt=mnt.value, stf=stf)
+
+
+def receiver_index(ni, nj, nk, rx, ry, rz, depth_flag):
+    load()
+    ix = np.zeros(8, dtype=np.int64)
+    fr = _z(3)
+    rc = _lib.cgfd_receiver_index(ni, nj, nk, rx, ry, rz, depth_flag, _ptr(ix), _ptr(fr))
+    return (ix, fr) if rc == 0 else (None, None)

@@ -1,0 +1,541 @@
+// cgfd_setup.cpp -- FP64 host preparation of the el_iso inputs (see cgfd_setup.h).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/cgfd_el_iso.h"
+#include "../../include/cgfd_setup.h"
+
+namespace {
+
+constexpr int NG = 3;
+
+struct G3 {
+  int ni, nj, nk, nx, ny, nz;
+  long sy, sz, n;
+  G3(int a, int b, int c) : ni(a), nj(b), nk(c), nx(a + 2 * NG), ny(b + 2 * NG), nz(c + 2 * NG) {
+    sy = nx;
+    sz = (long)nx * ny;
+    n = sz * nz;
+  }
+  long at(int i, int j, int k) const { return (long)i + (long)j * sy + (long)k * sz; }
+};
+
+// run f(k) for k in [0, n) over hardware threads, deterministic per k
+template <typename F> void par_for(int n, F f) {
+  unsigned nt = std::max(1u, std::min(std::thread::hardware_concurrency(), 32u));
+  if (n < 4 || nt == 1) {
+    for (int k = 0; k < n; k++) f(k);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; t++)
+    th.emplace_back([&, t]() {
+      for (int k = (int)t; k < n; k += (int)nt) f(k);
+    });
+  for (auto &x : th) x.join();
+}
+
+// centred 7-point operator of the metric evaluation (src/forward/fd_t.cu:164-170)
+const double kFdc[7] = {-0.02084, 0.1667, -0.7709, 0.0, 0.7709, -0.1667, 0.02084};
+
+uint64_t mix(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cgfd_grid_cartesian(int ni, int nj, int nk, double dx, double x0, double dy, double y0, double dz, double z0,
+                        double *x, double *y, double *z) {
+  if (ni < 1 || nj < 1 || nk < 1 || !x || !y || !z) return CGFD_E_ARG;
+  G3 g(ni, nj, nk);
+  for (int k = 0; k < g.nz; k++)
+    for (int j = 0; j < g.ny; j++)
+      for (int i = 0; i < g.nx; i++) {
+        long p = g.at(i, j, k);
+        x[p] = x0 + (i - NG) * dx;
+        y[p] = y0 + (j - NG) * dy;
+        z[p] = z0 + (k - NG) * dz;
+      }
+  return CGFD_OK;
+}
+
+int cgfd_grid_topography(const cgfd_topo_desc *d, double *x, double *y, double *z) {
+  if (!d || !x || !y || !z || d->ni < 2 || d->nj < 2 || d->nk < 3 || d->dh <= 0 || d->stretch <= 0)
+    return CGFD_E_ARG;
+  G3 g(d->ni, d->nj, d->nk);
+  const double h = d->dh;
+  const double Lx = (d->ni - 1) * h, Ly = (d->nj - 1) * h;
+  std::mt19937_64 rng(d->seed);
+  std::uniform_real_distribution<double> U(0.0, 1.0);
+  std::vector<double> hx, hy, ha, hs;
+  for (int q = 0; q < d->nhill; q++) {
+    hx.push_back(Lx * (0.15 + 0.7 * U(rng)));
+    hy.push_back(Ly * (0.15 + 0.7 * U(rng)));
+    ha.push_back(d->amp_min + (d->amp_max - d->amp_min) * U(rng));
+    hs.push_back(d->sig_min + (d->sig_max - d->sig_min) * U(rng));
+  }
+  // Fourier perturbation: modes with wavelengths between min_wavelength and the domain size
+  struct Mode { double kx, ky, ph, a; };
+  std::vector<Mode> modes;
+  const int nmode = 48;
+  for (int q = 0; q < nmode; q++) {
+    double lam = d->min_wavelength * std::pow(std::max(Lx, Ly) / d->min_wavelength, U(rng));
+    double th = 2 * M_PI * U(rng);
+    double kk = 2 * M_PI / lam;
+    modes.push_back({kk * std::cos(th), kk * std::sin(th), 2 * M_PI * U(rng), 1.0});
+  }
+  // rms of a sum of unit-amplitude cosines with random phases is sqrt(n/2)
+  const double mscale = d->rough_rms / std::sqrt(nmode / 2.0);
+  auto surf = [&](double xx, double yy) {
+    double s = 0;
+    for (size_t q = 0; q < hx.size(); q++) {
+      double r2 = (xx - hx[q]) * (xx - hx[q]) + (yy - hy[q]) * (yy - hy[q]);
+      s += ha[q] * std::exp(-r2 / (2 * hs[q] * hs[q]));
+    }
+    for (const Mode &m : modes) s += mscale * std::cos(m.kx * xx + m.ky * yy + m.ph);
+    return s;
+  };
+  // depth below the surface in units of h for unit index m (m = 0 at the surface):
+  // spacing grows linearly from 1 (m = 0) to stretch (m = nk-1), extended analytically
+  const int nk = d->nk;
+  auto spacing = [&](double m) { return 1.0 + (d->stretch - 1.0) * m / (nk - 1); };
+  auto depth = [&](double m) {  // integral of spacing from 0 to m (exact for linear spacing)
+    return m + (d->stretch - 1.0) * m * m / (2.0 * (nk - 1));
+  };
+  (void)spacing;
+  const double Dref = depth(nk - 1) * h;   // flat-surface thickness
+  const double zbot = -Dref;
+  std::vector<double> top((size_t)g.nx * g.ny);
+  for (int j = 0; j < g.ny; j++)
+    for (int i = 0; i < g.nx; i++) top[(size_t)j * g.nx + i] = surf((i - NG) * h, (j - NG) * h);
+  par_for(g.nz, [&](int k) {
+    const double m = (double)(nk - 1) - (k - NG);   // unit depth index below the surface
+    const double frac = depth(m) * h / Dref;        // 0 at the surface, 1 at the bottom
+    for (int j = 0; j < g.ny; j++)
+      for (int i = 0; i < g.nx; i++) {
+        long p = g.at(i, j, k);
+        double zt = top[(size_t)j * g.nx + i];
+        x[p] = (i - NG) * h;
+        y[p] = (j - NG) * h;
+        z[p] = zt - (zt - zbot) * frac;
+      }
+  });
+  return CGFD_OK;
+}
+
+int cgfd_metric_calc(int ni, int nj, int nk, const double *x, const double *y, const double *z, double *metric) {
+  if (ni < 1 || nj < 1 || nk < 1 || !x || !y || !z || !metric) return CGFD_E_ARG;
+  G3 g(ni, nj, nk);
+  const long n = g.n;
+  std::fill(metric, metric + 10 * n, 0.0);
+  par_for(nk, [&](int kk) {
+    const int k = kk + NG;
+    for (int j = NG; j < NG + nj; j++)
+      for (int i = NG; i < NG + ni; i++) {
+        const long p = g.at(i, j, k);
+        double v[3][3];  // v[a][c]: derivative of coordinate c along axis a
+        const long st[3] = {1, g.sy, g.sz};
+        const double *cs[3] = {x, y, z};
+        for (int a = 0; a < 3; a++)
+          for (int c = 0; c < 3; c++) {
+            double s = 0;
+            for (int q = 0; q < 7; q++) s += kFdc[q] * cs[c][p + (q - 3) * st[a]];
+            v[a][c] = s;
+          }
+        auto cross = [](const double *A, const double *B, double *C) {
+          C[0] = A[1] * B[2] - A[2] * B[1];
+          C[1] = A[2] * B[0] - A[0] * B[2];
+          C[2] = A[0] * B[1] - A[1] * B[0];
+        };
+        double c12[3], c23[3], c31[3];
+        cross(v[0], v[1], c12);
+        cross(v[1], v[2], c23);
+        cross(v[2], v[0], c31);
+        const double jac = c12[0] * v[2][0] + c12[1] * v[2][1] + c12[2] * v[2][2];
+        metric[0 * n + p] = jac;
+        for (int c = 0; c < 3; c++) {
+          metric[(1 + c) * n + p] = c23[c] / jac;
+          metric[(4 + c) * n + p] = c31[c] / jac;
+          metric[(7 + c) * n + p] = c12[c] / jac;
+        }
+      }
+  });
+  // ghost extrapolation 2 f(edge) - f(mirror), in the order x1,x2,y1,y2,z1,z2
+  const int ni1 = NG, ni2 = NG + ni - 1, nj1 = NG, nj2 = NG + nj - 1, nk1 = NG, nk2 = NG + nk - 1;
+  for (int c = 0; c < 10; c++) {
+    double *a = metric + (long)c * n;
+    for (int k = 0; k < g.nz; k++)
+      for (int j = 0; j < g.ny; j++) {
+        for (int i = 0; i < ni1; i++) a[g.at(i, j, k)] = 2 * a[g.at(ni1, j, k)] - a[g.at(2 * ni1 - i, j, k)];
+        for (int i = ni2 + 1; i < g.nx; i++) a[g.at(i, j, k)] = 2 * a[g.at(ni2, j, k)] - a[g.at(2 * ni2 - i, j, k)];
+      }
+    for (int k = 0; k < g.nz; k++) {
+      for (int j = 0; j < nj1; j++)
+        for (int i = 0; i < g.nx; i++) a[g.at(i, j, k)] = 2 * a[g.at(i, nj1, k)] - a[g.at(i, 2 * nj1 - j, k)];
+      for (int j = nj2 + 1; j < g.ny; j++)
+        for (int i = 0; i < g.nx; i++) a[g.at(i, j, k)] = 2 * a[g.at(i, nj2, k)] - a[g.at(i, 2 * nj2 - j, k)];
+    }
+    for (int k = 0; k < nk1; k++)
+      for (int j = 0; j < g.ny; j++)
+        for (int i = 0; i < g.nx; i++) a[g.at(i, j, k)] = 2 * a[g.at(i, j, nk1)] - a[g.at(i, j, 2 * nk1 - k)];
+    for (int k = nk2 + 1; k < g.nz; k++)
+      for (int j = 0; j < g.ny; j++)
+        for (int i = 0; i < g.nx; i++) a[g.at(i, j, k)] = 2 * a[g.at(i, j, nk2)] - a[g.at(i, j, 2 * nk2 - k)];
+  }
+  return CGFD_OK;
+}
+
+double cgfd_dt_estimate(int ni, int nj, int nk, const double *x, const double *y, const double *z, const double *lam,
+                        const double *mu, const double *rho, double cfl) {
+  if (ni < 2 || nj < 2 || nk < 2 || !x || !y || !z || !lam || !mu || !rho) return CGFD_E_ARG;
+  G3 g(ni, nj, nk);
+  std::vector<double> best(g.nz, 1e30);
+  par_for(nk - 1, [&](int kk) {
+    const int k = kk + NG;
+    double b = 1e30;
+    for (int j = NG; j < NG + nj - 1; j++)
+      for (int i = NG; i < NG + ni - 1; i++) {
+        const long p = g.at(i, j, k);
+        const double vp = std::sqrt((lam[p] + 2 * mu[p]) / rho[p]);
+        const double p0[3] = {x[p], y[p], z[p]};
+        double Le = 1e30;
+        for (int kc = -1; kc <= 1; kc += 2)
+          for (int jc = -1; jc <= 1; jc += 2)
+            for (int ic = -1; ic <= 1; ic += 2) {
+              const long a = p - ic, bq = p - jc * g.sy, c = p - kc * g.sz;
+              const double p1[3] = {x[a], y[a], z[a]}, p2[3] = {x[bq], y[bq], z[bq]}, p3[3] = {x[c], y[c], z[c]};
+              const double u[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+              const double w[3] = {p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2]};
+              const double nrm[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+              const double nn = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+              const double dist = std::fabs(nrm[0] * (p0[0] - p1[0]) + nrm[1] * (p0[1] - p1[1]) +
+                                            nrm[2] * (p0[2] - p1[2])) / nn;
+              Le = std::min(Le, dist);
+            }
+        b = std::min(b, cfl * Le / vp);
+      }
+    best[k] = b;
+  });
+  return *std::min_element(best.begin(), best.end());
+}
+
+int cgfd_media_homogeneous(int ni, int nj, int nk, double vp, double vs, double rho, double *lam, double *mu,
+                           double *rhoa) {
+  if (ni < 1 || nj < 1 || nk < 1 || !lam || !mu || !rhoa || vp <= 0 || vs < 0 || rho <= 0) return CGFD_E_ARG;
+  G3 g(ni, nj, nk);
+  const double m = rho * vs * vs, l = rho * vp * vp - 2 * m;
+  for (long p = 0; p < g.n; p++) { lam[p] = l; mu[p] = m; rhoa[p] = rho; }
+  return CGFD_OK;
+}
+
+int cgfd_media_layered(int ni, int nj, int nk, const double *x, const double *y, const double *z,
+                       const cgfd_layer_desc *d, double *lam, double *mu, double *rho) {
+  (void)x; (void)y;
+  if (!d || d->nlayer < 1 || !z || !lam || !mu || !rho || !d->vp || !d->vpvs || !d->rho) return CGFD_E_ARG;
+  if (d->nlayer > 1 && !d->z_interface) return CGFD_E_ARG;
+  G3 g(ni, nj, nk);
+  auto layer_of = [&](double zz) {
+    int l = 0;
+    while (l < d->nlayer - 1 && zz < d->z_interface[l]) l++;
+    return l;
+  };
+  std::vector<float> noise, tmp;
+  const bool pert = d->pert_std > 0 && d->corr_len > 0;
+  if (pert) { noise.resize(g.n); tmp.resize(g.n); }
+  // mean grid spacings for the correlation length in cells
+  double hx = std::fabs(x[g.at(NG + 1, NG, NG)] - x[g.at(NG, NG, NG)]);
+  double hy = std::fabs(y[g.at(NG, NG + 1, NG)] - y[g.at(NG, NG, NG)]);
+  double hz = std::fabs(z[g.at(NG, NG, NG + nk - 1)] - z[g.at(NG, NG, NG)]) / std::max(1, nk - 1);
+  for (int l = 0; l < d->nlayer; l++) {
+    if (pert) {
+      // white noise, seeded per (layer, plane) so results do not depend on threading
+      par_for(g.nz, [&](int k) {
+        std::mt19937_64 r(mix(d->seed * 1315423911ull + (uint64_t)l * 2654435761ull + (uint64_t)k));
+        std::normal_distribution<float> N(0.f, 1.f);
+        for (long q = (long)k * g.sz; q < (long)(k + 1) * g.sz; q++) noise[q] = N(r);
+      });
+      // separable Gaussian smoothing: correlation length = 1/e width
+      const double sig[3] = {d->corr_len / std::max(hx, 1e-9) / std::sqrt(2.0),
+                             d->corr_len / std::max(hy, 1e-9) / std::sqrt(2.0),
+                             d->corr_len / std::max(hz, 1e-9) / std::sqrt(2.0)};
+      const int dims[3] = {g.nx, g.ny, g.nz};
+      const long strides[3] = {1, g.sy, g.sz};
+      for (int a = 0; a < 3; a++) {
+        const int R = std::max(1, (int)std::ceil(3 * sig[a]));
+        std::vector<float> w(2 * R + 1);
+        double ws = 0;
+        for (int q = -R; q <= R; q++) ws += (w[q + R] = (float)std::exp(-0.5 * q * q / (sig[a] * sig[a])));
+        for (auto &v : w) v = (float)(v / ws);
+        par_for(g.nz, [&](int k) {
+          for (int j = 0; j < g.ny; j++)
+            for (int i = 0; i < g.nx; i++) {
+              const int pos = a == 0 ? i : a == 1 ? j : k;
+              const long p = g.at(i, j, k);
+              double s = 0;
+              for (int q = -R; q <= R; q++) {
+                int t = pos + q;
+                t = t < 0 ? -t : t >= dims[a] ? 2 * dims[a] - 2 - t : t;  // mirror at the edges
+                t = std::min(std::max(t, 0), dims[a] - 1);
+                s += w[q + R] * noise[p + (long)(t - pos) * strides[a]];
+              }
+              tmp[p] = (float)s;
+            }
+        });
+        noise.swap(tmp);
+      }
+      double s2 = 0;
+      for (long p = 0; p < g.n; p++) s2 += (double)noise[p] * noise[p];
+      const float inv = (float)(1.0 / std::sqrt(s2 / g.n + 1e-300));
+      for (long p = 0; p < g.n; p++) noise[p] *= inv;
+    }
+    par_for(g.nz, [&](int k) {
+      for (long p = (long)k * g.sz; p < (long)(k + 1) * g.sz; p++) {
+        if (layer_of(z[p]) != l) continue;
+        double f = 1.0;
+        if (pert) f = 1.0 + std::max(-d->pert_clip, std::min(d->pert_clip, d->pert_std * (double)noise[p]));
+        const double vp = d->vp[l] * f, vs = vp / d->vpvs[l];
+        const double r = d->rho[l] > 0 ? d->rho[l] : 310.0 * std::pow(vp, 0.25);
+        rho[p] = r;
+        mu[p] = r * vs * vs;
+        lam[p] = r * vp * vp - 2 * mu[p];
+      }
+    });
+  }
+  return CGFD_OK;
+}
+
+int cgfd_rho_to_slow(long long n, const double *rho, double *slw) {
+  if (n < 0 || !rho || !slw) return CGFD_E_ARG;
+  for (long long p = 0; p < n; p++) slw[p] = rho[p] > 1e-10 ? 1.0 / rho[p] : 0.0;
+  return CGFD_OK;
+}
+
+int cgfd_pml_len_dh(int ni, int nj, int nk, const double *x, const double *y, const double *z, int face, int nlay,
+                    double *L0, double *dh) {
+  if (face < 0 || face > 5 || nlay < 1 || !x || !y || !z || !L0 || !dh) return CGFD_E_ARG;
+  G3 g(ni, nj, nk);
+  int i1 = NG, i2 = NG + ni - 1, j1 = NG, j2 = NG + nj - 1, k1 = NG, k2 = NG + nk - 1;
+  switch (face) {
+    case 0: i2 = i1 + nlay; break;
+    case 1: i1 = i2 - nlay; break;
+    case 2: j2 = j1 + nlay; break;
+    case 3: j1 = j2 - nlay; break;
+    case 4: k2 = k1 + nlay; break;
+    default: k1 = k2 - nlay; break;
+  }
+  const int dim = face / 2;
+  const long st = dim == 0 ? 1 : dim == 1 ? g.sy : g.sz;
+  double L = 0;
+  long num = 0;
+  for (int k = k1; k <= k2 - (dim == 2); k++)
+    for (int j = j1; j <= j2 - (dim == 1); j++)
+      for (int i = i1; i <= i2 - (dim == 0); i++) {
+        const long p = g.at(i, j, k), q = p + st;
+        L += std::sqrt((x[q] - x[p]) * (x[q] - x[p]) + (y[q] - y[p]) * (y[q] - y[p]) + (z[q] - z[p]) * (z[q] - z[p]));
+        num++;
+      }
+  *dh = L / (double)num;
+  *L0 = *dh * nlay;
+  return CGFD_OK;
+}
+
+int cgfd_pml_profile(int nlay, int face, double L0, double dh, double vel, double amax, double bmax, double *A,
+                     double *B, double *D) {
+  if (nlay < 1 || face < 0 || face > 5 || L0 <= 0 || dh <= 0 || !A || !B || !D) return CGFD_E_ARG;
+  const int np = nlay + 1;
+  const double R = std::pow(10.0, -((std::log10((double)nlay) - 1.0) / std::log10(2.0) + 4.0));
+  const double d0 = -vel / (2.0 * L0) * std::log(R) * 3.0;
+  const bool low = (face % 2) == 0;
+  for (int n = 0; n < np; n++) {
+    const double r = n * dh / L0;
+    const int i = low ? np - 1 - n : n;
+    const double d = d0 * r * r, a = amax * (1.0 - r), b = 1.0 + (bmax - 1.0) * r * r;
+    D[i] = d / b;
+    A[i] = a + d / b;
+    B[i] = 1.0 / b;
+  }
+  return CGFD_OK;
+}
+
+int cgfd_free_surface_matrices(int ni, int nj, int nk, const double *metric, const double *lam, const double *mu,
+                               double *mvx, double *mvy) {
+  if (!metric || !lam || !mu || !mvx || !mvy) return CGFD_E_ARG;
+  G3 g(ni, nj, nk);
+  const long n = g.n;
+  std::fill(mvx, mvx + (long)g.nx * g.ny * 9, 0.0);
+  std::fill(mvy, mvy + (long)g.nx * g.ny * 9, 0.0);
+  const int k = NG + nk - 1;
+  for (int j = NG; j < NG + nj; j++)
+    for (int i = NG; i < NG + ni; i++) {
+      const long p = g.at(i, j, k);
+      double e[3][3];
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) e[r][c] = metric[(1 + 3 * r + c) * n + p];
+      const double l = lam[p], m = mu[p], l2m = l + 2 * m;
+      // zero traction on the zeta = const surface: sigma . grad(zeta) = 0 with
+      // dV/dzeta unknown; A dV/dzeta = B dV/dxi + C dV/deta
+      const double *z3 = e[2];
+      double A[3][3], B[3][3], Cm[3][3];
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+          const double zz = z3[0] * z3[0] + z3[1] * z3[1] + z3[2] * z3[2];
+          A[r][c] = l * z3[r] * z3[c] + m * z3[c] * z3[r] + (r == c ? m * zz : 0.0);
+          const double *x1 = e[0], *x2 = e[1];
+          const double dx1 = z3[0] * x1[0] + z3[1] * x1[1] + z3[2] * x1[2];
+          const double dx2 = z3[0] * x2[0] + z3[1] * x2[1] + z3[2] * x2[2];
+          B[r][c] = -(l * z3[r] * x1[c] + m * z3[c] * x1[r] + (r == c ? m * dx1 : 0.0));
+          Cm[r][c] = -(l * z3[r] * x2[c] + m * z3[c] * x2[r] + (r == c ? m * dx2 : 0.0));
+        }
+      (void)l2m;
+      // inverse of A (cofactors, FP64)
+      double inv[3][3];
+      inv[0][0] = A[1][1] * A[2][2] - A[2][1] * A[1][2];
+      inv[0][1] = A[2][1] * A[0][2] - A[0][1] * A[2][2];
+      inv[0][2] = A[0][1] * A[1][2] - A[0][2] * A[1][1];
+      inv[1][0] = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+      inv[1][1] = A[0][0] * A[2][2] - A[2][0] * A[0][2];
+      inv[1][2] = A[1][0] * A[0][2] - A[0][0] * A[1][2];
+      inv[2][0] = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+      inv[2][1] = A[2][0] * A[0][1] - A[0][0] * A[2][1];
+      inv[2][2] = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+      const double det = inv[0][0] * A[0][0] + inv[0][1] * A[1][0] + inv[0][2] * A[2][0];
+      if (!(std::fabs(det) > 0)) return CGFD_E_ARG;
+      const long ij = ((long)j * g.nx + i) * 9;
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+          double sb = 0, sc = 0;
+          for (int q = 0; q < 3; q++) { sb += inv[r][q] * B[q][c]; sc += inv[r][q] * Cm[q][c]; }
+          mvx[ij + r * 3 + c] = sb / det;
+          mvy[ij + r * 3 + c] = sc / det;
+        }
+    }
+  return CGFD_OK;
+}
+
+double cgfd_wavelet(int kind, double t, double c0, double c1) {
+  const double tau = t - c1;
+  switch (kind) {
+    case CGFD_WAVELET_RICKER: {
+      const double u = 2 * M_PI * c0 * tau;
+      return (1 - u * u / 2) * std::exp(-u * u / 4);
+    }
+    case CGFD_WAVELET_GAUSSIAN:
+      return std::exp(-tau * tau / (c0 * c0)) / (std::sqrt(M_PI) * c0);
+    case CGFD_WAVELET_RICKER_DERIV: {
+      const double u = 2 * M_PI * c0 * tau;
+      return u * (-3 + 0.5 * u * u) * std::exp(-u * u / 4) * M_PI * c0;
+    }
+    case CGFD_WAVELET_GAUSSIAN_DERIV:
+      return std::exp(-tau * tau / (c0 * c0)) / (std::sqrt(M_PI) * c0) * (-2 * tau / (c0 * c0));
+    default:
+      return 0.0;
+  }
+}
+
+int cgfd_angle2moment(double strike, double dip, double rake, double *m6) {
+  if (!m6) return CGFD_E_ARG;
+  const double s = strike * M_PI / 180, d = dip * M_PI / 180, r = rake * M_PI / 180;
+  // Aki & Richards (x north, y east, z down)
+  const double M11 = -(std::sin(d) * std::cos(r) * std::sin(2 * s) + std::sin(2 * d) * std::sin(r) * std::sin(s) * std::sin(s));
+  const double M22 = std::sin(d) * std::cos(r) * std::sin(2 * s) - std::sin(2 * d) * std::sin(r) * std::cos(s) * std::cos(s);
+  const double M33 = -(M11 + M22);
+  const double M12 = std::sin(d) * std::cos(r) * std::cos(2 * s) + 0.5 * std::sin(2 * d) * std::sin(r) * std::sin(2 * s);
+  const double M13 = -(std::cos(d) * std::cos(r) * std::cos(s) + std::cos(2 * d) * std::sin(r) * std::sin(s));
+  const double M23 = -(std::cos(d) * std::cos(r) * std::sin(s) - std::cos(2 * d) * std::sin(r) * std::cos(s));
+  // to x east, y north, z up
+  m6[0] = M22;
+  m6[1] = M11;
+  m6[2] = M33;
+  m6[3] = -M13;
+  m6[4] = -M23;
+  m6[5] = M12;
+  return CGFD_OK;
+}
+
+int cgfd_source_point(int ni, int nj, int nk, double sx, double sy, double sz, int depth_flag, double t0, double dt,
+                      double wavelet_tstart, int wavelet, double c0, double c1, double stf_length, int *it_begin,
+                      int *it_end, int *ext_num, long long *ext_indx, double *ext_coef, int *max_nt_out,
+                      double *stf) {
+  if (ni < 1 || nj < 1 || nk < 1 || dt <= 0 || !it_begin || !it_end || !ext_num || !ext_indx || !ext_coef ||
+      !max_nt_out)
+    return CGFD_E_ARG;
+  G3 g(ni, nj, nk);
+  if (depth_flag == 1) sz = (double)(nk - 1) - sz;
+  const int si = (int)(sx + 0.5), sj = (int)(sy + 0.5), sk = (int)(sz + 0.5);
+  if (si < 0 || si > ni - 1 || sj < 0 || sj > nj - 1 || sk < 0 || sk > nk - 1) return CGFD_E_ARG;
+  const double ox = sx - si, oy = sy - sj, oz = sz - sk;
+  const int half = 3;
+  const double wid = half / 2.0;
+  double w[343], sum = 0;
+  int q = 0;
+  for (int k = -half; k <= half; k++)
+    for (int j = -half; j <= half; j++)
+      for (int i = -half; i <= half; i++) {
+        auto gs = [&](double t) { return std::exp(-t * t / (wid * wid)) / (std::sqrt(M_PI) * wid); };
+        w[q] = gs(i - ox) * gs(j - oy) * gs(k - oz);
+        sum += w[q++];
+      }
+  int ne = 0;
+  for (int k = -half; k <= half; k++)
+    for (int j = -half; j <= half; j++)
+      for (int i = -half; i <= half; i++) {
+        const int gi = si + i, gj = sj + j, gk = sk + k;
+        if (gi < 0 || gi > ni - 1 || gj < 0 || gj > nj - 1 || gk < 0 || gk > nk - 1) continue;
+        const int c = (i + half) + 7 * (j + half) + 49 * (k + half);
+        ext_indx[ne] = g.at(gi + NG, gj + NG, gk + NG);
+        ext_coef[ne] = w[c] / sum;
+        ne++;
+      }
+  *ext_num = ne;
+  const int max_nt = (int)(stf_length / dt + 0.5);
+  *max_nt_out = max_nt;
+  const int ib = (int)((wavelet_tstart - t0) / dt);
+  *it_begin = ib;
+  *it_end = ib + max_nt - 1;
+  if (stf) {
+    const double frac[4] = {0.0, 0.5, 0.5, 1.0};
+    const double t_shift = wavelet_tstart - (ib * dt + t0);
+    for (int it = 0; it < max_nt; it++)
+      for (int s = 0; s < 4; s++) stf[it * 4 + s] = cgfd_wavelet(wavelet, it * dt + frac[s] * dt - t_shift, c0, c1);
+  }
+  return CGFD_OK;
+}
+
+int cgfd_receiver_index(int ni, int nj, int nk, double rx, double ry, double rz, int depth_flag, long long *indx8,
+                        double *frac3) {
+  if (!indx8 || !frac3) return CGFD_E_ARG;
+  G3 g(ni, nj, nk);
+  rx -= 1;
+  ry -= 1;
+  if (depth_flag == 1) rz = (double)(nk - 1) - rz;
+  else rz -= 1;
+  const int ix = (int)std::floor(rx), iy = (int)std::floor(ry), iz = (int)std::floor(rz);
+  if (ix < 0 || ix > ni - 1 || iy < 0 || iy > nj - 1 || iz < 0 || iz > nk - 1) return CGFD_E_ARG;
+  const long p = g.at(ix + NG, iy + NG, iz + NG);
+  indx8[0] = p;
+  indx8[1] = p + 1;
+  indx8[2] = p + g.sy;
+  indx8[3] = p + 1 + g.sy;
+  indx8[4] = p + g.sz;
+  indx8[5] = p + 1 + g.sz;
+  indx8[6] = p + g.sy + g.sz;
+  indx8[7] = p + 1 + g.sy + g.sz;
+  frac3[0] = rx - ix;
+  frac3[1] = ry - iy;
+  frac3[2] = rz - iz;
+  return CGFD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,145 @@
+/*
+ * cgfd_setup.h -- host-side preparation of the inputs of the el_iso path.
+ *
+ * These functions replace the reference's setup code that feeds
+ * drv_rk_curv_col_allstep: grid generation and metrics (gd_t), media
+ * (md_t / src/media), CFS-PML profiles (bdry_t), free-surface conversion
+ * matrices (sv_curv_col_el_iso_dvh2dvz), source tables (src_t) and receiver
+ * location (io_funcs).  All arithmetic is FP64; arrays are double and use the
+ * reference host layout described in cgfd_el_iso.h (nx = ni+6, i fastest).
+ * Return values: CGFD_OK (0) or a negative CGFD_E_* code (cgfd_el_iso.h).
+ */
+#ifndef CGFD_SETUP_H
+#define CGFD_SETUP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- grids --------------------------------------------------------------- */
+
+/* Cartesian grid with ghosts, x0/y0/z0 = coordinate of physical point 0
+ * (gd_curv_gen_cart, src/forward/gd_t.cu:599-630). */
+int cgfd_grid_cartesian(int ni, int nj, int nk, double dx, double x0, double dy, double y0, double dz, double z0,
+                        double *x, double *y, double *z);
+
+/* Curvilinear grid under a topographic free surface (no reference
+ * counterpart: the reference imports such grids).  Surface
+ * z(x,y) = sum of nhill Gaussian hills + a Fourier perturbation of given rms
+ * made of modes with wavelength >= min_wavelength; a flat bottom; vertical
+ * spacing stretched linearly from 1 (top) to stretch (bottom) times dh.
+ * Hill centres/amplitudes/widths and the Fourier modes come from seed. */
+typedef struct {
+  int ni, nj, nk;
+  double dh;               /* horizontal spacing and top vertical spacing [m] */
+  double stretch;          /* bottom/top vertical spacing ratio (e.g. 1.5) */
+  int nhill;               /* Gaussian hills */
+  double amp_min, amp_max; /* hill amplitudes [m] */
+  double sig_min, sig_max; /* hill widths (sigma) [m] */
+  double rough_rms;        /* rms of the Fourier perturbation [m] */
+  double min_wavelength;   /* shortest perturbation wavelength [m] */
+  unsigned long long seed;
+} cgfd_topo_desc;
+
+int cgfd_grid_topography(const cgfd_topo_desc *d, double *x, double *y, double *z);
+
+/* Metrics and Jacobian with the centred 7-point operator and the
+ * geometric ghost extrapolation, in FP64 (gd_curv_metric_cal +
+ * geometric_symmetry, src/forward/gd_t.cu:209-314, 404-498).
+ * metric: 10 components jac, xi_x..zeta_z. */
+int cgfd_metric_calc(int ni, int nj, int nk, const double *x, const double *y, const double *z, double *metric);
+
+/* Largest stable dt: min over cells of CFL * (distance to the 8 corner
+ * planes) / Vp (blk_dt_esti_curv, src/forward/blk_t.cu:673-763). rho is the
+ * density (not yet inverted).  Returns dtmax > 0, or a negative error code. */
+double cgfd_dt_estimate(int ni, int nj, int nk, const double *x, const double *y, const double *z,
+                        const double *lam, const double *mu, const double *rho, double cfl);
+
+/* ---- media ----------------------------------------------------------------- */
+
+/* Flat layers below interfaces at absolute depths z_top_of_layer[l] (layer 0
+ * is the top layer, unbounded above); Vs = Vp/vpvs; rho from Gardner
+ * rho = 310*Vp^0.25 (kg/m^3) when rho_l <= 0; each layer gets a Gaussian-
+ * correlated relative Vp/Vs perturbation of std pert_std clipped at
+ * +-pert_clip, correlation length corr_len [m], seeded per layer.  Values
+ * are set at every point, ghosts included. */
+typedef struct {
+  int nlayer;
+  const double *z_interface; /* nlayer-1 interface depths (z, decreasing) */
+  const double *vp;          /* nlayer */
+  const double *vpvs;        /* nlayer, Vp/Vs ratio */
+  const double *rho;         /* nlayer, <= 0 selects Gardner */
+  double pert_std, pert_clip, corr_len;
+  unsigned long long seed;
+} cgfd_layer_desc;
+
+int cgfd_media_layered(int ni, int nj, int nk, const double *x, const double *y, const double *z,
+                       const cgfd_layer_desc *d, double *lam, double *mu, double *rho);
+
+/* Homogeneous medium at every point. */
+int cgfd_media_homogeneous(int ni, int nj, int nk, double vp, double vs, double rho, double *lam, double *mu,
+                           double *rhoa);
+
+/* rho -> 1/rho (0 where rho <= 1e-10) (md_rho_to_slow, src/forward/md_t.cu:622-637). */
+int cgfd_rho_to_slow(long long n, const double *rho, double *slw);
+
+/* ---- boundaries -------------------------------------------------------------- */
+
+/* Mean spacing dh and layer thickness L0 = dh*nlay across the box of PML
+ * face `face` (0..5 = x1,x2,y1,y2,z1,z2) with nlay layers
+ * (bdry_cal_abl_len_dh, src/forward/bdry_t.cu:334-451). */
+int cgfd_pml_len_dh(int ni, int nj, int nk, const double *x, const double *y, const double *z, int face, int nlay,
+                    double *L0, double *dh);
+
+/* CFS-PML profile of one face: A = alpha + d/beta, B = 1/beta, D = d/beta
+ * with d = d0 (r)^2, alpha = amax (1-r), beta = 1 + (bmax-1) r^2, r = L/L0,
+ * d0 = -3 vel ln R / (2 L0), R = 10^-((log10 nlay - 1)/log10 2 + 4); ordered
+ * from low to high index (bdry_pml_set, src/forward/bdry_t.cu:193-241, 297-330). */
+int cgfd_pml_profile(int nlay, int face, double L0, double dh, double vel, double amax, double bmax, double *A,
+                     double *B, double *D);
+
+/* Zero-traction velocity-gradient conversion at the top surface:
+ * matVx2Vz = A^-1 B, matVy2Vz = A^-1 C per (i,j), nx*ny*9 values
+ * (sv_curv_col_el_iso_dvh2dvz, src/forward/sv_curv_col_el_iso_gpu.cu:1152-1261). */
+int cgfd_free_surface_matrices(int ni, int nj, int nk, const double *metric, const double *lam, const double *mu,
+                               double *matVx2Vz, double *matVy2Vz);
+
+/* ---- sources --------------------------------------------------------------------- */
+
+#define CGFD_WAVELET_RICKER 0
+#define CGFD_WAVELET_GAUSSIAN 1
+#define CGFD_WAVELET_RICKER_DERIV 2
+#define CGFD_WAVELET_GAUSSIAN_DERIV 3
+
+/* Wavelet value (fun_ricker & co, src/forward/src_t.cu:834-878). */
+double cgfd_wavelet(int kind, double t, double c0, double c1);
+
+/* Moment tensor (Mxx,Myy,Mzz,Myz,Mxz,Mxy; x east, y north, z up) of a unit
+ * double couple (angle2moment, src/forward/src_t.cu:910-950). */
+int cgfd_angle2moment(double strike, double dip, double rake, double *m6);
+
+/* One point source given by 0-based global grid index (sx,sy,sz); depth_flag
+ * = 1 counts sz down from the surface (k = nk-1-sz).  Fills the spreading
+ * (Gaussian of width 1.5 over 7^3 points, normalised before clipping to the
+ * physical grid) and the per-(step, RK stage) wavelet table, as
+ * src_read_locate_file does for index input and named wavelets
+ * (src/forward/src_t.cu:296-330, 596-712, 796-827).
+ * ext_indx/ext_coef hold 343 entries; stf holds max_nt*4 values (may be NULL). */
+int cgfd_source_point(int ni, int nj, int nk, double sx, double sy, double sz, int depth_flag, double t0, double dt,
+                      double wavelet_tstart, int wavelet, double c0, double c1, double stf_length, int *it_begin,
+                      int *it_end, int *ext_num, long long *ext_indx, double *ext_coef, int *max_nt,
+                      double *stf);
+
+/* ---- receivers ------------------------------------------------------------------- */
+
+/* Receiver given by grid index as in a station file (io_funcs.cu:162-251):
+ * x,y 1-based; z depth below the surface (depth_flag 1) or 1-based index.
+ * Fills the 8 trilinear corner indices and the fractions; returns
+ * CGFD_E_ARG when the point lies outside the grid. */
+int cgfd_receiver_index(int ni, int nj, int nk, double rx, double ry, double rz, int depth_flag, long long *indx8,
+                        double *frac3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -42,6 +42,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <chrono>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -540,6 +541,8 @@ template <typename R> void recv_keep(const Model<R> &M, const R *w, int it, int 
   }
 }
 
+double g_loop_seconds = 0.0;
+
 template <typename R>
 int run(const orc_problem *p, double dt_in, int nt_total, const double *init_wave, const double *init_aux,
         double *seismo, double *final_wave, double *final_aux) {
@@ -566,6 +569,7 @@ int run(const orc_problem *p, double dt_in, int nt_total, const double *init_wav
   R *a_pre[6], *a_tmp[6], *a_rhs[6], *a_end[6], *a_cur[6];
   for (int f = 0; f < 6; f++) { a_pre[f] = apre[f].data(); a_tmp[f] = atmp[f].data(); a_rhs[f] = arhs[f].data(); a_end[f] = aend[f].data(); }
   const R dt = (R)dt_in;
+  const auto t_loop0 = std::chrono::steady_clock::now();
   for (int it = 0; it < nt_total; it++) {
     if (seismo) recv_keep(M, pre, it, nt_total, seismo);
     const int ipair = it % 8;
@@ -594,6 +598,7 @@ int run(const orc_problem *p, double dt_in, int nt_total, const double *init_wav
     std::swap(pre, end);
     for (int f = 0; f < 6; f++) std::swap(a_pre[f], a_end[f]);
   }
+  g_loop_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_loop0).count();
   if (final_wave)
     for (size_t i = 0; i < lev; i++) final_wave[i] = (double)pre[i];
   if (final_aux) {
@@ -806,6 +811,9 @@ template <typename R> void free_matrices(int ni, int nj, int nk, const double *m
 extern "C" {
 
 int orc_version(void) { return 1; }
+
+/* wall seconds of the time loop (setup excluded) of the last orc_run */
+double orc_last_loop_seconds(void) { return g_loop_seconds; }
 
 void orc_set_threads(int n) {
 #ifdef _OPENMP

@@ -67,6 +67,7 @@ def lib():
         L.orc_rho_to_slow.argtypes = [C.c_size_t, vp, vp]
         L.orc_set_threads.argtypes = [i]
         L.orc_max_threads.restype = i
+        L.orc_last_loop_seconds.restype = d
         _lib = L
     return _lib
 
@@ -77,3 +78,39 @@ def p(a):
 
 def f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def problem_struct(m):
+    """OrcProblem for a workload dict of gpu-cgfd3d-elastic_amd/workloads.py (keeps its arrays alive)."""
+    keep = []
+
+    def k(a, dt=np.float64):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return p(a)
+
+    P = OrcProblem()
+    P.ni, P.nj, P.nk = m["ni"], m["nj"], m["nk"]
+    P.metric = k(m["metric"])
+    P.lam, P.mu, P.slw = k(m["lam"]), k(m["mu"]), k(m["slw"])
+    P.free_top = 1
+    P.matVx2Vz, P.matVy2Vz = k(m["mvx"]), k(m["mvy"])
+    for f, (nl, A, B, D) in m["pml"].items():
+        P.pml_on[f] = 1
+        P.pml_nlay[f] = nl
+        P.pml_A[f], P.pml_B[f], P.pml_D[f] = k(A), k(B), k(D)
+    s = m["src"]
+    P.nsrc, P.force_on, P.moment_on, P.max_nt, P.max_ext = s["nsrc"], s["force_on"], s["moment_on"], s["max_nt"], \
+        s["max_ext"]
+    P.src_it_begin = k(s["it_begin"], np.int32)
+    P.src_it_end = k(s["it_end"], np.int32)
+    P.src_ext_num = k(s["ext_num"], np.int32)
+    P.src_ext_indx = k(s["ext_indx"], np.int32)
+    P.src_ext_coef = k(s["ext_coef"])
+    P.src_M = k(s["M"]) if s["M"] is not None else None
+    P.src_F = k(s["F"]) if s["F"] is not None else None
+    P.nrecv = len(m["recv_indx"])
+    P.recv_indx = k(m["recv_indx"], np.int64)
+    P.recv_frac = k(m["recv_frac"])
+    P._keep = keep
+    return P
