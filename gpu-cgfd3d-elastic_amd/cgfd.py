@@ -26,6 +26,7 @@ EXPORTS = [
     "cgfd_upload_receivers", "cgfd_set_level", "cgfd_reset_shot", "cgfd_run",
     "cgfd_el_iso_onestage", "cgfd_read_seismo", "cgfd_fetch_level",
     "cgfd_fetch_pml_aux", "cgfd_set_timing", "cgfd_last_run_timing",
+    "cgfd_set_tile", "cgfd_get_tile",
     "cgfd_grid_cartesian", "cgfd_grid_topography", "cgfd_metric_calc", "cgfd_dt_estimate",
     "cgfd_media_layered", "cgfd_media_homogeneous", "cgfd_rho_to_slow", "cgfd_pml_len_dh",
     "cgfd_pml_profile", "cgfd_free_surface_matrices", "cgfd_wavelet", "cgfd_angle2moment",
@@ -95,6 +96,8 @@ def load(path=LIB_PATH):
         "cgfd_fetch_level": ([vp, i, i, i, vp], i),
         "cgfd_fetch_pml_aux": ([vp, i, i, i, i, vp], i),
         "cgfd_set_timing": ([vp, i], i),
+        "cgfd_set_tile": ([vp, i, i], i),
+        "cgfd_get_tile": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)], i),
         "cgfd_last_run_timing": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)], i),
     }
     d = C.c_double
@@ -255,6 +258,14 @@ class Session:
         _check(_lib.cgfd_fetch_pml_aux(self.h, shot, face, level, DT_F64 if dtype == np.float64 else DT_F32,
                                        _ptr(out)))
         return out
+
+    def set_tile(self, shape, zchunk=48):
+        _check(_lib.cgfd_set_tile(self.h, shape, zchunk))
+
+    def get_tile(self):
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(_lib.cgfd_get_tile(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return dict(shape=a.value, tx=b.value, ty=c.value, zchunk=d.value)
 
     def set_timing(self, on):
         _check(_lib.cgfd_set_timing(self.h, 1 if on else 0))

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/cgfd_el_iso.h"
@@ -174,6 +175,9 @@ struct cgfd_session {
   std::vector<long> seismo_off;  // per shot
   bool rec_dirty = false;
   int *clock = nullptr, *shot_start = nullptr, *shot_active = nullptr;
+  // interior tile choice (-1: not tuned yet)
+  int tile_shape = -1, tile_zchunk = 48;
+  bool tile_fixed = false;
   // timing
   bool timing = true;
   std::vector<hipEvent_t> ev;
@@ -504,8 +508,7 @@ Shell make_shell(cgfd_session *s) {
   if (!sh.has_interior) {
     boxes.push_back({0, ni - 1, 0, nj - 1, 0, nk - 1});
   } else {
-    int zc = 48;
-    sh.ib = InteriorBox{ib0, ib1, jb0, jb1, kb0, kb1, zc};
+    sh.ib = InteriorBox{ib0, ib1, jb0, jb1, kb0, kb1, s->tile_zchunk, s->tile_shape < 0 ? 0 : s->tile_shape};
     auto add = [&](int a0, int a1, int b0, int b1, int c0, int c1) {
       if (a1 >= a0 && b1 >= b0 && c1 >= c0) boxes.push_back({a0, a1, b0, b1, c0, c1});
     };
@@ -526,7 +529,77 @@ Shell make_shell(cgfd_session *s) {
   return sh;
 }
 
+// Tile autotuner: time one stage-0 interior sweep per (shape, z chunk) and
+// keep the fastest.  Stage 0 only writes the TMPA and END levels, which every
+// step rewrites, so tuning between steps leaves the state intact.  Results
+// are cached per (grid, storage, shots) for the life of the process.
+struct TileKey {
+  int ni, nj, nk, storage, nshots;
+  bool operator<(const TileKey &o) const {
+    return std::tie(ni, nj, nk, storage, nshots) < std::tie(o.ni, o.nj, o.nk, o.storage, o.nshots);
+  }
+};
+std::map<TileKey, std::pair<int, int>> g_tile_cache;
+
+template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
+  const TileKey key{s->d.ni, s->d.nj, s->d.nk, s->d.storage, s->d.nshots};
+  auto hit = g_tile_cache.find(key);
+  if (hit != g_tile_cache.end()) {
+    s->tile_shape = hit->second.first;
+    s->tile_zchunk = hit->second.second;
+    return CGFD_OK;
+  }
+  s->tile_shape = 0;
+  Shell sh = make_shell(s);
+  if (!sh.has_interior) {
+    g_tile_cache[key] = {s->tile_shape, s->tile_zchunk};
+    return CGFD_OK;
+  }
+  StageArgs<TS, TC> a = stage_args<TS, TC>(s, s->slot[0], s->slot[0], s->slot[1], s->slot[3], false);
+  a.mode = MODE_S0;
+  a.ca = TC(0);
+  a.cb = TC(0);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int nz = sh.ib.k1 - sh.ib.k0 + 1;
+  const int zcands[3] = {24, 48, 96};
+  float best = 1e30f;
+  int bs = 0, bz = 48;
+  for (int shape = 0; shape < interior_num_shapes(); shape++) {
+    for (int zi = 0; zi < 3; zi++) {
+      int zc = std::min(zcands[zi], nz);
+      if (zi > 0 && zc == std::min(zcands[zi - 1], nz)) continue;
+      InteriorBox ib = sh.ib;
+      ib.shape = shape;
+      ib.zchunk = zc;
+      float tmin = 1e30f;
+      for (int rep = 0; rep < 3; rep++) {
+        HIPCHK(hipEventRecord(e0, s->st));
+        hipError_t e = launch_interior<TS, TC>(a, ib, 0, 0, 0, s->d.nshots, s->st);
+        if (e != hipSuccess) return fail(CGFD_E_HIP, "autotune launch: %s", hipGetErrorString(e));
+        HIPCHK(hipEventRecord(e1, s->st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (rep > 0) tmin = std::min(tmin, ms);
+      }
+      if (tmin < best) { best = tmin; bs = shape; bz = zc; }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  s->tile_shape = bs;
+  s->tile_zchunk = bz;
+  g_tile_cache[key] = {bs, bz};
+  return CGFD_OK;
+}
+
 template <typename TS, typename TC> int run_t(cgfd_session *s, int it0, int nsteps, double dt_in) {
+  if (!s->tile_fixed && s->tile_shape < 0) {
+    int r = autotune_t<TS, TC>(s);
+    if (r) return r;
+  }
   if (s->src_dirty) { int r = build_sources<TC>(s); if (r) return r; }
   if (s->rec_dirty) { int r = build_receivers<TC>(s); if (r) return r; }
   const Shell sh = make_shell(s);
@@ -1025,6 +1098,31 @@ int cgfd_fetch_pml_aux(cgfd_session *s, int shot, int face, int level, int dtype
     if (dtype == CGFD_DT_F64) ((double *)out)[q] = v[q];
     else ((float *)out)[q] = (float)v[q];
   }
+  return CGFD_OK;
+}
+
+int cgfd_set_tile(cgfd_session *s, int shape, int zchunk) {
+  if (!s) return fail(CGFD_E_ARG, "null session");
+  if (shape < 0) {  // back to automatic selection
+    s->tile_fixed = false;
+    s->tile_shape = -1;
+    return CGFD_OK;
+  }
+  if (shape >= interior_num_shapes() || zchunk < 1) return fail(CGFD_E_ARG, "tile %d/%d invalid", shape, zchunk);
+  s->tile_fixed = true;
+  s->tile_shape = shape;
+  s->tile_zchunk = zchunk;
+  return CGFD_OK;
+}
+
+int cgfd_get_tile(cgfd_session *s, int *shape, int *tx, int *ty, int *zchunk) {
+  if (!s) return fail(CGFD_E_ARG, "null session");
+  int sh = s->tile_shape < 0 ? 0 : s->tile_shape, a = 0, b = 0;
+  interior_shape(sh, &a, &b);
+  if (shape) *shape = s->tile_shape;
+  if (tx) *tx = a;
+  if (ty) *ty = b;
+  if (zchunk) *zchunk = s->tile_zchunk;
   return CGFD_OK;
 }
 

@@ -39,6 +39,16 @@ template <> struct St<__hip_bfloat16> {
   __device__ __forceinline__ static void st(__hip_bfloat16 *p, float v) { *p = __float2bfloat16(v); }
 };
 
+// Load/store through a uniform base pointer and a 32-bit byte offset: the
+// compiler then addresses with a scalar base plus one 32-bit VGPR offset
+// (valid while a component stays below 4 GiB).
+template <typename T> __device__ __forceinline__ const T &at_b(const T *base, unsigned byte_off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+template <typename T> __device__ __forceinline__ T &at_b(T *base, unsigned byte_off) {
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off);
+}
+
 // ---- operator constants (src/forward/fd_t.cu:76-148) ----------------------
 // DIR 0 = backward {-3,-2,-1,0,1}, DIR 1 = forward {-1,0,1,2,3}.
 template <typename TC, int DIR> struct Drp;

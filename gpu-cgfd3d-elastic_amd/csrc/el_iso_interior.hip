@@ -12,6 +12,11 @@
 // (src/forward/wav_t.cu:362-378, drv_rk_curv_col.cu:339-491) is fused into
 // the epilogue so the RHS never goes to HBM.
 //
+// Per plane, every global load a thread needs (the queue's leading plane,
+// its share of the halo, the 12 coefficients and the RK levels it updates)
+// is issued before the first barrier, so one memory latency is exposed per
+// plane instead of one per dependent load.
+//
 // Only points that are interior in the reference's sense are handed to this
 // kernel: outside every PML face box and at least 4 rows below a free
 // surface.  The boundary shell goes to the per-point kernel.
@@ -20,15 +25,18 @@
 
 namespace cgfd {
 
-template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void k_interior(StageArgs<TS, TC> a, InteriorBox b, int ntx, int nty,
-                                                     int nzc) {
+template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINB>
+__global__ __launch_bounds__(TX *TY, MINB) void k_interior(StageArgs<TS, TC> a, InteriorBox b, int ntx, int nty,
+                                                           int nzc) {
   constexpr int NT = TX * TY;
-  constexpr int HXL = (DX == 0) ? 3 : 1, HXR = 4 - HXL;
-  constexpr int HYL = (DY == 0) ? 3 : 1, HYR = 4 - HYL;
+  constexpr int HXL = (DX == 0) ? 3 : 1;
+  constexpr int HYL = (DY == 0) ? 3 : 1;
   constexpr int LW = TX + 4, LH = TY + 4;
   constexpr int zlo = (DZ == 0) ? -3 : -1;
-  constexpr int zc0 = -zlo;  // queue slot of the centre plane
+  constexpr int zc0 = -zlo;                       // queue slot of the centre plane
+  constexpr int NXH = 4 * TY, NYH = 4 * TX, NH = NXH + NYH;  // halo cells per component
+  constexpr int NHT = kNC * NH;                   // halo cells per plane
+  constexpr int NPER = (NHT + NT - 1) / NT;       // halo loads per thread
   __shared__ TC lds[kNC][LH][LW];
 
   // XCD-aware remap: consecutive hardware workgroup ids land on different
@@ -45,7 +53,7 @@ __global__ __launch_bounds__(TX *TY) void k_interior(StageArgs<TS, TC> a, Interi
   if (a.shot_active && !a.shot_active[shot]) return;
 
   const Layout &L = a.L;
-  const long cs = L.cs, px = L.px, pz = L.pz;
+  const long cs = L.cs;
   const int tid = threadIdx.x;
   const int lx = tid % TX, ly = tid / TX;
   const int i0 = b.i0 + tx_t * TX, j0 = b.j0 + ty_t * TY;
@@ -54,115 +62,156 @@ __global__ __launch_bounds__(TX *TY) void k_interior(StageArgs<TS, TC> a, Interi
   int kend = kbeg + b.zchunk - 1;
   if (kend > b.k1) kend = b.k1;
   const bool owner = (i <= b.i1) && (j <= b.j1);
-  const bool inmem = (i <= L.ni + 2) && (j <= L.nj + 2);
+  // lanes past the last ghost column/row only feed LDS cells no owner reads;
+  // clamping their address keeps every load unconditional and in bounds
+  const int ic = i <= L.ni + 2 ? i : L.ni + 2, jc = j <= L.nj + 2 ? j : L.nj + 2;
   const long so = (long)shot * a.shot_stride;
-  const TS *win = a.w_in + so;
-  const long col = (long)(j + kNG) * px + (long)(i + L.xa);
+  const unsigned upz = (unsigned)L.pz, upx = (unsigned)L.px;
+  const unsigned col = (unsigned)(jc + kNG) * upx + (unsigned)(ic + L.xa);
 
+  // uniform per-component bases with 32-bit byte offsets: scalar-base addressing
+  auto wq = [&](int c) -> const TS * { return a.w_in + so + c * cs; };
   auto sc = [&](int c) -> TC { return St<TS>::scaled ? (TC)a.sc[c] : TC(1); };
-  auto gld = [&](int c, long off) -> TC { return (TC)St<TS>::ld(win + c * cs + off) * sc(c); };
+  auto gld = [&](int c, unsigned off) -> TC {
+    return (TC)St<TS>::ld(&at_b(wq(c), off * (unsigned)sizeof(TS))) * sc(c);
+  };
+
+  // this thread's halo cells: compile-time count, positions fixed over the sweep
+  int h_lds[NPER];
+  unsigned h_off[NPER];
+  int h_c[NPER];
+#pragma unroll
+  for (int p = 0; p < NPER; p++) {
+    const int e = tid + p * NT;
+    const int ee = e < NHT ? e : NHT - 1;
+    const int c = ee / NH, r = ee % NH;
+    int hi, hj, li, lj;
+    if (r < NXH) {
+      const int h = r % 4, row = r / 4;
+      li = (h < HXL) ? h : TX + h;
+      hi = i0 - HXL + li;
+      lj = row + HYL;
+      hj = j0 + row;
+    } else {
+      const int rr = r - NXH;
+      const int h = rr / TX, cc = rr % TX;
+      lj = (h < HYL) ? h : TY + h;
+      hj = j0 - HYL + lj;
+      li = cc + HXL;
+      hi = i0 + cc;
+    }
+    hi = hi <= L.ni + 2 ? hi : L.ni + 2;
+    hj = hj <= L.nj + 2 ? hj : L.nj + 2;
+    h_c[p] = c;
+    h_lds[p] = e < NHT ? (c * LH + lj) * LW + li : -1;
+    h_off[p] = (unsigned)(hj + kNG) * upx + (unsigned)(hi + L.xa);
+  }
+  TC *ldsf = &lds[0][0][0];
 
   // register queues: planes k+zlo .. k+zlo+4
   TC q[kNC][5];
 #pragma unroll
   for (int c = 0; c < kNC; c++)
 #pragma unroll
-    for (int s = 0; s < 4; s++)
-      q[c][s + 1] = inmem ? gld(c, (long)(kbeg + zlo + s + kNG) * pz + col) : TC(0);
+    for (int s = 0; s < 4; s++) q[c][s + 1] = gld(c, (unsigned)(kbeg + zlo + s + kNG) * upz + col);
 
+  const int mode = a.mode;
   for (int k = kbeg; k <= kend; k++) {
-    const long pk = (long)(k + kNG) * pz;
+    const unsigned pk = (unsigned)(k + kNG) * upz;
+    const unsigned o = pk + col;
+    // ---- issue every global load of this plane ----
 #pragma unroll
     for (int c = 0; c < kNC; c++) {
 #pragma unroll
       for (int s = 0; s < 4; s++) q[c][s] = q[c][s + 1];
-      q[c][4] = inmem ? gld(c, (long)(k + zlo + 4 + kNG) * pz + col) : TC(0);
-      lds[c][ly + HYL][lx + HXL] = q[c][zc0];
+      q[c][4] = gld(c, (unsigned)(k + zlo + 4 + kNG) * upz + col);
     }
-    // halo of the centre plane: x columns for the tile rows, y rows for the tile columns
-    constexpr int NXH = 4 * TY, NYH = 4 * TX, NH = NXH + NYH;
-    for (int e = tid; e < kNC * NH; e += NT) {
-      const int c = e / NH, r = e % NH;
-      int hi, hj, li, lj;
-      if (r < NXH) {
-        const int h = r % 4, row = r / 4;
-        li = (h < HXL) ? h : TX + h;
-        hi = i0 - HXL + li;
-        lj = row + HYL;
-        hj = j0 + row;
-      } else {
-        const int rr = r - NXH;
-        const int h = rr / TX, cc = rr % TX;
-        lj = (h < HYL) ? h : TY + h;
-        hj = j0 - HYL + lj;
-        li = cc + HXL;
-        hi = i0 + cc;
+    TC hv[NPER];
+#pragma unroll
+    for (int p = 0; p < NPER; p++) hv[p] = gld(h_c[p], pk + h_off[p]);
+    const unsigned ob = o * (unsigned)sizeof(TC);
+    const TC xix = at_b(a.met + XIX * cs, ob), xiy = at_b(a.met + XIY * cs, ob), xiz = at_b(a.met + XIZ * cs, ob);
+    const TC etx = at_b(a.met + ETX * cs, ob), ety = at_b(a.met + ETY * cs, ob), etz = at_b(a.met + ETZ * cs, ob);
+    const TC ztx = at_b(a.met + ZTX * cs, ob), zty = at_b(a.met + ZTY * cs, ob), ztz = at_b(a.met + ZTZ * cs, ob);
+    const TC lam = at_b(a.med, ob), mu = at_b(a.med + cs, ob), slw = at_b(a.med + 2 * cs, ob);
+    const unsigned qb = o * (unsigned)sizeof(TS);
+    TC pre_v[kNC], end_v[kNC];
+    if (mode == MODE_S12 || mode == MODE_S3) {
+#pragma unroll
+      for (int c = 0; c < kNC; c++) {
+        end_v[c] = (TC)St<TS>::ld(&at_b(a.w_end + so + c * cs, qb)) * sc(c);
+        pre_v[c] = mode == MODE_S12 ? (TC)St<TS>::ld(&at_b(a.w_pre + so + c * cs, qb)) * sc(c) : TC(0);
       }
-      TC v = TC(0);
-      if (hi <= L.ni + 2 && hj <= L.nj + 2) v = gld(c, pk + (long)(hj + kNG) * px + (hi + L.xa));
-      lds[c][lj][li] = v;
     }
+    // ---- stage the centre plane and its halo in LDS ----
+#pragma unroll
+    for (int c = 0; c < kNC; c++) lds[c][ly + HYL][lx + HXL] = q[c][zc0];
+#pragma unroll
+    for (int p = 0; p < NPER; p++)
+      if (h_lds[p] >= 0) ldsf[h_lds[p]] = hv[p];
     __syncthreads();
 
-    TC Dx[kNC], Dy[kNC], Dz[kNC];
+    // each component's derivatives fold straight into the velocity gradient
+    // (V) or the stress divergence (T): 12 accumulators instead of 27 values
+    TC Lg[3][3];                       // Lg[a][b] = dV_a / dx_b
+    TC dv[3] = {TC(0), TC(0), TC(0)};  // dT_ab / dx_b
 #pragma unroll
     for (int c = 0; c < kNC; c++) {
-      constexpr int xl = HXL + ((DX == 0) ? -3 : -1), yl = HYL + ((DY == 0) ? -3 : -1);
       TC dx = TC(0), dy = TC(0), dz = TC(0);
 #pragma unroll
       for (int n = 0; n < 5; n++) {
-        dx += Drp<TC, DX>::c(n) * lds[c][ly + HYL][lx + xl + n];
-        dy += Drp<TC, DY>::c(n) * lds[c][ly + yl + n][lx + HXL];
+        dx += Drp<TC, DX>::c(n) * lds[c][ly + HYL][lx + n];
+        dy += Drp<TC, DY>::c(n) * lds[c][ly + n][lx + HXL];
         dz += Drp<TC, DZ>::c(n) * q[c][n];
       }
-      Dx[c] = dx; Dy[c] = dy; Dz[c] = dz;
+      const TC g0 = xix * dx + etx * dy + ztx * dz;
+      const TC g1 = xiy * dx + ety * dy + zty * dz;
+      const TC g2 = xiz * dx + etz * dy + ztz * dz;
+      if (c < 3) {
+        Lg[c][0] = g0; Lg[c][1] = g1; Lg[c][2] = g2;
+      } else if (c == TXX) {
+        dv[0] += g0;
+      } else if (c == TYY) {
+        dv[1] += g1;
+      } else if (c == TZZ) {
+        dv[2] += g2;
+      } else if (c == TYZ) {
+        dv[1] += g2; dv[2] += g1;
+      } else if (c == TXZ) {
+        dv[0] += g2; dv[2] += g0;
+      } else {
+        dv[0] += g1; dv[1] += g0;
+      }
     }
 
     if (owner) {
-      const long o = pk + col;
-      const TC *met = a.met;
-      const TC xix = met[XIX * cs + o], xiy = met[XIY * cs + o], xiz = met[XIZ * cs + o];
-      const TC etx = met[ETX * cs + o], ety = met[ETY * cs + o], etz = met[ETZ * cs + o];
-      const TC ztx = met[ZTX * cs + o], zty = met[ZTY * cs + o], ztz = met[ZTZ * cs + o];
-      const TC lam = a.med[o], mu = a.med[cs + o], slw = a.med[2 * cs + o];
       const TC lam2mu = lam + TC(2) * mu;
       TC h[kNC];
-      h[VX] = slw * (xix * Dx[TXX] + xiy * Dx[TXY] + xiz * Dx[TXZ] + etx * Dy[TXX] + ety * Dy[TXY] +
-                     etz * Dy[TXZ] + ztx * Dz[TXX] + zty * Dz[TXY] + ztz * Dz[TXZ]);
-      h[VY] = slw * (xix * Dx[TXY] + xiy * Dx[TYY] + xiz * Dx[TYZ] + etx * Dy[TXY] + ety * Dy[TYY] +
-                     etz * Dy[TYZ] + ztx * Dz[TXY] + zty * Dz[TYY] + ztz * Dz[TYZ]);
-      h[VZ] = slw * (xix * Dx[TXZ] + xiy * Dx[TYZ] + xiz * Dx[TZZ] + etx * Dy[TXZ] + ety * Dy[TYZ] +
-                     etz * Dy[TZZ] + ztx * Dz[TXZ] + zty * Dz[TYZ] + ztz * Dz[TZZ]);
-      h[TXX] = lam2mu * (xix * Dx[VX] + etx * Dy[VX] + ztx * Dz[VX]) +
-               lam * (xiy * Dx[VY] + ety * Dy[VY] + zty * Dz[VY] + xiz * Dx[VZ] + etz * Dy[VZ] + ztz * Dz[VZ]);
-      h[TYY] = lam2mu * (xiy * Dx[VY] + ety * Dy[VY] + zty * Dz[VY]) +
-               lam * (xix * Dx[VX] + etx * Dy[VX] + ztx * Dz[VX] + xiz * Dx[VZ] + etz * Dy[VZ] + ztz * Dz[VZ]);
-      h[TZZ] = lam2mu * (xiz * Dx[VZ] + etz * Dy[VZ] + ztz * Dz[VZ]) +
-               lam * (xix * Dx[VX] + etx * Dy[VX] + ztx * Dz[VX] + xiy * Dx[VY] + ety * Dy[VY] + zty * Dz[VY]);
-      h[TXY] = mu * (xiy * Dx[VX] + xix * Dx[VY] + ety * Dy[VX] + etx * Dy[VY] + zty * Dz[VX] + ztx * Dz[VY]);
-      h[TXZ] = mu * (xiz * Dx[VX] + xix * Dx[VZ] + etz * Dy[VX] + etx * Dy[VZ] + ztz * Dz[VX] + ztx * Dz[VZ]);
-      h[TYZ] = mu * (xiz * Dx[VY] + xiy * Dx[VZ] + etz * Dy[VY] + ety * Dy[VZ] + ztz * Dz[VY] + zty * Dz[VZ]);
-
-      TS *wout = a.w_out + so, *wend = a.w_end + so;
-      const TS *wpre = a.w_pre + so;
+      h[VX] = slw * dv[0];
+      h[VY] = slw * dv[1];
+      h[VZ] = slw * dv[2];
+      h[TXX] = lam2mu * Lg[0][0] + lam * (Lg[1][1] + Lg[2][2]);
+      h[TYY] = lam2mu * Lg[1][1] + lam * (Lg[0][0] + Lg[2][2]);
+      h[TZZ] = lam2mu * Lg[2][2] + lam * (Lg[0][0] + Lg[1][1]);
+      h[TXY] = mu * (Lg[0][1] + Lg[1][0]);
+      h[TXZ] = mu * (Lg[0][2] + Lg[2][0]);
+      h[TYZ] = mu * (Lg[1][2] + Lg[2][1]);
 #pragma unroll
       for (int c = 0; c < kNC; c++) {
-        const long qo = c * cs + o;
+        TS *wout = a.w_out + so + c * cs;
+        TS *wend = a.w_end + so + c * cs;
         const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
-        if (a.mode == MODE_S12) {
-          const TC pre = (TC)St<TS>::ld(wpre + qo) * sc(c);
-          const TC en = (TC)St<TS>::ld(wend + qo) * sc(c);
-          St<TS>::st(wout + qo, (pre + a.ca * h[c]) * is);
-          St<TS>::st(wend + qo, (en + a.cb * h[c]) * is);
-        } else if (a.mode == MODE_S0) {
+        if (mode == MODE_S12) {
+          St<TS>::st(&at_b(wout, qb), (pre_v[c] + a.ca * h[c]) * is);
+          St<TS>::st(&at_b(wend, qb), (end_v[c] + a.cb * h[c]) * is);
+        } else if (mode == MODE_S0) {
           const TC v = q[c][zc0];
-          St<TS>::st(wout + qo, (v + a.ca * h[c]) * is);
-          St<TS>::st(wend + qo, (v + a.cb * h[c]) * is);
-        } else if (a.mode == MODE_S3) {
-          const TC en = (TC)St<TS>::ld(wend + qo) * sc(c);
-          St<TS>::st(wend + qo, (en + a.cb * h[c]) * is);
+          St<TS>::st(&at_b(wout, qb), (v + a.ca * h[c]) * is);
+          St<TS>::st(&at_b(wend, qb), (v + a.cb * h[c]) * is);
+        } else if (mode == MODE_S3) {
+          St<TS>::st(&at_b(wend, qb), (end_v[c] + a.cb * h[c]) * is);
         } else {
-          St<TS>::st(wout + qo, h[c] * is);
+          St<TS>::st(&at_b(wout, qb), h[c] * is);
         }
       }
     }
@@ -170,7 +219,7 @@ __global__ __launch_bounds__(TX *TY) void k_interior(StageArgs<TS, TC> a, Interi
   }
 }
 
-template <typename TS, typename TC, int TX, int TY>
+template <typename TS, typename TC, int TX, int TY, int MINB>
 static hipError_t launch_interior_t(const StageArgs<TS, TC> &a, const InteriorBox &b, int dx, int dy, int dz,
                                     int nshots, hipStream_t st) {
   const int ntx = (b.i1 - b.i0 + TX) / TX;
@@ -180,7 +229,7 @@ static hipError_t launch_interior_t(const StageArgs<TS, TC> &a, const InteriorBo
   dim3 grid((unsigned)nwg), block(TX * TY);
   const int sel = dx * 4 + dy * 2 + dz;
 #define CGFD_L(X, Y, Z) \
-  hipLaunchKernelGGL((k_interior<X, Y, Z, TS, TC, TX, TY>), grid, block, 0, st, a, b, ntx, nty, nzc)
+  hipLaunchKernelGGL((k_interior<X, Y, Z, TS, TC, TX, TY, MINB>), grid, block, 0, st, a, b, ntx, nty, nzc)
   switch (sel) {
     case 0: CGFD_L(0, 0, 0); break;
     case 1: CGFD_L(0, 0, 1); break;
@@ -195,11 +244,26 @@ static hipError_t launch_interior_t(const StageArgs<TS, TC> &a, const InteriorBo
   return hipGetLastError();
 }
 
+// tile shapes the autotuner chooses from (TX x TY threads)
+int interior_num_shapes() { return 4; }
+
+void interior_shape(int id, int *tx, int *ty) {
+  static const int T[4][2] = {{64, 4}, {32, 8}, {64, 8}, {32, 16}};
+  id = id < 0 ? 0 : id > 3 ? 3 : id;
+  *tx = T[id][0];
+  *ty = T[id][1];
+}
+
 template <typename TS, typename TC>
 hipError_t launch_interior(const StageArgs<TS, TC> &a, const InteriorBox &b, int dx, int dy, int dz, int nshots,
                            hipStream_t st) {
   if (b.i1 < b.i0 || b.j1 < b.j0 || b.k1 < b.k0) return hipSuccess;
-  return launch_interior_t<TS, TC, 64, 4>(a, b, dx, dy, dz, nshots, st);
+  switch (b.shape) {
+    case 1: return launch_interior_t<TS, TC, 32, 8, 3>(a, b, dx, dy, dz, nshots, st);
+    case 2: return launch_interior_t<TS, TC, 64, 8, 1>(a, b, dx, dy, dz, nshots, st);
+    case 3: return launch_interior_t<TS, TC, 32, 16, 1>(a, b, dx, dy, dz, nshots, st);
+    default: return launch_interior_t<TS, TC, 64, 4, 3>(a, b, dx, dy, dz, nshots, st);
+  }
 }
 
 template hipError_t launch_interior<float, float>(const StageArgs<float, float> &, const InteriorBox &, int, int, int,

@@ -21,7 +21,11 @@ template <typename T> hipError_t launch_fill(T *p, long n, T v, hipStream_t st);
 struct InteriorBox {
   int i0, i1, j0, j1, k0, k1;
   int zchunk;  // planes per workgroup along z
+  int shape;   // tile shape id, see interior_shape()
 };
+
+int interior_num_shapes();
+void interior_shape(int id, int *tx, int *ty);
 
 template <typename TS, typename TC>
 hipError_t launch_interior(const StageArgs<TS, TC> &a, const InteriorBox &b, int dx, int dy, int dz, int nshots,

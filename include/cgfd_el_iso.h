@@ -171,6 +171,13 @@ int cgfd_fetch_level(cgfd_session *s, int shot, int level, int dtype, void *w9);
  * with the face box indexed iz*(nj*ni)+iy*ni+ix (sv_curv_col_el_iso_gpu.cu:775). */
 int cgfd_fetch_pml_aux(cgfd_session *s, int shot, int face, int level, int dtype, void *out);
 
+/* Interior tile of the z-marching kernel: shape id (0..3: 64x4, 32x8, 64x8,
+ * 32x16 threads) and z planes per workgroup.  By default the first
+ * cgfd_run times every candidate once and keeps the fastest (cached per grid
+ * and storage for the process); shape < 0 returns to that automatic choice. */
+int cgfd_set_tile(cgfd_session *s, int shape, int zchunk);
+int cgfd_get_tile(cgfd_session *s, int *shape, int *tx, int *ty, int *zchunk);
+
 /* Enable (default) or disable HIP-event timing of interior kernel launches. */
 int cgfd_set_timing(cgfd_session *s, int on);
 

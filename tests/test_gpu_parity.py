@@ -133,3 +133,27 @@ def test_multishot_equals_single_shot():
         assert np.array_equal(many.fetch_level(shot, cgfd.LEVEL_PRE), one.fetch_level(0, cgfd.LEVEL_PRE))
     one.close()
     many.close()
+
+
+@pytest.mark.parametrize("shape,zchunk", [(0, 7), (1, 48), (2, 5), (3, 24)])
+def test_every_interior_tile_shape(shape, zchunk):
+    """Each z-marching tile shape and a ragged z chunk reproduce the oracle."""
+    P = Problem(ni=75, nj=41, nk=37, curvilinear=True, pml=(4, 3, 5, 2, 4, 0), nt_total=12, seed=12, hetero=0.04)
+    s = _sess(P)
+    s.set_tile(shape, zchunk)
+    s.run(0, P.nt_total, P.dt)
+    t = s.get_tile()
+    assert t["shape"] == shape and t["zchunk"] == zchunk
+    so, wo, _ = P.oracle_run(32)
+    assert rel_l2(s.read_seismo(0), so) <= TOL32
+    _cmp_levels(s.fetch_level(0, cgfd.LEVEL_PRE), wo, TOL32, f"shape {shape}")
+    s.close()
+
+
+def test_autotune_picks_a_valid_tile():
+    P = Problem(ni=48, nj=40, nk=36, nt_total=4, seed=13)
+    s = _sess(P)
+    s.run(0, 2, P.dt)
+    t = s.get_tile()
+    assert 0 <= t["shape"] < 4 and t["zchunk"] >= 1
+    s.close()
