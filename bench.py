@@ -72,7 +72,20 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain", action="store_true", help="per-point kernel everywhere")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch  # plumbing only: a plain copy of known size for HBM counter scaling
+
+        n = args.copy_calib * 1024 * 1024 // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes_each_way": n * 4, "copies": 5}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(the decomposition is described in DESIGN.md)", file=sys.stderr)

@@ -17,6 +17,7 @@ STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
 DT_F32, DT_F64 = 0, 1
 LEVEL_PRE, LEVEL_TMPA, LEVEL_TMPB, LEVEL_END, LEVEL_RHS = 0, 1, 2, 3, 4
 FLAG_PLAIN = 1
+FLAG_PML_MARCH = 2
 
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = [

@@ -487,9 +487,13 @@ StageArgs<TS, TC> stage_args(cgfd_session *s, int slot_in, int slot_pre, int slo
   return a;
 }
 
+// Work split of one stage: the z-marching interior box, the PML face slabs
+// (z-marching PML variant) and, under a free surface, the top 4 rows
+// (per-point kernel, which also covers everything for tiny grids or the
+// plain method).
 struct Shell {
-  InteriorBox ib;
-  bool has_interior;
+  MarchSet inner;
+  MarchSet pml;
   int nbox;
   int box[6][6];
   long box_end[6];
@@ -499,34 +503,83 @@ Shell make_shell(cgfd_session *s) {
   Shell sh{};
   const int ni = s->d.ni, nj = s->d.nj, nk = s->d.nk;
   auto lay = [&](int f) { return s->face[f].on ? s->face[f].nlay + 1 : 0; };
-  int ib0 = lay(0), ib1 = ni - 1 - lay(1);
-  int jb0 = lay(2), jb1 = nj - 1 - lay(3);
-  int kb0 = lay(4), kb1 = nk - 1 - std::max(s->free_top ? 4 : 0, lay(5));
+  const int ib0 = lay(0), ib1 = ni - 1 - lay(1);
+  const int jb0 = lay(2), jb1 = nj - 1 - lay(3);
+  const int kb0 = lay(4);
+  const int ktopm = s->free_top ? nk - 5 : nk - 1;  // highest plane the z-march may sweep
+  const int kb1 = ktopm - (s->free_top ? 0 : lay(5));
   const bool plain = (s->d.flags & CGFD_FLAG_PLAIN) != 0;
-  sh.has_interior = !plain && ib1 >= ib0 && jb1 >= jb0 && kb1 >= kb0;
-  std::vector<std::array<int, 6>> boxes;
-  if (!sh.has_interior) {
-    boxes.push_back({0, ni - 1, 0, nj - 1, 0, nk - 1});
+  std::vector<std::array<int, 6>> pboxes;
+  auto add = [](MarchSet &m, int a0, int a1, int b0, int b1, int c0, int c1) {
+    if (a1 >= a0 && b1 >= b0 && c1 >= c0) {
+      int q = m.nbox++;
+      m.box[q][0] = a0; m.box[q][1] = a1; m.box[q][2] = b0; m.box[q][3] = b1; m.box[q][4] = c0; m.box[q][5] = c1;
+    }
+  };
+  if (plain || ib1 < ib0 || jb1 < jb0 || kb1 < kb0) {
+    pboxes.push_back({0, ni - 1, 0, nj - 1, 0, nk - 1});
   } else {
-    sh.ib = InteriorBox{ib0, ib1, jb0, jb1, kb0, kb1, s->tile_zchunk, s->tile_shape < 0 ? 0 : s->tile_shape};
-    auto add = [&](int a0, int a1, int b0, int b1, int c0, int c1) {
-      if (a1 >= a0 && b1 >= b0 && c1 >= c0) boxes.push_back({a0, a1, b0, b1, c0, c1});
-    };
-    add(0, ib0 - 1, 0, nj - 1, 0, nk - 1);
-    add(ib1 + 1, ni - 1, 0, nj - 1, 0, nk - 1);
-    add(ib0, ib1, 0, jb0 - 1, 0, nk - 1);
-    add(ib0, ib1, jb1 + 1, nj - 1, 0, nk - 1);
-    add(ib0, ib1, jb0, jb1, 0, kb0 - 1);
-    add(ib0, ib1, jb0, jb1, kb1 + 1, nk - 1);
+    sh.inner.zchunk = s->tile_zchunk;
+    sh.inner.shape = s->tile_shape < 0 ? 0 : s->tile_shape;
+    add(sh.inner, ib0, ib1, jb0, jb1, kb0, kb1);
+    if (s->d.flags & CGFD_FLAG_PML_MARCH) {
+      sh.pml.zchunk = 24;
+      sh.pml.shape = 0;
+      add(sh.pml, 0, ib0 - 1, 0, nj - 1, 0, ktopm);
+      add(sh.pml, ib1 + 1, ni - 1, 0, nj - 1, 0, ktopm);
+      add(sh.pml, ib0, ib1, 0, jb0 - 1, 0, ktopm);
+      add(sh.pml, ib0, ib1, jb1 + 1, nj - 1, 0, ktopm);
+      add(sh.pml, ib0, ib1, jb0, jb1, 0, kb0 - 1);
+      add(sh.pml, ib0, ib1, jb0, jb1, kb1 + 1, ktopm);
+      if (ktopm < nk - 1) pboxes.push_back({0, ni - 1, 0, nj - 1, ktopm + 1, nk - 1});
+    } else {
+      auto addp = [&](int a0, int a1, int b0, int b1, int c0, int c1) {
+        if (a1 >= a0 && b1 >= b0 && c1 >= c0) pboxes.push_back({a0, a1, b0, b1, c0, c1});
+      };
+      addp(0, ib0 - 1, 0, nj - 1, 0, nk - 1);
+      addp(ib1 + 1, ni - 1, 0, nj - 1, 0, nk - 1);
+      addp(ib0, ib1, 0, jb0 - 1, 0, nk - 1);
+      addp(ib0, ib1, jb1 + 1, nj - 1, 0, nk - 1);
+      addp(ib0, ib1, jb0, jb1, 0, kb0 - 1);
+      addp(ib0, ib1, jb0, jb1, kb1 + 1, nk - 1);
+    }
   }
-  sh.nbox = (int)boxes.size();
+  sh.nbox = (int)pboxes.size();
   long acc = 0;
   for (int b = 0; b < sh.nbox; b++) {
-    for (int q = 0; q < 6; q++) sh.box[b][q] = boxes[b][q];
-    acc += (long)(boxes[b][1] - boxes[b][0] + 1) * (boxes[b][3] - boxes[b][2] + 1) * (boxes[b][5] - boxes[b][4] + 1);
+    for (int q = 0; q < 6; q++) sh.box[b][q] = pboxes[b][q];
+    acc += (long)(pboxes[b][1] - pboxes[b][0] + 1) * (pboxes[b][3] - pboxes[b][2] + 1) *
+           (pboxes[b][5] - pboxes[b][4] + 1);
     sh.box_end[b] = acc;
   }
   return sh;
+}
+
+// one stage over the whole grid: interior sweep (timed when requested), PML
+// slabs, free-surface rows
+template <typename TS, typename TC>
+int launch_stage(cgfd_session *s, const Shell &sh, StageArgs<TS, TC> &a, int dx, int dy, int dz, int nshots,
+                 hipEvent_t ev0, hipEvent_t ev1) {
+  if (sh.inner.nbox) {
+    if (ev0) HIPCHK(hipEventRecord(ev0, s->st));
+    hipError_t e = launch_interior<TS, TC>(a, sh.inner, dx, dy, dz, nshots, s->st);
+    if (e != hipSuccess) return fail(CGFD_E_HIP, "interior kernel: %s", hipGetErrorString(e));
+    if (ev1) HIPCHK(hipEventRecord(ev1, s->st));
+  }
+  if (sh.pml.nbox) {
+    hipError_t e = launch_pml_march<TS, TC>(a, sh.pml, dx, dy, dz, nshots, s->st);
+    if (e != hipSuccess) return fail(CGFD_E_HIP, "PML sweep kernel: %s", hipGetErrorString(e));
+  }
+  if (sh.nbox) {
+    a.nbox = sh.nbox;
+    for (int b = 0; b < sh.nbox; b++) {
+      for (int q = 0; q < 6; q++) a.box[b][q] = sh.box[b][q];
+      a.box_end[b] = sh.box_end[b];
+    }
+    hipError_t e = launch_point<TS, TC>(a, dx, dy, dz, nshots, s->st);
+    if (e != hipSuccess) return fail(CGFD_E_HIP, "point kernel: %s", hipGetErrorString(e));
+  }
+  return CGFD_OK;
 }
 
 // Tile autotuner: time one stage-0 interior sweep per (shape, z chunk) and
@@ -551,7 +604,7 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
   }
   s->tile_shape = 0;
   Shell sh = make_shell(s);
-  if (!sh.has_interior) {
+  if (!sh.inner.nbox) {
     g_tile_cache[key] = {s->tile_shape, s->tile_zchunk};
     return CGFD_OK;
   }
@@ -562,7 +615,7 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  const int nz = sh.ib.k1 - sh.ib.k0 + 1;
+  const int nz = sh.inner.box[0][5] - sh.inner.box[0][4] + 1;
   const int zcands[3] = {24, 48, 96};
   float best = 1e30f;
   int bs = 0, bz = 48;
@@ -570,7 +623,7 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
     for (int zi = 0; zi < 3; zi++) {
       int zc = std::min(zcands[zi], nz);
       if (zi > 0 && zc == std::min(zcands[zi - 1], nz)) continue;
-      InteriorBox ib = sh.ib;
+      MarchSet ib = sh.inner;
       ib.shape = shape;
       ib.zchunk = zc;
       float tmin = 1e30f;
@@ -605,7 +658,7 @@ template <typename TS, typename TC> int run_t(cgfd_session *s, int it0, int nste
   const Shell sh = make_shell(s);
   const TC dt = (TC)dt_in;
   HIPCHK(hipMemcpyAsync(s->clock, &it0, sizeof(int), hipMemcpyHostToDevice, s->st));
-  const size_t nev_need = (size_t)(sh.has_interior && s->timing ? 2 * 4 * nsteps : 0);
+  const size_t nev_need = (size_t)(sh.inner.nbox && s->timing ? 2 * 4 * nsteps : 0);
   if (s->timing && s->ev.size() < nev_need) {
     size_t old = s->ev.size();
     s->ev.resize(nev_need);
@@ -644,19 +697,12 @@ template <typename TS, typename TC> int run_t(cgfd_session *s, int it0, int nste
       // coef = rk_a[istage]*dt, both float in the reference (drv_rk_curv_col.cu:341-342)
       a.ca = (istage < 3) ? (TC)kRkA[istage] * dt : TC(0);
       a.cb = (TC)kRkB[istage] * dt;
-      if (sh.has_interior) {
-        if (s->timing) HIPCHK(hipEventRecord(s->ev[ne++], s->st));
-        hipError_t e = launch_interior<TS, TC>(a, sh.ib, dx, dy, dz, s->d.nshots, s->st);
-        if (e != hipSuccess) return fail(CGFD_E_HIP, "interior kernel: %s", hipGetErrorString(e));
-        if (s->timing) HIPCHK(hipEventRecord(s->ev[ne++], s->st));
-      }
-      a.nbox = sh.nbox;
-      for (int b = 0; b < sh.nbox; b++) {
-        for (int q = 0; q < 6; q++) a.box[b][q] = sh.box[b][q];
-        a.box_end[b] = sh.box_end[b];
-      }
-      hipError_t e = launch_point<TS, TC>(a, dx, dy, dz, s->d.nshots, s->st);
-      if (e != hipSuccess) return fail(CGFD_E_HIP, "point kernel: %s", hipGetErrorString(e));
+      const bool tim = s->timing && sh.inner.nbox;
+      int r = launch_stage<TS, TC>(s, sh, a, dx, dy, dz, s->d.nshots, tim ? s->ev[ne] : nullptr,
+                                   tim ? s->ev[ne + 1] : nullptr);
+      if (r) return r;
+      if (tim) ne += 2;
+      hipError_t e;
       if (s->s_npt > 0) {
         SrcArgs<TS, TC> sa{};
         sa.L = s->L; sa.npt = s->s_npt; sa.pt_shot = s->s_pt_shot; sa.pt_off = s->s_pt_off; sa.pt_beg = s->s_pt_beg;
@@ -718,17 +764,9 @@ template <typename TS, typename TC> int onestage_t(cgfd_session *s, int shot, in
   a.mode = MODE_RHS;
   a.ca = TC(0); a.cb = TC(0);
   const Shell sh = make_shell(s);
-  if (sh.has_interior) {
-    hipError_t e = launch_interior<TS, TC>(a, sh.ib, dx, dy, dz, 1, s->st);
-    if (e != hipSuccess) return fail(CGFD_E_HIP, "interior kernel: %s", hipGetErrorString(e));
-  }
-  a.nbox = sh.nbox;
-  for (int b = 0; b < sh.nbox; b++) {
-    for (int q = 0; q < 6; q++) a.box[b][q] = sh.box[b][q];
-    a.box_end[b] = sh.box_end[b];
-  }
-  hipError_t e = launch_point<TS, TC>(a, dx, dy, dz, 1, s->st);
-  if (e != hipSuccess) return fail(CGFD_E_HIP, "point kernel: %s", hipGetErrorString(e));
+  int r = launch_stage<TS, TC>(s, sh, a, dx, dy, dz, 1, nullptr, nullptr);
+  if (r) return r;
+  hipError_t e;
   if (s->s_npt > 0) {
     // the source kernel scans all shots' points; keep only this shot's via the active mask
     std::vector<int> act(s->d.nshots, 0);
@@ -866,6 +904,8 @@ int cgfd_upload_medium(cgfd_session *s, int dtype, const void *lam, const void *
 
 int cgfd_upload_free_surface(cgfd_session *s, int free_top, int dtype, const void *mvx, const void *mvy) {
   if (!s) return fail(CGFD_E_ARG, "null session");
+  if (free_top && s->face[5].on)
+    return fail(CGFD_E_ARG, "the top cannot be both a free surface and a PML face");
   s->free_top = free_top ? 1 : 0;
   if (!free_top) return CGFD_OK;
   if (!mvx || !mvy) return fail(CGFD_E_ARG, "free surface enabled without matrices");
@@ -892,6 +932,7 @@ int cgfd_upload_pml(cgfd_session *s, int face, int nlay, int dtype, const void *
   F.on = false;
   F.nlay = 0;
   if (nlay <= 0) return CGFD_OK;
+  if (face == 5 && s->free_top) return fail(CGFD_E_ARG, "the top cannot be both a free surface and a PML face");
   const int axis_n = face < 2 ? s->d.ni : face < 4 ? s->d.nj : s->d.nk;
   if (nlay + 1 > axis_n) return fail(CGFD_E_ARG, "face %d: %d layers exceed the grid", face, nlay);
   if (!A || !B || !D) return fail(CGFD_E_ARG, "face %d: null profile", face);

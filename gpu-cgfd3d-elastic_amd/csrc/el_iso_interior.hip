@@ -17,17 +17,18 @@
 // is issued before the first barrier, so one memory latency is exposed per
 // plane instead of one per dependent load.
 //
-// Only points that are interior in the reference's sense are handed to this
-// kernel: outside every PML face box and at least 4 rows below a free
-// surface.  The boundary shell goes to the per-point kernel.
+// The interior variant (PML = false) sweeps the box outside every PML face
+// box; the PML variant also keeps the raw derivatives and applies the CFS-PML
+// ADE terms and memory-variable update to the face slabs.  Points within 4
+// rows of a free surface go to the per-point kernel.
 #include "el_iso_device.h"
 #include "el_iso_launch.h"
+#include "el_iso_pml.h"
 
 namespace cgfd {
 
-template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINB>
-__global__ __launch_bounds__(TX *TY, MINB) void k_interior(StageArgs<TS, TC> a, InteriorBox b, int ntx, int nty,
-                                                           int nzc) {
+template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINB, bool PML>
+__global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, MarchSet b, MarchGrid g) {
   constexpr int NT = TX * TY;
   constexpr int HXL = (DX == 0) ? 3 : 1;
   constexpr int HYL = (DY == 0) ? 3 : 1;
@@ -37,7 +38,7 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_interior(StageArgs<TS, TC> a, 
   constexpr int NXH = 4 * TY, NYH = 4 * TX, NH = NXH + NYH;  // halo cells per component
   constexpr int NHT = kNC * NH;                   // halo cells per plane
   constexpr int NPER = (NHT + NT - 1) / NT;       // halo loads per thread
-  __shared__ TC lds[kNC][LH][LW];
+  __shared__ TC lds[2][kNC][LH][LW];
 
   // XCD-aware remap: consecutive hardware workgroup ids land on different
   // XCDs; give each XCD a contiguous range of tiles so halos hit its L2.
@@ -46,22 +47,30 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_interior(StageArgs<TS, TC> a, 
   const int q8 = nwg / 8, r8 = nwg % 8;
   const int xcd = hw % 8, m = hw / 8;
   const int wg = xcd * q8 + (xcd < r8 ? xcd : r8) + m;
-  const int tx_t = wg % ntx;
-  const int ty_t = (wg / ntx) % nty;
-  const int zc = (wg / (ntx * nty)) % nzc;
-  const int shot = wg / (ntx * nty * nzc);
+  const int per_shot = g.wg_beg[b.nbox];
+  const int shot = wg / per_shot;
+  const int rw = wg - shot * per_shot;
+  int bx = 0;
+  while (bx < b.nbox - 1 && rw >= g.wg_beg[bx + 1]) bx++;
+  const int lw = rw - g.wg_beg[bx];
+  const int ntx = g.ntx[bx], nty = g.nty[bx];
+  const int tx_t = lw % ntx;
+  const int ty_t = (lw / ntx) % nty;
+  const int zc = lw / (ntx * nty);
+  const int bi0 = b.box[bx][0], bi1 = b.box[bx][1], bj0 = b.box[bx][2], bj1 = b.box[bx][3];
+  const int bk0 = b.box[bx][4], bk1 = b.box[bx][5];
   if (a.shot_active && !a.shot_active[shot]) return;
 
   const Layout &L = a.L;
   const long cs = L.cs;
   const int tid = threadIdx.x;
   const int lx = tid % TX, ly = tid / TX;
-  const int i0 = b.i0 + tx_t * TX, j0 = b.j0 + ty_t * TY;
+  const int i0 = bi0 + tx_t * TX, j0 = bj0 + ty_t * TY;
   const int i = i0 + lx, j = j0 + ly;
-  const int kbeg = b.k0 + zc * b.zchunk;
+  const int kbeg = bk0 + zc * b.zchunk;
   int kend = kbeg + b.zchunk - 1;
-  if (kend > b.k1) kend = b.k1;
-  const bool owner = (i <= b.i1) && (j <= b.j1);
+  if (kend > bk1) kend = bk1;
+  const bool owner = (i <= bi1) && (j <= bj1);
   // lanes past the last ghost column/row only feed LDS cells no owner reads;
   // clamping their address keeps every load unconditional and in bounds
   const int ic = i <= L.ni + 2 ? i : L.ni + 2, jc = j <= L.nj + 2 ? j : L.nj + 2;
@@ -106,7 +115,8 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_interior(StageArgs<TS, TC> a, 
     h_lds[p] = e < NHT ? (c * LH + lj) * LW + li : -1;
     h_off[p] = (unsigned)(hj + kNG) * upx + (unsigned)(hi + L.xa);
   }
-  TC *ldsf = &lds[0][0][0];
+  TC *ldsf = &lds[0][0][0][0];
+  constexpr int LBUF = kNC * LH * LW;  // one LDS plane buffer
 
   // register queues: planes k+zlo .. k+zlo+4
   TC q[kNC][5];
@@ -115,55 +125,80 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_interior(StageArgs<TS, TC> a, 
 #pragma unroll
     for (int s = 0; s < 4; s++) q[c][s + 1] = gld(c, (unsigned)(kbeg + zlo + s + kNG) * upz + col);
 
+  // Everything plane k needs from HBM/L2, fetched one plane ahead so its
+  // latency overlaps the previous plane's arithmetic.
   const int mode = a.mode;
+  const bool need_end = mode == MODE_S12 || mode == MODE_S3, need_pre = mode == MODE_S12;
+  TC n_lead[kNC], n_hv[NPER], n_met[12], n_pre[kNC], n_end[kNC];
+#pragma unroll
+  for (int c = 0; c < kNC; c++) { n_pre[c] = TC(0); n_end[c] = TC(0); }
+  auto issue = [&](int kk) {
+    const unsigned pk = (unsigned)(kk + kNG) * upz;
+    const unsigned o = pk + col;
+#pragma unroll
+    for (int c = 0; c < kNC; c++) n_lead[c] = gld(c, (unsigned)(kk + zlo + 4 + kNG) * upz + col);
+#pragma unroll
+    for (int p = 0; p < NPER; p++) n_hv[p] = gld(h_c[p], pk + h_off[p]);
+    const unsigned ob = o * (unsigned)sizeof(TC);
+#pragma unroll
+    for (int m = 0; m < 9; m++) n_met[m] = at_b(a.met + (XIX + m) * cs, ob);
+#pragma unroll
+    for (int m = 0; m < 3; m++) n_met[9 + m] = at_b(a.med + m * cs, ob);
+    const unsigned qb = o * (unsigned)sizeof(TS);
+    if (need_end) {
+#pragma unroll
+      for (int c = 0; c < kNC; c++) {
+        n_end[c] = (TC)St<TS>::ld(&at_b(a.w_end + so + c * cs, qb)) * sc(c);
+        n_pre[c] = need_pre ? (TC)St<TS>::ld(&at_b(a.w_pre + so + c * cs, qb)) * sc(c) : TC(0);
+      }
+    }
+  };
+  issue(kbeg);
+
   for (int k = kbeg; k <= kend; k++) {
     const unsigned pk = (unsigned)(k + kNG) * upz;
     const unsigned o = pk + col;
-    // ---- issue every global load of this plane ----
+    const unsigned qb = o * (unsigned)sizeof(TS);
+    TC hv[NPER], mt[12], pre_v[kNC], end_v[kNC];
 #pragma unroll
     for (int c = 0; c < kNC; c++) {
 #pragma unroll
       for (int s = 0; s < 4; s++) q[c][s] = q[c][s + 1];
-      q[c][4] = gld(c, (unsigned)(k + zlo + 4 + kNG) * upz + col);
+      q[c][4] = n_lead[c];
+      pre_v[c] = n_pre[c];
+      end_v[c] = n_end[c];
     }
-    TC hv[NPER];
 #pragma unroll
-    for (int p = 0; p < NPER; p++) hv[p] = gld(h_c[p], pk + h_off[p]);
-    const unsigned ob = o * (unsigned)sizeof(TC);
-    const TC xix = at_b(a.met + XIX * cs, ob), xiy = at_b(a.met + XIY * cs, ob), xiz = at_b(a.met + XIZ * cs, ob);
-    const TC etx = at_b(a.met + ETX * cs, ob), ety = at_b(a.met + ETY * cs, ob), etz = at_b(a.met + ETZ * cs, ob);
-    const TC ztx = at_b(a.met + ZTX * cs, ob), zty = at_b(a.met + ZTY * cs, ob), ztz = at_b(a.met + ZTZ * cs, ob);
-    const TC lam = at_b(a.med, ob), mu = at_b(a.med + cs, ob), slw = at_b(a.med + 2 * cs, ob);
-    const unsigned qb = o * (unsigned)sizeof(TS);
-    TC pre_v[kNC], end_v[kNC];
-    if (mode == MODE_S12 || mode == MODE_S3) {
+    for (int p = 0; p < NPER; p++) hv[p] = n_hv[p];
 #pragma unroll
-      for (int c = 0; c < kNC; c++) {
-        end_v[c] = (TC)St<TS>::ld(&at_b(a.w_end + so + c * cs, qb)) * sc(c);
-        pre_v[c] = mode == MODE_S12 ? (TC)St<TS>::ld(&at_b(a.w_pre + so + c * cs, qb)) * sc(c) : TC(0);
-      }
-    }
-    // ---- stage the centre plane and its halo in LDS ----
+    for (int m = 0; m < 12; m++) mt[m] = n_met[m];
+    issue(k < kend ? k + 1 : kend);
+    const TC xix = mt[0], xiy = mt[1], xiz = mt[2], etx = mt[3], ety = mt[4], etz = mt[5];
+    const TC ztx = mt[6], zty = mt[7], ztz = mt[8], lam = mt[9], mu = mt[10], slw = mt[11];
+    // ---- stage the centre plane and its halo in LDS (double buffered) ----
+    const int buf = (k - kbeg) & 1;
 #pragma unroll
-    for (int c = 0; c < kNC; c++) lds[c][ly + HYL][lx + HXL] = q[c][zc0];
+    for (int c = 0; c < kNC; c++) lds[buf][c][ly + HYL][lx + HXL] = q[c][zc0];
 #pragma unroll
     for (int p = 0; p < NPER; p++)
-      if (h_lds[p] >= 0) ldsf[h_lds[p]] = hv[p];
+      if (h_lds[p] >= 0) ldsf[buf * LBUF + h_lds[p]] = hv[p];
     __syncthreads();
 
     // each component's derivatives fold straight into the velocity gradient
     // (V) or the stress divergence (T): 12 accumulators instead of 27 values
     TC Lg[3][3];                       // Lg[a][b] = dV_a / dx_b
     TC dv[3] = {TC(0), TC(0), TC(0)};  // dT_ab / dx_b
+    TC Dxr[PML ? kNC : 1], Dyr[PML ? kNC : 1], Dzr[PML ? kNC : 1];  // raw derivatives for the PML terms
 #pragma unroll
     for (int c = 0; c < kNC; c++) {
       TC dx = TC(0), dy = TC(0), dz = TC(0);
 #pragma unroll
       for (int n = 0; n < 5; n++) {
-        dx += Drp<TC, DX>::c(n) * lds[c][ly + HYL][lx + n];
-        dy += Drp<TC, DY>::c(n) * lds[c][ly + n][lx + HXL];
+        dx += Drp<TC, DX>::c(n) * lds[buf][c][ly + HYL][lx + n];
+        dy += Drp<TC, DY>::c(n) * lds[buf][c][ly + n][lx + HXL];
         dz += Drp<TC, DZ>::c(n) * q[c][n];
       }
+      if constexpr (PML) { Dxr[c] = dx; Dyr[c] = dy; Dzr[c] = dz; }
       const TC g0 = xix * dx + etx * dy + ztx * dz;
       const TC g1 = xiy * dx + ety * dy + zty * dz;
       const TC g2 = xiz * dx + etz * dy + ztz * dz;
@@ -196,6 +231,9 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_interior(StageArgs<TS, TC> a, 
       h[TXY] = mu * (Lg[0][1] + Lg[1][0]);
       h[TXZ] = mu * (Lg[0][2] + Lg[2][0]);
       h[TYZ] = mu * (Lg[1][2] + Lg[2][1]);
+      if constexpr (PML)
+        pml_apply<TS, TC>(a, shot, i, j, k, Dxr, Dyr, Dzr, xix, xiy, xiz, etx, ety, etz, ztx, zty, ztz, lam, mu,
+                          lam2mu, slw, h);
 #pragma unroll
       for (int c = 0; c < kNC; c++) {
         TS *wout = a.w_out + so + c * cs;
@@ -215,21 +253,30 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_interior(StageArgs<TS, TC> a, 
         }
       }
     }
-    __syncthreads();
+    // no second barrier: the next plane writes the other LDS buffer, and the
+    // one after that is only written once every wave has passed the next barrier
   }
 }
 
-template <typename TS, typename TC, int TX, int TY, int MINB>
-static hipError_t launch_interior_t(const StageArgs<TS, TC> &a, const InteriorBox &b, int dx, int dy, int dz,
-                                    int nshots, hipStream_t st) {
-  const int ntx = (b.i1 - b.i0 + TX) / TX;
-  const int nty = (b.j1 - b.j0 + TY) / TY;
-  const int nzc = (b.k1 - b.k0 + b.zchunk) / b.zchunk;
-  const long nwg = (long)ntx * nty * nzc * nshots;
+template <typename TS, typename TC, int TX, int TY, int MINB, bool PML>
+static hipError_t launch_march_t(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz,
+                                 int nshots, hipStream_t st) {
+  MarchGrid g{};
+  int acc = 0;
+  for (int q = 0; q < b.nbox; q++) {
+    g.wg_beg[q] = acc;
+    g.ntx[q] = (b.box[q][1] - b.box[q][0] + TX) / TX;
+    g.nty[q] = (b.box[q][3] - b.box[q][2] + TY) / TY;
+    g.nzc[q] = (b.box[q][5] - b.box[q][4] + b.zchunk) / b.zchunk;
+    acc += g.ntx[q] * g.nty[q] * g.nzc[q];
+  }
+  g.wg_beg[b.nbox] = acc;
+  const long nwg = (long)acc * nshots;
+  if (nwg == 0) return hipSuccess;
   dim3 grid((unsigned)nwg), block(TX * TY);
   const int sel = dx * 4 + dy * 2 + dz;
 #define CGFD_L(X, Y, Z) \
-  hipLaunchKernelGGL((k_interior<X, Y, Z, TS, TC, TX, TY, MINB>), grid, block, 0, st, a, b, ntx, nty, nzc)
+  hipLaunchKernelGGL((k_march<X, Y, Z, TS, TC, TX, TY, MINB, PML>), grid, block, 0, st, a, b, g)
   switch (sel) {
     case 0: CGFD_L(0, 0, 0); break;
     case 1: CGFD_L(0, 0, 1); break;
@@ -255,24 +302,35 @@ void interior_shape(int id, int *tx, int *ty) {
 }
 
 template <typename TS, typename TC>
-hipError_t launch_interior(const StageArgs<TS, TC> &a, const InteriorBox &b, int dx, int dy, int dz, int nshots,
+hipError_t launch_interior(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
                            hipStream_t st) {
-  if (b.i1 < b.i0 || b.j1 < b.j0 || b.k1 < b.k0) return hipSuccess;
+  if (b.nbox < 1) return hipSuccess;
   switch (b.shape) {
-    case 1: return launch_interior_t<TS, TC, 32, 8, 3>(a, b, dx, dy, dz, nshots, st);
-    case 2: return launch_interior_t<TS, TC, 64, 8, 1>(a, b, dx, dy, dz, nshots, st);
-    case 3: return launch_interior_t<TS, TC, 32, 16, 1>(a, b, dx, dy, dz, nshots, st);
-    default: return launch_interior_t<TS, TC, 64, 4, 3>(a, b, dx, dy, dz, nshots, st);
+    case 1: return launch_march_t<TS, TC, 32, 8, 2, false>(a, b, dx, dy, dz, nshots, st);
+    case 2: return launch_march_t<TS, TC, 64, 8, 1, false>(a, b, dx, dy, dz, nshots, st);
+    case 3: return launch_march_t<TS, TC, 32, 16, 1, false>(a, b, dx, dy, dz, nshots, st);
+    default: return launch_march_t<TS, TC, 64, 4, 2, false>(a, b, dx, dy, dz, nshots, st);
   }
 }
 
-template hipError_t launch_interior<float, float>(const StageArgs<float, float> &, const InteriorBox &, int, int, int,
-                                                  int, hipStream_t);
-template hipError_t launch_interior<double, double>(const StageArgs<double, double> &, const InteriorBox &, int, int,
-                                                    int, int, hipStream_t);
-template hipError_t launch_interior<__half, float>(const StageArgs<__half, float> &, const InteriorBox &, int, int,
-                                                   int, int, hipStream_t);
-template hipError_t launch_interior<__hip_bfloat16, float>(const StageArgs<__hip_bfloat16, float> &,
-                                                           const InteriorBox &, int, int, int, int, hipStream_t);
+// PML slabs are thin (nlay+1 points) across one axis: a square 16 x 16 tile
+// keeps most lanes busy on x, y and z slabs alike
+template <typename TS, typename TC>
+hipError_t launch_pml_march(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
+                            hipStream_t st) {
+  if (b.nbox < 1) return hipSuccess;
+  return launch_march_t<TS, TC, 16, 16, 1, true>(a, b, dx, dy, dz, nshots, st);
+}
+
+#define CGFD_INST(TS, TC)                                                                                     \
+  template hipError_t launch_interior<TS, TC>(const StageArgs<TS, TC> &, const MarchSet &, int, int, int, int,   \
+                                              hipStream_t);                                                   \
+  template hipError_t launch_pml_march<TS, TC>(const StageArgs<TS, TC> &, const MarchSet &, int, int, int, int,  \
+                                               hipStream_t);
+CGFD_INST(float, float)
+CGFD_INST(double, double)
+CGFD_INST(__half, float)
+CGFD_INST(__hip_bfloat16, float)
+#undef CGFD_INST
 
 }  // namespace cgfd

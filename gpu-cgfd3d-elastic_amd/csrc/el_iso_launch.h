@@ -15,20 +15,32 @@ hipError_t launch_clock_advance(int *clock, hipStream_t st);
 
 template <typename T> hipError_t launch_fill(T *p, long n, T v, hipStream_t st);
 
-// z-marching interior kernel over the box [i0,i1]x[j0,j1]x[k0,k1] (physical,
-// inclusive); the box must keep every point at least 4 rows below the top and
-// outside every PML face box.
-struct InteriorBox {
-  int i0, i1, j0, j1, k0, k1;
+// Boxes swept by the z-marching kernel ([i0,i1]x[j0,j1]x[k0,k1], physical,
+// inclusive).  Every point must lie at least 4 rows below a free surface.
+// The interior sweep takes one box outside every PML face box; the PML sweep
+// takes the face slabs and applies the CFS-PML terms.
+struct MarchSet {
+  int nbox;
+  int box[6][6];
   int zchunk;  // planes per workgroup along z
   int shape;   // tile shape id, see interior_shape()
+};
+
+// per-box tile counts, filled by the launcher for the chosen tile shape
+struct MarchGrid {
+  int ntx[6], nty[6], nzc[6];
+  int wg_beg[7];
 };
 
 int interior_num_shapes();
 void interior_shape(int id, int *tx, int *ty);
 
 template <typename TS, typename TC>
-hipError_t launch_interior(const StageArgs<TS, TC> &a, const InteriorBox &b, int dx, int dy, int dz, int nshots,
+hipError_t launch_interior(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
                            hipStream_t st);
+
+template <typename TS, typename TC>
+hipError_t launch_pml_march(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
+                            hipStream_t st);
 
 }  // namespace cgfd

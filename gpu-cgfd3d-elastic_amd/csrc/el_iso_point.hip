@@ -17,6 +17,7 @@
 // drv_rk_curv_col.cu:339-491.
 #include "el_iso_device.h"
 #include "el_iso_launch.h"
+#include "el_iso_pml.h"
 
 namespace cgfd {
 
@@ -31,11 +32,15 @@ __device__ __forceinline__ TC drp5(F f, long s) {
 template <int DX, int DY, int DZ, typename TS, typename TC>
 __device__ __forceinline__ void point_update(const StageArgs<TS, TC> &a, int shot, int i, int j, int k) {
   const Layout &L = a.L;
-  const long o = L.off(i, j, k);
+  // element offsets inside one component stay below 2^31: 32-bit offsets from
+  // uniform component bases let the compiler use scalar-base addressing
+  const int o = (int)L.off(i, j, k);
   const long so = (long)shot * a.shot_stride;
   const TS *win = a.w_in + so;
   const long cs = L.cs;
-  auto W = [&](int c, long d) -> TC { return (TC)St<TS>::ld(win + c * cs + o + d); };
+  auto W = [&](int c, long d) -> TC {
+    return (TC)St<TS>::ld(&at_b(win + c * cs, (unsigned)(o + (int)d) * (unsigned)sizeof(TS)));
+  };
   auto S = [&](int c) -> TC { return St<TS>::scaled ? (TC)a.sc[c] : TC(1); };
 
   TC Dx[kNC], Dy[kNC], Dz[kNC];
@@ -48,11 +53,14 @@ __device__ __forceinline__ void point_update(const StageArgs<TS, TC> &a, int sho
     if (St<TS>::scaled) { Dx[c] *= S(c); Dy[c] *= S(c); Dz[c] *= S(c); }
   }
   const TC *met = a.met;
-  auto Mt = [&](int m, long d) -> TC { return met[m * cs + o + d]; };
+  auto Mt = [&](int m, long d) -> TC {
+    return at_b(met + m * cs, (unsigned)(o + (int)d) * (unsigned)sizeof(TC));
+  };
   const TC xix = Mt(XIX, 0), xiy = Mt(XIY, 0), xiz = Mt(XIZ, 0);
   const TC etx = Mt(ETX, 0), ety = Mt(ETY, 0), etz = Mt(ETZ, 0);
   const TC ztx = Mt(ZTX, 0), zty = Mt(ZTY, 0), ztz = Mt(ZTZ, 0);
-  const TC lam = a.med[o], mu = a.med[cs + o], slw = a.med[2 * cs + o];
+  const unsigned ob = (unsigned)o * (unsigned)sizeof(TC);
+  const TC lam = at_b(a.med, ob), mu = at_b(a.med + cs, ob), slw = at_b(a.med + 2 * cs, ob);
   // lam + 2.0*mu: the double-promoted sum of the reference rounds to the same
   // float as the float sum whenever lam and 2mu are within 2^29 of each other.
   const TC lam2mu = lam + TC(2) * mu;
@@ -142,105 +150,29 @@ __device__ __forceinline__ void point_update(const StageArgs<TS, TC> &a, int sho
   h[TYZ] = mu * (xiz * Dx[VY] + xiy * Dx[VZ] + etz * Dy[VY] + ety * Dy[VZ] + ztz * DzVy + zty * DzVz);
 
   // ---- CFS-PML faces, in the reference's face order ----
-#pragma unroll
-  for (int fi = 0; fi < 6; fi++) {
-    const FaceArgs<TS, TC> &F = a.f[fi];
-    if (!F.on) continue;
-    if (i < F.i1 || i > F.i2 || j < F.j1 || j > F.j2 || k < F.k1 || k > F.k2) continue;
-    const int dim = fi >> 1;
-    const int ic = (dim == 0) ? i - F.i1 : (dim == 1) ? j - F.j1 : k - F.k1;
-    const long ia = ((long)(k - F.k1) * F.bj + (j - F.j1)) * F.bi + (i - F.i1);
-    const TC cD = F.D[ic], cA = F.A[ic], cB = F.B[ic];
-    const TC Bm1 = cB - TC(1);
-    const TC *Dd = (dim == 0) ? Dx : (dim == 1) ? Dy : Dz;
-    const TC e1 = (dim == 0) ? xix : (dim == 1) ? etx : ztx;
-    const TC e2 = (dim == 0) ? xiy : (dim == 1) ? ety : zty;
-    const TC e3 = (dim == 0) ? xiz : (dim == 1) ? etz : ztz;
-    TC R[kNC];
-    R[VX] = slw * (e1 * Dd[TXX] + e2 * Dd[TXY] + e3 * Dd[TXZ]);
-    R[VY] = slw * (e1 * Dd[TXY] + e2 * Dd[TYY] + e3 * Dd[TYZ]);
-    R[VZ] = slw * (e1 * Dd[TXZ] + e2 * Dd[TYZ] + e3 * Dd[TZZ]);
-    R[TXX] = lam2mu * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam * e3 * Dd[VZ];
-    R[TYY] = lam * e1 * Dd[VX] + lam2mu * e2 * Dd[VY] + lam * e3 * Dd[VZ];
-    R[TZZ] = lam * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam2mu * e3 * Dd[VZ];
-    R[TXY] = mu * (e2 * Dd[VX] + e1 * Dd[VY]);
-    R[TXZ] = mu * (e3 * Dd[VX] + e1 * Dd[VZ]);
-    R[TYZ] = mu * (e3 * Dd[VY] + e2 * Dd[VZ]);
-    const long fso = (long)shot * F.shot_stride;
-    const TS *Pin = F.a_in + fso;
-    TC P[kNC], Ph[kNC];
-#pragma unroll
-    for (int c = 0; c < kNC; c++) {
-      P[c] = (TC)St<TS>::ld(Pin + c * F.fsize + ia) * S(c);
-      h[c] += Bm1 * R[c] - cB * P[c];
-      Ph[c] = cD * R[c] - cA * P[c];
-    }
-    if (top && dim < 2 && k == ktop) {
-      const long pl = (long)L.ni * L.nj, q = (long)j * L.ni + i;
-      const TC *A = a.mat + (dim == 0 ? 0 : 9 * pl) + q;
-      const TC dvx = A[0] * Dd[VX] + A[pl] * Dd[VY] + A[2 * pl] * Dd[VZ];
-      const TC dvy = A[3 * pl] * Dd[VX] + A[4 * pl] * Dd[VY] + A[5 * pl] * Dd[VZ];
-      const TC dvz = A[6 * pl] * Dd[VX] + A[7 * pl] * Dd[VY] + A[8 * pl] * Dd[VZ];
-      TC t[kNC];
-      t[TXX] = lam2mu * (ztx * dvx) + lam * (zty * dvy + ztz * dvz);
-      t[TYY] = lam2mu * (zty * dvy) + lam * (ztx * dvx + ztz * dvz);
-      t[TZZ] = lam2mu * (ztz * dvz) + lam * (ztx * dvx + zty * dvy);
-      t[TXY] = mu * (zty * dvx + ztx * dvy);
-      t[TXZ] = mu * (ztz * dvx + ztx * dvz);
-      t[TYZ] = mu * (ztz * dvy + zty * dvz);
-      // (coef_B - 1.0) * rhs accumulated in double (sv_curv_col_el_iso_gpu.cu:900-905)
-      const double bm1 = (double)cB - 1.0;
-#pragma unroll
-      for (int c = TXX; c < kNC; c++) {
-        h[c] = (TC)((double)h[c] + bm1 * (double)t[c]);
-        Ph[c] += cD * t[c];
-      }
-    }
-    // RK update of this face's memory variables
-    TS *Pout = F.a_out + fso, *Pend = F.a_end + fso;
-    const TS *Ppre = F.a_pre + fso;
-#pragma unroll
-    for (int c = 0; c < kNC; c++) {
-      const long q = c * F.fsize + ia;
-      const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
-      if (a.mode == MODE_RHS) {
-        St<TS>::st(Pout + q, Ph[c] * is);
-      } else if (a.mode == MODE_S0) {
-        St<TS>::st(Pout + q, (P[c] + a.ca * Ph[c]) * is);
-        St<TS>::st(Pend + q, (P[c] + a.cb * Ph[c]) * is);
-      } else if (a.mode == MODE_S12) {
-        const TC pre = (TC)St<TS>::ld(Ppre + q) * S(c);
-        const TC en = (TC)St<TS>::ld(Pend + q) * S(c);
-        St<TS>::st(Pout + q, (pre + a.ca * Ph[c]) * is);
-        St<TS>::st(Pend + q, (en + a.cb * Ph[c]) * is);
-      } else {
-        const TC en = (TC)St<TS>::ld(Pend + q) * S(c);
-        St<TS>::st(Pend + q, (en + a.cb * Ph[c]) * is);
-      }
-    }
-  }
+  pml_apply<TS, TC>(a, shot, i, j, k, Dx, Dy, Dz, xix, xiy, xiz, etx, ety, etz, ztx, zty, ztz, lam, mu, lam2mu, slw, h);
 
   // ---- RK update of the wavefield ----
-  TS *wout = a.w_out + so, *wend = a.w_end + so;
-  const TS *wpre = a.w_pre + so;
+  const unsigned qb = (unsigned)o * (unsigned)sizeof(TS);
 #pragma unroll
   for (int c = 0; c < kNC; c++) {
-    const long q = c * cs + o;
+    TS *wout = a.w_out + so + c * cs, *wend = a.w_end + so + c * cs;
+    const TS *wpre = a.w_pre + so + c * cs;
     const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
     if (a.mode == MODE_RHS) {
-      St<TS>::st(wout + q, h[c] * is);
+      St<TS>::st(&at_b(wout, qb), h[c] * is);
     } else if (a.mode == MODE_S0) {
       const TC v = W(c, 0) * S(c);
-      St<TS>::st(wout + q, (v + a.ca * h[c]) * is);
-      St<TS>::st(wend + q, (v + a.cb * h[c]) * is);
+      St<TS>::st(&at_b(wout, qb), (v + a.ca * h[c]) * is);
+      St<TS>::st(&at_b(wend, qb), (v + a.cb * h[c]) * is);
     } else if (a.mode == MODE_S12) {
-      const TC pre = (TC)St<TS>::ld(wpre + q) * S(c);
-      const TC en = (TC)St<TS>::ld(wend + q) * S(c);
-      St<TS>::st(wout + q, (pre + a.ca * h[c]) * is);
-      St<TS>::st(wend + q, (en + a.cb * h[c]) * is);
+      const TC pre = (TC)St<TS>::ld(&at_b(wpre, qb)) * S(c);
+      const TC en = (TC)St<TS>::ld(&at_b(wend, qb)) * S(c);
+      St<TS>::st(&at_b(wout, qb), (pre + a.ca * h[c]) * is);
+      St<TS>::st(&at_b(wend, qb), (en + a.cb * h[c]) * is);
     } else {
-      const TC en = (TC)St<TS>::ld(wend + q) * S(c);
-      St<TS>::st(wend + q, (en + a.cb * h[c]) * is);
+      const TC en = (TC)St<TS>::ld(&at_b(wend, qb)) * S(c);
+      St<TS>::st(&at_b(wend, qb), (en + a.cb * h[c]) * is);
     }
   }
 }

@@ -70,6 +70,9 @@ typedef struct {
 
 /* use the plain per-point kernel everywhere (no z-marching interior kernel) */
 #define CGFD_FLAG_PLAIN 1
+/* sweep the PML face slabs with the z-marching kernel's PML variant instead of
+ * the per-point kernel */
+#define CGFD_FLAG_PML_MARCH 2
 
 /* Point-source tables of one shot: the device mirror of src_t
  * (src/forward/src_t.h:14-53, filled by src_read_locate_file src_t.cu:143-790). */

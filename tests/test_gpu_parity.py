@@ -37,12 +37,12 @@ def _aux_concat(sess, P, level):
     return np.concatenate(parts) if parts else np.zeros(0)
 
 
-@pytest.mark.parametrize("plain", [True, False], ids=["plain", "zmarch"])
-def test_onestage_rhs_all_pairs(plain):
+@pytest.mark.parametrize("flags", [cgfd.FLAG_PLAIN, 0, cgfd.FLAG_PML_MARCH], ids=["plain", "zmarch", "pmlmarch"])
+def test_onestage_rhs_all_pairs(flags):
     """Every (pair, stage) RHS incl. traction image, vlow rows, PML and source."""
     P = Problem(ni=40, nj=34, nk=30, curvilinear=True, pml=(4, 4, 4, 4, 4, 0), hetero=0.05,
                 init_random=True, nt_total=6, seed=3)
-    s = _sess(P, flags=cgfd.FLAG_PLAIN if plain else 0)
+    s = _sess(P, flags=flags)
     s.run(0, 3, P.dt)  # non-zero PML memory
     w = s.fetch_level(0, cgfd.LEVEL_PRE)
     aux = _aux_concat(s, P, cgfd.LEVEL_PRE)
@@ -67,12 +67,12 @@ def test_onestage_rhs_all_pairs(plain):
     s.close()
 
 
-@pytest.mark.parametrize("plain", [True, False], ids=["plain", "zmarch"])
+@pytest.mark.parametrize("flags", [cgfd.FLAG_PLAIN, 0, cgfd.FLAG_PML_MARCH], ids=["plain", "zmarch", "pmlmarch"])
 @pytest.mark.parametrize("curv", [False, True], ids=["cart", "curv"])
-def test_run_matches_oracle_fp32(plain, curv):
+def test_run_matches_oracle_fp32(flags, curv):
     P = Problem(ni=40, nj=36, nk=32, curvilinear=curv, pml=(5, 5, 5, 5, 5, 0), nt_total=40, seed=1,
                 hetero=0.03 if curv else 0.0)
-    s = _sess(P, flags=cgfd.FLAG_PLAIN if plain else 0)
+    s = _sess(P, flags=flags)
     s.run(0, P.nt_total, P.dt)
     sg = s.read_seismo(0)
     wg = s.fetch_level(0, cgfd.LEVEL_PRE)
