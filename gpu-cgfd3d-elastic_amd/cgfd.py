@@ -27,7 +27,8 @@ EXPORTS = [
     "cgfd_upload_receivers", "cgfd_set_level", "cgfd_reset_shot", "cgfd_run",
     "cgfd_el_iso_onestage", "cgfd_read_seismo", "cgfd_fetch_level",
     "cgfd_fetch_pml_aux", "cgfd_set_timing", "cgfd_last_run_timing",
-    "cgfd_set_tile", "cgfd_get_tile",
+    "cgfd_set_tile", "cgfd_get_tile", "cgfd_set_graph", "cgfd_set_shot",
+    "cgfd_run_async", "cgfd_poll", "cgfd_wait",
     "cgfd_grid_cartesian", "cgfd_grid_topography", "cgfd_metric_calc", "cgfd_dt_estimate",
     "cgfd_media_layered", "cgfd_media_homogeneous", "cgfd_rho_to_slow", "cgfd_pml_len_dh",
     "cgfd_pml_profile", "cgfd_free_surface_matrices", "cgfd_wavelet", "cgfd_angle2moment",
@@ -97,6 +98,11 @@ def load(path=LIB_PATH):
         "cgfd_fetch_level": ([vp, i, i, i, vp], i),
         "cgfd_fetch_pml_aux": ([vp, i, i, i, i, vp], i),
         "cgfd_set_timing": ([vp, i], i),
+        "cgfd_set_graph": ([vp, i], i),
+        "cgfd_set_shot": ([vp, i, i, i], i),
+        "cgfd_run_async": ([vp, i, i, C.c_double], i),
+        "cgfd_poll": ([vp], i),
+        "cgfd_wait": ([vp], i),
         "cgfd_set_tile": ([vp, i, i], i),
         "cgfd_get_tile": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)], i),
         "cgfd_last_run_timing": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)], i),
@@ -270,6 +276,24 @@ class Session:
 
     def set_timing(self, on):
         _check(_lib.cgfd_set_timing(self.h, 1 if on else 0))
+
+    def set_graph(self, on):
+        _check(_lib.cgfd_set_graph(self.h, 1 if on else 0))
+
+    def set_shot(self, shot, active, it_start=0):
+        _check(_lib.cgfd_set_shot(self.h, shot, 1 if active else 0, it_start))
+
+    def run_async(self, it0, nsteps, dt):
+        _check(_lib.cgfd_run_async(self.h, it0, nsteps, C.c_double(dt)))
+
+    def poll(self):
+        r = _lib.cgfd_poll(self.h)
+        if r < 0:
+            _check(r)
+        return r == 0
+
+    def wait(self):
+        _check(_lib.cgfd_wait(self.h))
 
     def last_run_timing(self):
         a, b, n = C.c_double(), C.c_double(), C.c_int()

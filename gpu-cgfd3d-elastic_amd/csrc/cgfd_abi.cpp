@@ -150,6 +150,8 @@ struct cgfd_session {
   void *rhs = nullptr;     // nshots x 1 level (lazy)
   void *met = nullptr, *med = nullptr, *mat = nullptr;
   bool have_met = false, have_med = false;
+  bool met_const = false;  // all metric components constant (Cartesian grid)
+  double cmet[9] = {0};
   int free_top = 0;
   FaceHost face[6];
   int slot[4] = {0, 1, 2, 3};  // physical level slot of PRE, TMPA, TMPB, END
@@ -179,9 +181,21 @@ struct cgfd_session {
   int tile_shape = -1, tile_zchunk = 48;
   bool tile_fixed = false;
   // timing
-  bool timing = true;
+  bool timing = false;
   std::vector<hipEvent_t> ev;
   hipEvent_t ev_run0 = nullptr, ev_run1 = nullptr;
+  size_t ev_used = 0;
+  // 8-step periods captured as graphs, one per orientation of the PRE/END slots;
+  // any change of kernel arguments bumps the epoch and retires them
+  bool use_graph = true;
+  unsigned long epoch = 1;
+  struct GraphSlot {
+    hipGraphExec_t exec = nullptr;
+    unsigned long epoch = 0;
+    double dt = 0;
+    int slot0 = -1;
+  } graph[2];
+  std::vector<char> rec_changed;  // per shot: receivers replaced since the last rebuild
   double last_run_ms = 0, last_int_avg_ms = 0;
   int last_int_n = 0;
   std::vector<void *> allocs;
@@ -208,6 +222,26 @@ void dfree(cgfd_session *s, void *&p) {
   (void)hipFree(p);
   p = nullptr;
 }
+
+// stream-ordered copies; the session stream never waits on other streams
+hipError_t h2d(cgfd_session *s, void *dst, const void *src, size_t n) {
+  hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s->st);
+  return e != hipSuccess ? e : hipStreamSynchronize(s->st);
+}
+hipError_t d2h(cgfd_session *s, void *dst, const void *src, size_t n) {
+  hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s->st);
+  return e != hipSuccess ? e : hipStreamSynchronize(s->st);
+}
+
+void drop_graphs(cgfd_session *s) {
+  for (auto &g : s->graph) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = cgfd_session::GraphSlot{};
+  }
+}
+
+// kernel arguments changed: captured periods no longer apply
+void invalidate(cgfd_session *s) { s->epoch++; }
 
 Layout make_layout(int ni, int nj, int nk, size_t es) {
   Layout L;
@@ -246,11 +280,11 @@ void face_box(const cgfd_session *s, int f, int nlay, FaceHost &F) {
 // copy host model values (any dtype) into a device buffer of the model type
 int put_model(cgfd_session *s, void *dst, const std::vector<double> &v) {
   if (s->ms == 8) {
-    HIPCHK(hipMemcpy(dst, v.data(), v.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(h2d(s, dst, v.data(), v.size() * 8));
   } else {
     std::vector<float> f(v.size());
     for (size_t i = 0; i < v.size(); i++) f[i] = (float)v[i];
-    HIPCHK(hipMemcpy(dst, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(h2d(s, dst, f.data(), f.size() * 4));
   }
   return CGFD_OK;
 }
@@ -270,11 +304,11 @@ void ref_to_dev_comp(const Layout &L, int dtype, const void *src, size_t comp, s
 int put_storage(cgfd_session *s, void *dst, const std::vector<double> &v, long comp_stride) {
   const size_t n = v.size();
   if (s->d.storage == CGFD_STORE_F64) {
-    HIPCHK(hipMemcpy(dst, v.data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHK(h2d(s, dst, v.data(), n * 8));
   } else if (s->d.storage == CGFD_STORE_F32) {
     std::vector<float> f(n);
     for (size_t i = 0; i < n; i++) f[i] = (float)v[i];
-    HIPCHK(hipMemcpy(dst, f.data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(h2d(s, dst, f.data(), n * 4));
   } else {
     std::vector<uint16_t> h(n);
     const bool bf = s->d.storage == CGFD_STORE_BF16;
@@ -283,7 +317,7 @@ int put_storage(cgfd_session *s, void *dst, const std::vector<double> &v, long c
       float x = (float)v[i] * s->isc[c];
       h[i] = bf ? f2bf(x) : f2h(x);
     }
-    HIPCHK(hipMemcpy(dst, h.data(), n * 2, hipMemcpyHostToDevice));
+    HIPCHK(h2d(s, dst, h.data(), n * 2));
   }
   return CGFD_OK;
 }
@@ -291,14 +325,14 @@ int put_storage(cgfd_session *s, void *dst, const std::vector<double> &v, long c
 int get_storage(cgfd_session *s, const void *src, size_t n, long comp_stride, std::vector<double> &v) {
   v.resize(n);
   if (s->d.storage == CGFD_STORE_F64) {
-    HIPCHK(hipMemcpy(v.data(), src, n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(d2h(s, v.data(), src, n * 8));
   } else if (s->d.storage == CGFD_STORE_F32) {
     std::vector<float> f(n);
-    HIPCHK(hipMemcpy(f.data(), src, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(d2h(s, f.data(), src, n * 4));
     for (size_t i = 0; i < n; i++) v[i] = f[i];
   } else {
     std::vector<uint16_t> h(n);
-    HIPCHK(hipMemcpy(h.data(), src, n * 2, hipMemcpyDeviceToHost));
+    HIPCHK(d2h(s, h.data(), src, n * 2));
     const bool bf = s->d.storage == CGFD_STORE_BF16;
     for (size_t i = 0; i < n; i++) {
       int c = comp_stride > 0 ? (int)(i / comp_stride) % 9 : 0;
@@ -377,7 +411,7 @@ template <typename TM> int build_sources(cgfd_session *s) {
   auto up = [&](void **dst, const void *src, size_t bytes) -> int {
     int r = dalloc(s, dst, bytes);
     if (r) return r;
-    if (bytes) HIPCHK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    if (bytes) HIPCHK(h2d(s, *dst, src, bytes));
     return CGFD_OK;
   };
   int r;
@@ -396,16 +430,23 @@ template <typename TM> int build_sources(cgfd_session *s) {
   if ((r = up(&s->s_F, Fall.data(), Fall.size() * sizeof(TM)))) return r;
   if ((r = up(&s->s_M, Mall.data(), Mall.size() * sizeof(TM)))) return r;
   s->src_dirty = false;
+  invalidate(s);
   return CGFD_OK;
 }
 
+// Rebuild the receiver tables of all shots.  Seismograms of shots whose
+// receivers did not change are carried over, so a shot can be given new
+// receivers while the others keep what they recorded.
 template <typename TC> int build_receivers(cgfd_session *s) {
-  for (void *p : {(void *)s->r_shot, (void *)s->r_nt, (void *)s->r_off, (void *)s->r_out, s->r_frac, s->seismo}) {
+  for (void *p : {(void *)s->r_shot, (void *)s->r_nt, (void *)s->r_off, (void *)s->r_out, s->r_frac}) {
     if (p) { void *q = p; dfree(s, q); }
   }
   s->r_shot = s->r_nt = nullptr;
   s->r_off = s->r_out = nullptr;
-  s->r_frac = s->seismo = nullptr;
+  s->r_frac = nullptr;
+  void *old_seismo = s->seismo;
+  const std::vector<long> old_off = s->seismo_off;
+  s->seismo = nullptr;
   std::vector<int> shot_of, nt_of;
   std::vector<long> off, out;
   std::vector<TC> frac;
@@ -425,11 +466,17 @@ template <typename TC> int build_receivers(cgfd_session *s) {
   }
   s->r_n = (int)shot_of.size();
   s->seismo_elems = total;
-  if (s->r_n == 0) { s->rec_dirty = false; return CGFD_OK; }
+  invalidate(s);
+  if (s->r_n == 0) {
+    if (old_seismo) dfree(s, old_seismo);
+    s->rec_dirty = false;
+    std::fill(s->rec_changed.begin(), s->rec_changed.end(), 0);
+    return CGFD_OK;
+  }
   auto up = [&](void **dst, const void *src, size_t bytes) -> int {
     int r = dalloc(s, dst, bytes);
     if (r) return r;
-    HIPCHK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    HIPCHK(h2d(s, *dst, src, bytes));
     return CGFD_OK;
   };
   int r;
@@ -439,7 +486,18 @@ template <typename TC> int build_receivers(cgfd_session *s) {
   if ((r = up((void **)&s->r_out, out.data(), out.size() * 8))) return r;
   if ((r = up(&s->r_frac, frac.data(), frac.size() * sizeof(TC)))) return r;
   if ((r = dalloc(s, &s->seismo, (size_t)total * sizeof(TC)))) return r;
-  HIPCHK(hipMemset(s->seismo, 0, (size_t)total * sizeof(TC)));
+  if ((r = fill_zero(s, s->seismo, (size_t)total, sizeof(TC)))) return r;
+  if (old_seismo) {
+    for (int shot = 0; shot < (int)s->rec.size(); shot++) {
+      const size_t n = (size_t)s->rec[shot].nrecv * kNC * s->rec[shot].nt;
+      if (s->rec_changed[shot] || n == 0) continue;
+      HIPCHK(hipMemcpyAsync((TC *)s->seismo + s->seismo_off[shot], (const TC *)old_seismo + old_off[shot],
+                            n * sizeof(TC), hipMemcpyDeviceToDevice, s->st));
+    }
+    HIPCHK(hipStreamSynchronize(s->st));
+    dfree(s, old_seismo);
+  }
+  std::fill(s->rec_changed.begin(), s->rec_changed.end(), 0);
   s->rec_dirty = false;
   return CGFD_OK;
 }
@@ -453,6 +511,8 @@ StageArgs<TS, TC> stage_args(cgfd_session *s, int slot_in, int slot_pre, int slo
   a.med = (const TC *)s->med;
   a.mat = (const TC *)s->mat;
   a.free_top = s->free_top;
+  a.met_const = s->met_const ? 1 : 0;
+  for (int c = 0; c < 9; c++) a.cmet[c] = (TC)s->cmet[c];
   const long aux_lvl_shot = 5;
   for (int f = 0; f < 6; f++) {
     FaceHost &F = s->face[f];
@@ -645,28 +705,15 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
   s->tile_shape = bs;
   s->tile_zchunk = bz;
   g_tile_cache[key] = {bs, bz};
+  invalidate(s);
   return CGFD_OK;
 }
 
-template <typename TS, typename TC> int run_t(cgfd_session *s, int it0, int nsteps, double dt_in) {
-  if (!s->tile_fixed && s->tile_shape < 0) {
-    int r = autotune_t<TS, TC>(s);
-    if (r) return r;
-  }
-  if (s->src_dirty) { int r = build_sources<TC>(s); if (r) return r; }
-  if (s->rec_dirty) { int r = build_receivers<TC>(s); if (r) return r; }
-  const Shell sh = make_shell(s);
-  const TC dt = (TC)dt_in;
-  HIPCHK(hipMemcpyAsync(s->clock, &it0, sizeof(int), hipMemcpyHostToDevice, s->st));
-  const size_t nev_need = (size_t)(sh.inner.nbox && s->timing ? 2 * 4 * nsteps : 0);
-  if (s->timing && s->ev.size() < nev_need) {
-    size_t old = s->ev.size();
-    s->ev.resize(nev_need);
-    for (size_t e = old; e < nev_need; e++) HIPCHK(hipEventCreate(&s->ev[e]));
-  }
-  if (!s->ev_run0) { HIPCHK(hipEventCreate(&s->ev_run0)); HIPCHK(hipEventCreate(&s->ev_run1)); }
-  HIPCHK(hipEventRecord(s->ev_run0, s->st));
-  size_t ne = 0;
+// Enqueue steps it0 .. it0+nsteps-1 of the RK4 loop on the session stream:
+// receivers, four stages (interior sweep, PML slabs / free-surface rows,
+// source) and the clock advance per step, then the PRE <-> END swap.
+template <typename TS, typename TC>
+int enqueue_steps(cgfd_session *s, const Shell &sh, int it0, int nsteps, TC dt, bool timed) {
   for (int n = 0; n < nsteps; n++) {
     const int it = it0 + n;
     // receivers read W^n before the step (drv_rk_curv_col.cu:195-216)
@@ -697,12 +744,15 @@ template <typename TS, typename TC> int run_t(cgfd_session *s, int it0, int nste
       // coef = rk_a[istage]*dt, both float in the reference (drv_rk_curv_col.cu:341-342)
       a.ca = (istage < 3) ? (TC)kRkA[istage] * dt : TC(0);
       a.cb = (TC)kRkB[istage] * dt;
-      const bool tim = s->timing && sh.inner.nbox;
-      int r = launch_stage<TS, TC>(s, sh, a, dx, dy, dz, s->d.nshots, tim ? s->ev[ne] : nullptr,
-                                   tim ? s->ev[ne + 1] : nullptr);
+      const bool tim = timed && sh.inner.nbox;
+      if (tim && s->ev.size() < s->ev_used + 2) {
+        s->ev.resize(s->ev_used + 2);
+        for (size_t q = s->ev_used; q < s->ev.size(); q++) HIPCHK(hipEventCreate(&s->ev[q]));
+      }
+      int r = launch_stage<TS, TC>(s, sh, a, dx, dy, dz, s->d.nshots, tim ? s->ev[s->ev_used] : nullptr,
+                                   tim ? s->ev[s->ev_used + 1] : nullptr);
       if (r) return r;
-      if (tim) ne += 2;
-      hipError_t e;
+      if (tim) s->ev_used += 2;
       if (s->s_npt > 0) {
         SrcArgs<TS, TC> sa{};
         sa.L = s->L; sa.npt = s->s_npt; sa.pt_shot = s->s_pt_shot; sa.pt_off = s->s_pt_off; sa.pt_beg = s->s_pt_beg;
@@ -712,8 +762,9 @@ template <typename TS, typename TC> int run_t(cgfd_session *s, int it0, int nste
         sa.w_out = a.w_out; sa.w_end = a.w_end; sa.shot_stride = s->shot_elems;
         sa.mode = mode; sa.istage = istage; sa.ca = a.ca; sa.cb = a.cb;
         sa.clock = s->clock; sa.shot_start = s->shot_start; sa.shot_active = s->shot_active;
+        sa.only_shot = -1; sa.fixed_it = 0;
         for (int c = 0; c < kNC; c++) sa.isc[c] = s->isc[c];
-        e = launch_source<TS, TC>(sa, s->st);
+        hipError_t e = launch_source<TS, TC>(sa, s->st);
         if (e != hipSuccess) return fail(CGFD_E_HIP, "source kernel: %s", hipGetErrorString(e));
       }
     }
@@ -721,18 +772,85 @@ template <typename TS, typename TC> int run_t(cgfd_session *s, int it0, int nste
     if (e != hipSuccess) return fail(CGFD_E_HIP, "clock kernel: %s", hipGetErrorString(e));
     std::swap(s->slot[0], s->slot[3]);  // pre <-> end (drv_rk_curv_col.cu:529-541)
   }
+  return CGFD_OK;
+}
+
+// The graph of one 8-step period (it0 % 8 == 0) for the current PRE/END
+// orientation; captured on first use.  Eight swaps bring the slots back, and
+// the step number lives in device memory, so one capture serves every period.
+template <typename TS, typename TC>
+int period_graph(cgfd_session *s, const Shell &sh, TC dt, double dt_key, hipGraphExec_t *out) {
+  cgfd_session::GraphSlot &g = s->graph[s->slot[0] == 0 ? 0 : 1];
+  if (g.exec && g.epoch == s->epoch && g.dt == dt_key && g.slot0 == s->slot[0]) {
+    *out = g.exec;
+    return CGFD_OK;
+  }
+  if (g.exec) { (void)hipGraphExecDestroy(g.exec); g.exec = nullptr; }
+  const int slot0 = s->slot[0];
+  HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+  int r = enqueue_steps<TS, TC>(s, sh, 0, 8, dt, false);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(s->st, &graph);
+  if (r) { if (graph) (void)hipGraphDestroy(graph); return r; }
+  if (e != hipSuccess) return fail(CGFD_E_HIP, "graph capture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) { g.exec = nullptr; return fail(CGFD_E_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
+  g.epoch = s->epoch; g.dt = dt_key; g.slot0 = slot0;
+  *out = g.exec;
+  return CGFD_OK;
+}
+
+// Enqueue a run without waiting for it.  Whole 8-step periods go through the
+// captured graphs unless per-launch timing is on.
+template <typename TS, typename TC> int enqueue_t(cgfd_session *s, int it0, int nsteps, double dt_in) {
+  if (!s->tile_fixed && s->tile_shape < 0) {
+    int r = autotune_t<TS, TC>(s);
+    if (r) return r;
+  }
+  if (s->src_dirty) { int r = build_sources<TC>(s); if (r) return r; }
+  if (s->rec_dirty) { int r = build_receivers<TC>(s); if (r) return r; }
+  const Shell sh = make_shell(s);
+  const TC dt = (TC)dt_in;
+  hipError_t e = launch_fill<int>(s->clock, 1, it0, s->st);
+  if (e != hipSuccess) return fail(CGFD_E_HIP, "clock set: %s", hipGetErrorString(e));
+  s->ev_used = 0;
+  int it = it0, left = nsteps;
+  const bool graphs = s->use_graph && !s->timing;
+  while (left > 0) {
+    if (graphs && it % 8 == 0 && left >= 8) {
+      hipGraphExec_t g = nullptr;
+      int r = period_graph<TS, TC>(s, sh, dt, dt_in, &g);
+      if (r) return r;
+      HIPCHK(hipGraphLaunch(g, s->st));
+      it += 8; left -= 8;
+    } else {
+      const int n = graphs ? std::min(left, 8 - it % 8) : left;
+      int r = enqueue_steps<TS, TC>(s, sh, it, n, dt, s->timing);
+      if (r) return r;
+      it += n; left -= n;
+    }
+  }
+  return CGFD_OK;
+}
+
+template <typename TS, typename TC> int run_t(cgfd_session *s, int it0, int nsteps, double dt_in) {
+  if (!s->ev_run0) { HIPCHK(hipEventCreate(&s->ev_run0)); HIPCHK(hipEventCreate(&s->ev_run1)); }
+  HIPCHK(hipEventRecord(s->ev_run0, s->st));
+  int r = enqueue_t<TS, TC>(s, it0, nsteps, dt_in);
+  if (r) { (void)hipStreamSynchronize(s->st); return r; }
   HIPCHK(hipEventRecord(s->ev_run1, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, s->ev_run0, s->ev_run1));
   s->last_run_ms = ms;
   double tot = 0;
-  for (size_t q = 0; q + 1 < ne; q += 2) {
+  for (size_t q = 0; q + 1 < s->ev_used; q += 2) {
     float m = 0;
     HIPCHK(hipEventElapsedTime(&m, s->ev[q], s->ev[q + 1]));
     tot += m;
   }
-  s->last_int_n = (int)(ne / 2);
+  s->last_int_n = (int)(s->ev_used / 2);
   s->last_int_avg_ms = s->last_int_n ? tot / s->last_int_n : 0.0;
   return CGFD_OK;
 }
@@ -766,29 +884,21 @@ template <typename TS, typename TC> int onestage_t(cgfd_session *s, int shot, in
   const Shell sh = make_shell(s);
   int r = launch_stage<TS, TC>(s, sh, a, dx, dy, dz, 1, nullptr, nullptr);
   if (r) return r;
-  hipError_t e;
   if (s->s_npt > 0) {
-    // the source kernel scans all shots' points; keep only this shot's via the active mask
-    std::vector<int> act(s->d.nshots, 0);
-    act[shot] = 1;
-    HIPCHK(hipMemcpyAsync(s->shot_active, act.data(), act.size() * 4, hipMemcpyHostToDevice, s->st));
-    HIPCHK(hipMemcpyAsync(s->clock, &it, sizeof(int), hipMemcpyHostToDevice, s->st));
-    std::vector<int> zero(s->d.nshots, 0);
-    HIPCHK(hipMemcpyAsync(s->shot_start, zero.data(), zero.size() * 4, hipMemcpyHostToDevice, s->st));
+    // the source kernel scans all shots' points; inject this shot's at step it
     SrcArgs<TS, TC> sa{};
     sa.L = s->L; sa.npt = s->s_npt; sa.pt_shot = s->s_pt_shot; sa.pt_off = s->s_pt_off; sa.pt_beg = s->s_pt_beg;
     sa.ct_src = s->s_ct_src; sa.ct_coef = (const TC *)s->s_ct_coef; sa.s_itb = s->s_itb; sa.s_ite = s->s_ite;
     sa.s_tab = s->s_tab; sa.s_kind = s->s_kind; sa.F = (const TC *)s->s_F; sa.M = (const TC *)s->s_M;
     sa.ntab = s->s_ntab; sa.met = (const TC *)s->met; sa.med = (const TC *)s->med;
-    sa.w_out = (TS *)s->rhs - (long)shot * 0; sa.w_end = (TS *)s->rhs;
+    sa.w_out = (TS *)s->rhs; sa.w_end = (TS *)s->rhs;
     sa.shot_stride = 0;
     sa.mode = MODE_RHS; sa.istage = istage; sa.ca = TC(0); sa.cb = TC(0);
     sa.clock = s->clock; sa.shot_start = s->shot_start; sa.shot_active = s->shot_active;
+    sa.only_shot = shot; sa.fixed_it = it;
     for (int c = 0; c < kNC; c++) sa.isc[c] = s->isc[c];
-    e = launch_source<TS, TC>(sa, s->st);
+    hipError_t e = launch_source<TS, TC>(sa, s->st);
     if (e != hipSuccess) return fail(CGFD_E_HIP, "source kernel: %s", hipGetErrorString(e));
-    std::vector<int> one(s->d.nshots, 1);
-    HIPCHK(hipMemcpyAsync(s->shot_active, one.data(), one.size() * 4, hipMemcpyHostToDevice, s->st));
   }
   HIPCHK(hipStreamSynchronize(s->st));
   return CGFD_OK;
@@ -843,7 +953,8 @@ int cgfd_create(const cgfd_desc *d, cgfd_session **out) {
     cgfd_destroy(s);
     return r;
   };
-  if (hipStreamCreate(&s->st) != hipSuccess) return cleanup(fail(CGFD_E_HIP, "hipStreamCreate failed"));
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(CGFD_E_HIP, "hipStreamCreate failed"));
   int r;
   if ((r = dalloc(s, &s->wav, (size_t)d->nshots * s->shot_elems * s->es))) return cleanup(r);
   if ((r = dalloc(s, &s->met, (size_t)NMET * s->L.cs * s->ms))) return cleanup(r);
@@ -856,13 +967,13 @@ int cgfd_create(const cgfd_desc *d, cgfd_session **out) {
   if ((r = fill_zero(s, s->met, (size_t)NMET * s->L.cs, s->ms))) return cleanup(r);
   if ((r = fill_zero(s, s->med, (size_t)3 * s->L.cs, s->ms))) return cleanup(r);
   if ((r = fill_zero(s, s->mat, (size_t)18 * d->ni * d->nj, s->ms))) return cleanup(r);
-  std::vector<int> zero(d->nshots, 0), one(d->nshots, 1);
-  if (hipMemcpy(s->shot_start, zero.data(), zero.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(s->shot_active, one.data(), one.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(s->clock, zero.data(), 4, hipMemcpyHostToDevice) != hipSuccess)
-    return cleanup(fail(CGFD_E_HIP, "initial copies failed"));
+  if (launch_fill<int>(s->shot_start, d->nshots, 0, s->st) != hipSuccess ||
+      launch_fill<int>(s->shot_active, d->nshots, 1, s->st) != hipSuccess ||
+      launch_fill<int>(s->clock, 1, 0, s->st) != hipSuccess)
+    return cleanup(fail(CGFD_E_HIP, "initial fills failed"));
   s->src.resize(d->nshots);
   s->rec.resize(d->nshots);
+  s->rec_changed.assign(d->nshots, 0);
   if (hipStreamSynchronize(s->st) != hipSuccess) return cleanup(fail(CGFD_E_HIP, "stream sync failed"));
   *out = s;
   return CGFD_OK;
@@ -871,6 +982,7 @@ int cgfd_create(const cgfd_desc *d, cgfd_session **out) {
 void cgfd_destroy(cgfd_session *s) {
   if (!s) return;
   if (s->st) (void)hipStreamSynchronize(s->st);
+  drop_graphs(s);
   for (void *p : s->allocs) (void)hipFree(p);
   s->allocs.clear();
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
@@ -887,6 +999,17 @@ int cgfd_upload_metric(cgfd_session *s, int dtype, const void *metric) {
   int r = put_model(s, s->met, v);
   if (r) return r;
   s->have_met = true;
+  // a Cartesian grid gives one value per metric component everywhere (ghosts too)
+  const size_t n = (size_t)(s->d.ni + 6) * (s->d.nj + 6) * (s->d.nk + 6);
+  bool cst = true;
+  for (int c = 0; c < 9 && cst; c++) {
+    const double v0 = rd(dtype, metric, (size_t)(c + 1) * n);
+    for (size_t q = 1; q < n; q++)
+      if (rd(dtype, metric, (size_t)(c + 1) * n + q) != v0) { cst = false; break; }
+    s->cmet[c] = v0;
+  }
+  s->met_const = cst;
+  invalidate(s);
   return CGFD_OK;
 }
 
@@ -899,6 +1022,7 @@ int cgfd_upload_medium(cgfd_session *s, int dtype, const void *lam, const void *
   int r = put_model(s, s->med, v);
   if (r) return r;
   s->have_med = true;
+  invalidate(s);
   return CGFD_OK;
 }
 
@@ -907,6 +1031,7 @@ int cgfd_upload_free_surface(cgfd_session *s, int free_top, int dtype, const voi
   if (free_top && s->face[5].on)
     return fail(CGFD_E_ARG, "the top cannot be both a free surface and a PML face");
   s->free_top = free_top ? 1 : 0;
+  invalidate(s);
   if (!free_top) return CGFD_OK;
   if (!mvx || !mvy) return fail(CGFD_E_ARG, "free surface enabled without matrices");
   const int ni = s->d.ni, nj = s->d.nj;
@@ -927,6 +1052,7 @@ int cgfd_upload_free_surface(cgfd_session *s, int free_top, int dtype, const voi
 int cgfd_upload_pml(cgfd_session *s, int face, int nlay, int dtype, const void *A, const void *B, const void *D) {
   if (!s || face < 0 || face > 5) return fail(CGFD_E_ARG, "bad session or face %d", face);
   FaceHost &F = s->face[face];
+  invalidate(s);
   if (F.coef) dfree(s, F.coef);
   if (F.aux) dfree(s, F.aux);
   F.on = false;
@@ -964,6 +1090,7 @@ int cgfd_set_storage_scale(cgfd_session *s, const int exp2[9]) {
     s->sc[c] = std::ldexp(1.0f, exp2[c]);
     s->isc[c] = std::ldexp(1.0f, -exp2[c]);
   }
+  invalidate(s);
   return CGFD_OK;
 }
 
@@ -1019,6 +1146,7 @@ int cgfd_upload_receivers(cgfd_session *s, int shot, int nrecv, const long long 
   h.frac.resize((size_t)nrecv * 3);
   for (size_t q = 0; q < h.frac.size(); q++) h.frac[q] = rd(dtype, frac3, q);
   s->rec[shot] = std::move(h);
+  s->rec_changed[shot] = 1;
   s->rec_dirty = true;
   return CGFD_OK;
 }
@@ -1080,14 +1208,14 @@ int cgfd_read_seismo(cgfd_session *s, int shot, int dtype, void *out) {
   const size_t off = (size_t)s->seismo_off[shot];
   if (s->ms == 8) {
     std::vector<double> v(n);
-    HIPCHK(hipMemcpy(v.data(), (double *)s->seismo + off, n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(d2h(s, v.data(), (double *)s->seismo + off, n * 8));
     for (size_t q = 0; q < n; q++) {
       if (dtype == CGFD_DT_F64) ((double *)out)[q] = v[q];
       else ((float *)out)[q] = (float)v[q];
     }
   } else {
     std::vector<float> v(n);
-    HIPCHK(hipMemcpy(v.data(), (float *)s->seismo + off, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(d2h(s, v.data(), (float *)s->seismo + off, n * 4));
     for (size_t q = 0; q < n; q++) {
       if (dtype == CGFD_DT_F64) ((double *)out)[q] = v[q];
       else ((float *)out)[q] = v[q];
@@ -1144,6 +1272,7 @@ int cgfd_fetch_pml_aux(cgfd_session *s, int shot, int face, int level, int dtype
 
 int cgfd_set_tile(cgfd_session *s, int shape, int zchunk) {
   if (!s) return fail(CGFD_E_ARG, "null session");
+  invalidate(s);
   if (shape < 0) {  // back to automatic selection
     s->tile_fixed = false;
     s->tile_shape = -1;
@@ -1170,6 +1299,52 @@ int cgfd_get_tile(cgfd_session *s, int *shape, int *tx, int *ty, int *zchunk) {
 int cgfd_set_timing(cgfd_session *s, int on) {
   if (!s) return fail(CGFD_E_ARG, "null session");
   s->timing = on != 0;
+  return CGFD_OK;
+}
+
+int cgfd_set_graph(cgfd_session *s, int on) {
+  if (!s) return fail(CGFD_E_ARG, "null session");
+  s->use_graph = on != 0;
+  return CGFD_OK;
+}
+
+int cgfd_set_shot(cgfd_session *s, int shot, int active, int it_start) {
+  if (!s || shot < 0 || shot >= s->d.nshots) return fail(CGFD_E_ARG, "bad session or shot %d", shot);
+  if (active && (it_start < 0 || it_start % 8 != 0))
+    return fail(CGFD_E_ARG, "shot start %d must be a non-negative multiple of 8", it_start);
+  hipError_t e = launch_fill<int>(s->shot_active + shot, 1, active ? 1 : 0, s->st);
+  if (e == hipSuccess && active) e = launch_fill<int>(s->shot_start + shot, 1, it_start, s->st);
+  if (e != hipSuccess) return fail(CGFD_E_HIP, "shot state: %s", hipGetErrorString(e));
+  return CGFD_OK;
+}
+
+int cgfd_run_async(cgfd_session *s, int it0, int nsteps, double dt) {
+  if (!s) return fail(CGFD_E_ARG, "null session");
+  if (!s->have_met || !s->have_med) return fail(CGFD_E_STATE, "metric and medium must be uploaded before running");
+  if (nsteps < 0 || it0 < 0) return fail(CGFD_E_ARG, "bad step range");
+  if (!(dt > 0.0)) return fail(CGFD_E_ARG, "dt must be positive");
+  if (s->timing) return fail(CGFD_E_STATE, "per-launch timing needs the synchronous cgfd_run");
+  int r;
+  switch (s->d.storage) {
+    case CGFD_STORE_F32: r = enqueue_t<float, float>(s, it0, nsteps, dt); break;
+    case CGFD_STORE_F64: r = enqueue_t<double, double>(s, it0, nsteps, dt); break;
+    case CGFD_STORE_F16: r = enqueue_t<__half, float>(s, it0, nsteps, dt); break;
+    default: r = enqueue_t<__hip_bfloat16, float>(s, it0, nsteps, dt); break;
+  }
+  return r;
+}
+
+int cgfd_poll(cgfd_session *s) {
+  if (!s) return fail(CGFD_E_ARG, "null session");
+  hipError_t e = hipStreamQuery(s->st);
+  if (e == hipSuccess) return 0;
+  if (e == hipErrorNotReady) return 1;
+  return fail(CGFD_E_HIP, "stream query: %s", hipGetErrorString(e));
+}
+
+int cgfd_wait(cgfd_session *s) {
+  if (!s) return fail(CGFD_E_ARG, "null session");
+  HIPCHK(hipStreamSynchronize(s->st));
   return CGFD_OK;
 }
 

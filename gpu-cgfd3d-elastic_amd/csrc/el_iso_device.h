@@ -96,6 +96,8 @@ template <typename TS, typename TM> struct StageArgs {
   const TM *med;   // lam, mu, 1/rho, stride L.cs
   const TM *mat;   // 18 components [c][j][i] on the physical top plane
   int free_top;
+  int met_const;   // 1: every metric component is one constant (Cartesian grid)
+  TM cmet[9];      // xi_x..zeta_z when met_const
   FaceArgs<TS, TM> f[6];
   TS *w_in, *w_pre, *w_out, *w_end;  // shot-0 level bases
   long shot_stride;
@@ -132,6 +134,8 @@ template <typename TS, typename TM> struct SrcArgs {
   const int *clock;         // current global step
   const int *shot_start;    // per-shot start step
   const int *shot_active;
+  int only_shot;            // >= 0: inject this shot only, at step fixed_it
+  int fixed_it;
   float isc[kNC];
 };
 

@@ -17,6 +17,9 @@
 // is issued before the first barrier, so one memory latency is exposed per
 // plane instead of one per dependent load.
 //
+// On a Cartesian grid every metric component is one constant (CART = true):
+// the nine metric values come in as arguments instead of nine HBM streams.
+//
 // The interior variant (PML = false) sweeps the box outside every PML face
 // box; the PML variant also keeps the raw derivatives and applies the CFS-PML
 // ADE terms and memory-variable update to the face slabs.  Points within 4
@@ -27,7 +30,7 @@
 
 namespace cgfd {
 
-template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINB, bool PML>
+template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINB, bool PML, bool CART>
 __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, MarchSet b, MarchGrid g) {
   constexpr int NT = TX * TY;
   constexpr int HXL = (DX == 0) ? 3 : 1;
@@ -141,7 +144,7 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
     for (int p = 0; p < NPER; p++) n_hv[p] = gld(h_c[p], pk + h_off[p]);
     const unsigned ob = o * (unsigned)sizeof(TC);
 #pragma unroll
-    for (int m = 0; m < 9; m++) n_met[m] = at_b(a.met + (XIX + m) * cs, ob);
+    for (int m = 0; m < 9; m++) n_met[m] = CART ? a.cmet[m] : at_b(a.met + (XIX + m) * cs, ob);
 #pragma unroll
     for (int m = 0; m < 3; m++) n_met[9 + m] = at_b(a.med + m * cs, ob);
     const unsigned qb = o * (unsigned)sizeof(TS);
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
   }
 }
 
-template <typename TS, typename TC, int TX, int TY, int MINB, bool PML>
+template <typename TS, typename TC, int TX, int TY, int MINB, bool PML, bool CART>
 static hipError_t launch_march_t(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz,
                                  int nshots, hipStream_t st) {
   MarchGrid g{};
@@ -276,7 +279,7 @@ static hipError_t launch_march_t(const StageArgs<TS, TC> &a, const MarchSet &b, 
   dim3 grid((unsigned)nwg), block(TX * TY);
   const int sel = dx * 4 + dy * 2 + dz;
 #define CGFD_L(X, Y, Z) \
-  hipLaunchKernelGGL((k_march<X, Y, Z, TS, TC, TX, TY, MINB, PML>), grid, block, 0, st, a, b, g)
+  hipLaunchKernelGGL((k_march<X, Y, Z, TS, TC, TX, TY, MINB, PML, CART>), grid, block, 0, st, a, b, g)
   switch (sel) {
     case 0: CGFD_L(0, 0, 0); break;
     case 1: CGFD_L(0, 0, 1); break;
@@ -305,11 +308,19 @@ template <typename TS, typename TC>
 hipError_t launch_interior(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
                            hipStream_t st) {
   if (b.nbox < 1) return hipSuccess;
+  if (a.met_const) {
+    switch (b.shape) {
+      case 1: return launch_march_t<TS, TC, 32, 8, 2, false, true>(a, b, dx, dy, dz, nshots, st);
+      case 2: return launch_march_t<TS, TC, 64, 8, 1, false, true>(a, b, dx, dy, dz, nshots, st);
+      case 3: return launch_march_t<TS, TC, 32, 16, 1, false, true>(a, b, dx, dy, dz, nshots, st);
+      default: return launch_march_t<TS, TC, 64, 4, 2, false, true>(a, b, dx, dy, dz, nshots, st);
+    }
+  }
   switch (b.shape) {
-    case 1: return launch_march_t<TS, TC, 32, 8, 2, false>(a, b, dx, dy, dz, nshots, st);
-    case 2: return launch_march_t<TS, TC, 64, 8, 1, false>(a, b, dx, dy, dz, nshots, st);
-    case 3: return launch_march_t<TS, TC, 32, 16, 1, false>(a, b, dx, dy, dz, nshots, st);
-    default: return launch_march_t<TS, TC, 64, 4, 2, false>(a, b, dx, dy, dz, nshots, st);
+    case 1: return launch_march_t<TS, TC, 32, 8, 2, false, false>(a, b, dx, dy, dz, nshots, st);
+    case 2: return launch_march_t<TS, TC, 64, 8, 1, false, false>(a, b, dx, dy, dz, nshots, st);
+    case 3: return launch_march_t<TS, TC, 32, 16, 1, false, false>(a, b, dx, dy, dz, nshots, st);
+    default: return launch_march_t<TS, TC, 64, 4, 2, false, false>(a, b, dx, dy, dz, nshots, st);
   }
 }
 
@@ -319,7 +330,7 @@ template <typename TS, typename TC>
 hipError_t launch_pml_march(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
                             hipStream_t st) {
   if (b.nbox < 1) return hipSuccess;
-  return launch_march_t<TS, TC, 16, 16, 1, true>(a, b, dx, dy, dz, nshots, st);
+  return launch_march_t<TS, TC, 16, 16, 1, true, false>(a, b, dx, dy, dz, nshots, st);
 }
 
 #define CGFD_INST(TS, TC)                                                                                     \

@@ -202,8 +202,9 @@ __global__ __launch_bounds__(256) void k_source(SrcArgs<TS, TC> a) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.npt) return;
   const int shot = a.pt_shot[p];
-  if (a.shot_active && !a.shot_active[shot]) return;
-  const int it = *a.clock - a.shot_start[shot];
+  if (a.only_shot >= 0 && shot != a.only_shot) return;
+  if (a.only_shot < 0 && !a.shot_active[shot]) return;
+  const int it = a.only_shot >= 0 ? a.fixed_it : *a.clock - a.shot_start[shot];
   const long o = a.pt_off[p];
   const long cs = a.L.cs;
   const TC jac = a.met[JAC * cs + o], slw = a.med[2 * cs + o];
@@ -344,5 +345,6 @@ CGFD_INST(__hip_bfloat16, float)
 template hipError_t launch_fill<float>(float *, long, float, hipStream_t);
 template hipError_t launch_fill<double>(double *, long, double, hipStream_t);
 template hipError_t launch_fill<uint16_t>(uint16_t *, long, uint16_t, hipStream_t);
+template hipError_t launch_fill<int>(int *, long, int, hipStream_t);
 
 }  // namespace cgfd

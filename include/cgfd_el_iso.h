@@ -181,8 +181,30 @@ int cgfd_fetch_pml_aux(cgfd_session *s, int shot, int face, int level, int dtype
 int cgfd_set_tile(cgfd_session *s, int shape, int zchunk);
 int cgfd_get_tile(cgfd_session *s, int *shape, int *tx, int *ty, int *zchunk);
 
-/* Enable (default) or disable HIP-event timing of interior kernel launches. */
+/* Enable or disable (default) HIP-event timing of every interior kernel
+ * launch.  Timed runs are enqueued launch by launch, without graphs. */
 int cgfd_set_timing(cgfd_session *s, int on);
+
+/* Run whole 8-step periods (starting at a step that is a multiple of 8) as
+ * one captured hipGraph each (default on); no reference counterpart, the
+ * reference launches and synchronises per kernel. */
+int cgfd_set_graph(cgfd_session *s, int on);
+
+/* Continuous batching of shots (no reference counterpart).  A shot with
+ * active = 1 advances with every later run, its own step number being the
+ * session step minus it_start; its source and receivers use that number, and
+ * it_start must be a multiple of 8 so that its operator pairs are those of a
+ * run started at 0.  Inactive shots are skipped by every kernel.  Pair with
+ * cgfd_reset_shot and cgfd_upload_source / cgfd_upload_receivers (which keep
+ * the other shots' seismograms) to admit a new shot between runs. */
+int cgfd_set_shot(cgfd_session *s, int shot, int active, int it_start);
+
+/* cgfd_run without waiting: enqueue on the session stream and return.
+ * cgfd_poll returns 0 when all enqueued work is done, 1 while it runs;
+ * cgfd_wait blocks until it is done.  Timing must be off. */
+int cgfd_run_async(cgfd_session *s, int it0, int nsteps, double dt);
+int cgfd_poll(cgfd_session *s);
+int cgfd_wait(cgfd_session *s);
 
 /* Milliseconds of device time spent in the last cgfd_run, and the average
  * duration of its z-marching interior kernel launches (events on the

@@ -236,18 +236,22 @@ class Problem:
                 nl, A, B, D = self.pml[f]
                 sess.upload_pml(f, nl, np.asarray(A, dtype), np.asarray(B, dtype), np.asarray(D, dtype))
         for shot in (range(sess.nshots) if shots is None else shots):
-            if self.src is not None:
-                s = dict(self.src)
-                s["ext_coef"] = np.asarray(s["ext_coef"], dtype)
-                if s["F"] is not None:
-                    s["F"] = np.asarray(s["F"], dtype)
-                if s["M"] is not None:
-                    s["M"] = np.asarray(s["M"], dtype)
-                sess.upload_source(shot, s)
-            if len(self.recv_indx):
-                sess.upload_receivers(shot, self.recv_indx, np.asarray(self.recv_frac, dtype), self.nt_total)
-            if self.init is not None:
-                sess.set_level(shot, 0, np.asarray(self.init, dtype))
+            self.upload_shot(sess, shot, dtype)
+
+    def upload_shot(self, sess, shot, dtype=np.float32):
+        """This problem's source, receivers and initial state into one shot slot."""
+        if self.src is not None:
+            s = dict(self.src)
+            s["ext_coef"] = np.asarray(s["ext_coef"], dtype)
+            if s["F"] is not None:
+                s["F"] = np.asarray(s["F"], dtype)
+            if s["M"] is not None:
+                s["M"] = np.asarray(s["M"], dtype)
+            sess.upload_source(shot, s)
+        if len(self.recv_indx):
+            sess.upload_receivers(shot, self.recv_indx, np.asarray(self.recv_frac, dtype), self.nt_total)
+        if self.init is not None:
+            sess.set_level(shot, 0, np.asarray(self.init, dtype))
 
 
 def rel_l2(a, b):
