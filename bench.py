@@ -62,6 +62,21 @@ def cpu_baseline(m, steps):
                        f"unfused reference stage order, {sec:.1f} s loop")
 
 
+def measured_traffic(workload, storage):
+    """HBM bytes per interior launch from the committed counter summary, if any.
+
+    The summary holds FETCH_SIZE + WRITE_SIZE per dispatch of the interior
+    kernel on this workload, scaled by the same counters on a copy kernel of
+    known size."""
+    path = os.path.join(ROOT, "profiles", f"k_march_{workload}_{storage}_hbm_counters.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +87,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain", action="store_true", help="per-point kernel everywhere")
+    ap.add_argument("--tile", default="", metavar="SHAPE,ZCHUNK",
+                    help="pin the interior tile instead of autotuning (profiling runs)")
+    ap.add_argument("--timed-launch-steps", type=int, default=8,
+                    help="steps of the per-launch event timing pass (0: skip)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
@@ -101,17 +120,30 @@ def main():
     nt_total = args.warmup + args.steps
     sess = cgfd.Session(m["ni"], m["nj"], m["nk"], storage=STORE[args.storage],
                         flags=cgfd.FLAG_PLAIN if args.plain else 0)
+    nt_total += args.timed_launch_steps
     workloads.upload(sess, m, nt_total=nt_total)
-    sess.set_timing(True)
+    if args.tile:
+        sh, zc = (int(v) for v in args.tile.split(","))
+        sess.set_tile(sh, zc)
     log("uploaded; warmup")
     sess.run(0, args.warmup, m["dt"])
     log(f"timing {args.steps} steps")
+    # the timed region: K steps, 8-step periods replayed as captured graphs
     t1 = time.perf_counter()
     sess.run(args.warmup, args.steps, m["dt"])
     t2 = time.perf_counter()
-    run_ms, int_avg_ms, n_int = sess.last_run_timing()
+    run_ms = sess.last_run_timing()[0]
+    # then a short pass with HIP events around every interior launch (on the
+    # session stream the kernels run on) for the kernel's average duration
+    int_avg_ms, n_int = 0.0, 0
+    if args.timed_launch_steps > 0:
+        sess.set_timing(True)
+        sess.run(args.warmup + args.steps, args.timed_launch_steps, m["dt"])
+        _, int_avg_ms, n_int = sess.last_run_timing()
+        step_ms_timed = sess.last_run_timing()[0] / args.timed_launch_steps
     seis = sess.read_seismo(0)
     finite = bool(np.isfinite(seis).all())
+    tile = sess.get_tile()
     sess.close()
     wall = t2 - t1
     cells = workloads.cells(m)
@@ -124,8 +156,12 @@ def main():
         ach = ALGO_BYTES[args.storage] * icells / (int_avg_ms * 1e-3) / 1e9
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_interior", interior_cells=icells, avg_launch_ms=round(int_avg_ms, 5),
-                    interior_share_of_step=round(n_int * int_avg_ms / run_ms, 4) if run_ms > 0 else None)
+                    kernel="k_march (interior z-marching sweep)", interior_cells=icells, avg_launch_ms=round(int_avg_ms, 5),
+                    tile=f"{tile['tx']}x{tile['ty']} threads, {tile['zchunk']} planes",
+                    interior_share_of_step=round(4 * int_avg_ms / step_ms_timed, 4))
+        traffic = measured_traffic(m["name"], args.storage)
+        if traffic:
+            roof["traffic"] = traffic
 
     cpu = None
     if not args.no_cpu:

@@ -32,7 +32,7 @@ EXPORTS = [
     "cgfd_grid_cartesian", "cgfd_grid_topography", "cgfd_metric_calc", "cgfd_dt_estimate",
     "cgfd_media_layered", "cgfd_media_homogeneous", "cgfd_rho_to_slow", "cgfd_pml_len_dh",
     "cgfd_pml_profile", "cgfd_free_surface_matrices", "cgfd_wavelet", "cgfd_angle2moment",
-    "cgfd_source_point", "cgfd_receiver_index",
+    "cgfd_source_point", "cgfd_receiver_index", "cgfd_storage_scales",
 ]
 
 
@@ -64,6 +64,22 @@ class SourceDesc(C.Structure):
 
 class CgfdError(RuntimeError):
     pass
+
+
+def _source_desc(src):
+    """cgfd_source_desc for a source dict, and the arrays it points into."""
+    t = np.float64 if np.asarray(src["ext_coef"]).dtype == np.float64 else np.float32
+    keep = dict(
+        itb=_arr(src["it_begin"], np.int32), ite=_arr(src["it_end"], np.int32),
+        en=_arr(src["ext_num"], np.int32), ei=_arr(src["ext_indx"], np.int64),
+        ec=_arr(src["ext_coef"], t),
+        F=_arr(src["F"], t) if src.get("F") is not None else None,
+        M=_arr(src["M"], t) if src.get("M") is not None else None)
+    d = SourceDesc(src["nsrc"], src["force_on"], src["moment_on"], src["max_nt"], src["max_ext"],
+                   DT_F64 if t == np.float64 else DT_F32,
+                   _ptr(keep["itb"]), _ptr(keep["ite"]), _ptr(keep["en"]), _ptr(keep["ei"]),
+                   _ptr(keep["ec"]), _ptr(keep["F"]), _ptr(keep["M"]))
+    return d, keep
 
 
 _lib = None
@@ -123,6 +139,7 @@ def load(path=LIB_PATH):
         "cgfd_angle2moment": ([d, d, d, vp], i),
         "cgfd_source_point": ([i, i, i, d, d, d, i, d, d, d, i, d, d, d, vp, vp, vp, vp, vp, vp, vp], i),
         "cgfd_receiver_index": ([i, i, i, d, d, d, i, vp, vp], i),
+        "cgfd_storage_scales": ([i, i, i, i, vp, vp, vp, vp, C.POINTER(SourceDesc), d, vp], i),
     })
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -215,17 +232,7 @@ class Session:
         if src is None or src["nsrc"] == 0:
             _check(_lib.cgfd_upload_source(self.h, shot, None))
             return
-        t = np.float64 if np.asarray(src["ext_coef"]).dtype == np.float64 else np.float32
-        keep = dict(
-            itb=_arr(src["it_begin"], np.int32), ite=_arr(src["it_end"], np.int32),
-            en=_arr(src["ext_num"], np.int32), ei=_arr(src["ext_indx"], np.int64),
-            ec=_arr(src["ext_coef"], t),
-            F=_arr(src["F"], t) if src.get("F") is not None else None,
-            M=_arr(src["M"], t) if src.get("M") is not None else None)
-        d = SourceDesc(src["nsrc"], src["force_on"], src["moment_on"], src["max_nt"], src["max_ext"],
-                       DT_F64 if t == np.float64 else DT_F32,
-                       _ptr(keep["itb"]), _ptr(keep["ite"]), _ptr(keep["en"]), _ptr(keep["ei"]),
-                       _ptr(keep["ec"]), _ptr(keep["F"]), _ptr(keep["M"]))
+        d, keep = _source_desc(src)
         _check(_lib.cgfd_upload_source(self.h, shot, C.byref(d)))
 
     def upload_receivers(self, shot, indx8, frac3, nt_total):
@@ -424,3 +431,17 @@ def receiver_index(ni, nj, nk, rx, ry, rz, depth_flag):
     fr = _z(3)
     rc = _lib.cgfd_receiver_index(ni, nj, nk, rx, ry, rz, depth_flag, _ptr(ix), _ptr(fr))
     return (ix, fr) if rc == 0 else (None, None)
+
+
+def storage_scales(storage, ni, nj, nk, metric, lam, mu, slw, src, dt):
+    """Power-of-two scale exponents of a reduced-precision session (cgfd_storage_scales)."""
+    load()
+    ex = np.zeros(9, dtype=np.int32)
+    arrs = [_arr(a, np.float64) for a in (metric, lam, mu, slw)]
+    if src is not None and src["nsrc"] > 0:
+        d, keep = _source_desc(src)
+        dp = C.byref(d)
+    else:
+        dp = None
+    _check(_lib.cgfd_storage_scales(storage, ni, nj, nk, *[_ptr(a) for a in arrs], dp, float(dt), _ptr(ex)))
+    return [int(v) for v in ex]

@@ -538,4 +538,65 @@ int cgfd_receiver_index(int ni, int nj, int nk, double rx, double ry, double rz,
   return CGFD_OK;
 }
 
+int cgfd_storage_scales(int storage, int ni, int nj, int nk, const double *metric, const double *lam,
+                        const double *mu, const double *slw, const cgfd_source_desc *src, double dt, int *exp2) {
+  if (!exp2 || ni < 1 || nj < 1 || nk < 1 || !(dt > 0)) return CGFD_E_ARG;
+  for (int c = 0; c < 9; c++) exp2[c] = 0;
+  if (storage != CGFD_STORE_F16) return CGFD_OK;
+  if (!metric || !lam || !mu || !slw) return CGFD_E_ARG;
+  const G3 g(ni, nj, nk);
+  // impedances over the physical grid
+  double zs_min = HUGE_VAL, zp_max = 0.0;
+  for (int k = NG; k < NG + nk; k++)
+    for (int j = NG; j < NG + nj; j++)
+      for (int i = NG; i < NG + ni; i++) {
+        const long q = g.at(i, j, k);
+        if (!(slw[q] > 0)) continue;
+        const double rho = 1.0 / slw[q];
+        if (mu[q] > 0) zs_min = std::min(zs_min, std::sqrt(mu[q] * rho));
+        zp_max = std::max(zp_max, std::sqrt((lam[q] + 2.0 * mu[q]) * rho));
+      }
+  double s_inj = 0.0, v_inj = 0.0;  // injected stress [Pa] and velocity [m/s]
+  if (src && src->nsrc > 0) {
+    auto rdv = [&](const void *p, size_t i) {
+      return src->dtype == CGFD_DT_F64 ? ((const double *)p)[i] : (double)((const float *)p)[i];
+    };
+    const size_t per = (size_t)src->max_nt * 4;
+    for (int is = 0; is < src->nsrc; is++) {
+      double mint = 0.0, fint = 0.0;  // time integrals of max |M_c| and max |F_c| (stage 0 samples)
+      const int nst = std::min(src->max_nt, src->it_end[is] - src->it_begin[is] + 1);
+      for (int it = 0; it < nst; it++) {
+        const size_t q = (size_t)is * per + (size_t)it * 4;
+        double mm = 0.0, ff = 0.0;
+        if (src->moment_on && src->M)
+          for (int c = 0; c < 6; c++) mm = std::max(mm, std::fabs(rdv(src->M, (size_t)c * src->nsrc * per + q)));
+        if (src->force_on && src->F)
+          for (int c = 0; c < 3; c++) ff = std::max(ff, std::fabs(rdv(src->F, (size_t)c * src->nsrc * per + q)));
+        mint += mm * dt;
+        fint += ff * dt;
+      }
+      for (int e = 0; e < src->ext_num[is]; e++) {
+        const long long q = src->ext_indx[(size_t)is * src->max_ext + e];
+        if (q < 0 || q >= g.n) return CGFD_E_ARG;
+        const double coef = std::fabs(rdv(src->ext_coef, (size_t)is * src->max_ext + e));
+        const double jac = std::fabs(metric[q]);
+        if (!(jac > 0)) return CGFD_E_ARG;
+        s_inj += mint * coef / jac;
+        v_inj += fint * coef * std::fabs(slw[q]) / jac;
+      }
+    }
+  }
+  const double s_bound = s_inj + v_inj * zp_max;
+  const double v_bound = (zs_min < HUGE_VAL ? s_inj / zs_min : 0.0) + v_inj;
+  auto ex = [](double b) -> int {
+    if (!(b > 0) || !std::isfinite(b)) return 0;
+    int e = (int)std::ceil(std::log2(b / 4096.0));
+    return std::max(-126, std::min(127, e));
+  };
+  const int ev = ex(v_bound), es = ex(s_bound);
+  for (int c = 0; c < 3; c++) exp2[c] = ev;
+  for (int c = 3; c < 9; c++) exp2[c] = es;
+  return CGFD_OK;
+}
+
 }  // extern "C"

@@ -139,6 +139,21 @@ int cgfd_source_point(int ni, int nj, int nk, double sx, double sy, double sz, i
 int cgfd_receiver_index(int ni, int nj, int nk, double rx, double ry, double rz, int depth_flag, long long *indx8,
                         double *frac3);
 
+/* ---- reduced-precision storage -------------------------------------------------- */
+
+/* Power-of-two scale exponents (for cgfd_set_storage_scale) that keep the
+ * wavefield of a reduced-precision session inside the format's range (no
+ * reference counterpart).  The bound is physical: a stress component cannot
+ * exceed the total stress the moment sources inject (sum over source points
+ * of coef/J times the time integral of |M|) plus what the force sources'
+ * velocity times the largest impedance rho*Vp gives; a velocity cannot exceed
+ * that stress over the smallest shear impedance rho*Vs plus the injected
+ * force velocity.  FP16 maps the bound to 2^12 (16x headroom below 65504);
+ * BF16, FP32 and FP64 keep exponent 0.  src may hold several sources; pass
+ * the shot with the strongest sources when shots share one session. */
+int cgfd_storage_scales(int storage, int ni, int nj, int nk, const double *metric, const double *lam,
+                        const double *mu, const double *slw, const cgfd_source_desc *src, double dt, int *exp2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,51 @@
+"""GPU: reduced-precision storage (FP16 / BF16 wavefield and PML memory with
+power-of-two scales, FP32 arithmetic) against the FP64 path of the same scheme.
+
+The FP32 path must agree with FP64 to FP32 rounding; the 16-bit formats are
+held to the bounds below (relative L2 over all seismogram components), and
+the measured errors are printed for the record.
+"""
+import numpy as np
+import pytest
+
+import cgfd
+from problem import Problem, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+BOUND = {cgfd.STORE_F32: 1e-4, cgfd.STORE_F16: 3e-2, cgfd.STORE_BF16: 2e-1}
+NAME = {cgfd.STORE_F32: "f32", cgfd.STORE_F16: "f16", cgfd.STORE_BF16: "bf16"}
+
+
+@pytest.fixture(scope="module")
+def fp64_run():
+    P = Problem(ni=64, nj=60, nk=48, curvilinear=True, hetero=0.03, nt_total=320, seed=31,
+                pml=(8, 8, 8, 8, 8, 0), f0=3.0, t0=0.3, nrecv=12)
+    s = cgfd.Session(P.ni, P.nj, P.nk, storage=cgfd.STORE_F64)
+    P.upload(s, dtype=np.float64)
+    s.run(0, P.nt_total, P.dt)
+    ref = s.read_seismo(0)
+    w = s.fetch_level(0, cgfd.LEVEL_PRE)
+    s.close()
+    return P, ref, w
+
+
+@pytest.mark.parametrize("storage", [cgfd.STORE_F32, cgfd.STORE_F16, cgfd.STORE_BF16], ids=["f32", "f16", "bf16"])
+def test_storage_error_vs_fp64(fp64_run, storage):
+    P, ref, wref = fp64_run
+    s = cgfd.Session(P.ni, P.nj, P.nk, storage=storage)
+    P.upload(s, dtype=np.float32)
+    ex = cgfd.storage_scales(storage, P.ni, P.nj, P.nk, P.metric, P.lam, P.mu, P.slw, P.src, P.dt)
+    if storage in (cgfd.STORE_F16, cgfd.STORE_BF16):
+        s.set_storage_scale(ex)
+    s.run(0, P.nt_total, P.dt)
+    seis = s.read_seismo(0)
+    w = s.fetch_level(0, cgfd.LEVEL_PRE)
+    s.close()
+    assert np.isfinite(seis).all() and np.isfinite(w).all()
+    e_seis = rel_l2(seis, ref)
+    e_snap = rel_l2(w, wref)
+    print(f"\n[precision] {NAME[storage]} scales {ex}: seismogram rel L2 {e_seis:.3e}, "
+          f"final wavefield rel L2 {e_snap:.3e}")
+    assert e_seis <= BOUND[storage]
+    assert e_snap <= BOUND[storage]
