@@ -77,6 +77,87 @@ def measured_traffic(workload, storage):
     return None
 
 
+def serving_sample(n, nt_range, load, seed, nseq):
+    """Served stream on the 100x100x60 medium: n seeded shot requests (half
+    FP32, half FP16 storage) arriving as a Poisson stream at `load` times the
+    rate of sequential plain FP32 runs; returns shots/hour both ways and the
+    served latency percentiles."""
+    m = workloads.small()
+    reqs = workloads.shot_requests(m, n, seed, nt_range=nt_range)
+    rng = np.random.default_rng(seed + 1)
+    fp16 = rng.random(n) < 0.5
+    ni, nj, nk, dt = m["ni"], m["nj"], m["nk"], m["dt"]
+    arrs = workloads.model_arrays(m)
+    nrec = len(m["recv_indx"])
+    # sequential plain runs: one FP32 single-shot session, model uploaded once
+    sess = cgfd.Session(ni, nj, nk)
+    workloads.upload(sess, m, nt_total=8)
+    sess.run(0, 8, dt)  # autotune and first captures outside the clock
+    alone = {}
+    t0 = time.perf_counter()
+    steps_seq = 0
+    for q in range(nseq):
+        r = reqs[q]
+        sess.reset_shot(0)
+        sess.upload_source(0, r["src"])
+        sess.upload_receivers(0, m["recv_indx"], np.asarray(m["recv_frac"], np.float32), r["nt"])
+        sess.run(0, r["nt"], dt)
+        alone[q] = sess.read_seismo(0)
+        steps_seq += r["nt"]
+    t_seq_sample = time.perf_counter() - t0
+    sess.close()
+    steps_all = sum(r["nt"] for r in reqs)
+    t_seq = t_seq_sample * steps_all / steps_seq
+    seq_rate = n / t_seq
+    # served: FP32 and FP16 models resident, continuous batching
+    ex = np.max([cgfd.storage_scales(cgfd.STORE_F16, ni, nj, nk, m["metric"], m["lam"], m["mu"], m["slw"],
+                                     r["src"], dt) for r, h in zip(reqs, fp16) if h] or [[0] * 9], axis=0)
+    srv = cgfd.Server(max_batch=16, chunk_steps=16)
+    nt_max = nt_range[1]
+    m32 = srv.add_model(ni, nj, nk, dt=dt, storage=cgfd.STORE_F32, max_nrecv=nrec, max_nt=nt_max, **arrs)
+    m16 = srv.add_model(ni, nj, nk, dt=dt, storage=cgfd.STORE_F16, scale_exp2=ex, max_nrecv=nrec, max_nt=nt_max,
+                        **arrs)
+    # warm both groups (autotune, graph capture) with one short request each
+    for mid in (m32, m16):
+        t = srv.submit(mid, 16, reqs[0]["src"], m["recv_indx"], m["recv_frac"])
+        srv.wait(t)
+        srv.result(t)
+    gaps = rng.exponential(1.0 / (load * seq_rate), n)
+    tickets = []
+    start = time.perf_counter()
+    t_next = start
+    for q, r in enumerate(reqs):
+        t_next += gaps[q]
+        while time.perf_counter() < t_next:
+            time.sleep(0.0005)
+        tickets.append(srv.submit(m16 if fp16[q] else m32, r["nt"], r["src"], m["recv_indx"], m["recv_frac"]))
+    lat, errs = [], []
+    for q, t in enumerate(tickets):
+        srv.wait(t)
+        seis, _, t_done = srv.result(t)
+        lat.append(t_done)
+        if q in alone and not fp16[q]:
+            d = np.linalg.norm(alone[q])
+            errs.append(float(np.linalg.norm(seis - alone[q]) / d) if d > 0 else 0.0)
+    end = time.perf_counter()
+    stats = srv.stats()
+    srv.close()
+    served_rate = n / (end - start)
+    lat = np.array(lat)
+    return dict(
+        workload=f"{n} shot requests on the 100x100x60 medium, steps U{list(nt_range)}, f0 U[1,4] Hz, depth "
+                 f"1-10 km clamped, {int(fp16.sum())} FP16-storage + {int((~fp16).sum())} FP32, Poisson arrivals "
+                 f"at {load}x the sequential rate",
+        served_shots_per_hour=round(served_rate * 3600, 1),
+        sequential_plain_shots_per_hour=round(seq_rate * 3600, 1),
+        speedup=round(served_rate / seq_rate, 3),
+        p50_latency_s=round(float(np.percentile(lat, 50)), 4),
+        p99_latency_s=round(float(np.percentile(lat, 99)), 4),
+        sequential_sample=f"{nseq} shots timed alone ({steps_seq} steps), scaled to all {steps_all} steps",
+        served_vs_alone_max_rel_l2=max(errs) if errs else None,
+        groups=stats["groups"], rejected=stats["rejected"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +172,11 @@ def main():
                     help="pin the interior tile instead of autotuning (profiling runs)")
     ap.add_argument("--timed-launch-steps", type=int, default=8,
                     help="steps of the per-launch event timing pass (0: skip)")
+    ap.add_argument("--reduced-steps", type=int, default=200,
+                    help="timed steps of the FP16/BF16-storage runs of the same workload (0: skip)")
+    ap.add_argument("--serve-shots", type=int, default=32, help="served-stream sample size (0: skip)")
+    ap.add_argument("--serve-steps", default="120,480", help="step range of the served requests")
+    ap.add_argument("--serve-load", type=float, default=4.0, help="arrival rate / sequential plain rate")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
@@ -163,6 +249,38 @@ def main():
         if traffic:
             roof["traffic"] = traffic
 
+    reduced = None
+    if args.reduced_steps > 0 and args.storage == "f32":
+        reduced = {}
+        ex = cgfd.storage_scales(cgfd.STORE_F16, m["ni"], m["nj"], m["nk"], m["metric"], m["lam"], m["mu"], m["slw"],
+                                 m["src"], m["dt"])
+        for name, st in (("f16", cgfd.STORE_F16), ("bf16", cgfd.STORE_BF16)):
+            rs = cgfd.Session(m["ni"], m["nj"], m["nk"], storage=st)
+            workloads.upload(rs, m, nt_total=args.warmup + args.reduced_steps)
+            if st == cgfd.STORE_F16:
+                rs.set_storage_scale(ex)
+            rs.run(0, args.warmup, m["dt"])
+            ta = time.perf_counter()
+            rs.run(args.warmup, args.reduced_steps, m["dt"])
+            tb = time.perf_counter()
+            fin = bool(np.isfinite(rs.read_seismo(0)).all())
+            rs.close()
+            v = cells * 4 * args.reduced_steps / (tb - ta) / 1e9
+            reduced[name] = dict(value=round(v, 4), unit="Gcell-RK-stages/s",
+                                 ms_per_step=round((tb - ta) * 1e3 / args.reduced_steps, 4),
+                                 speedup_vs_f32=round(v / value, 3), steps=args.reduced_steps,
+                                 scales_exp2=ex if st == cgfd.STORE_F16 else [0] * 9, seismograms_finite=fin)
+            log(f"{name} storage: {v:.3f} Gcell-RK-stages/s ({v / value:.2f}x FP32)")
+
+    serving = None
+    if args.serve_shots > 0:
+        lo, hi = (int(v) for v in args.serve_steps.split(","))
+        log(f"serving sample: {args.serve_shots} shots")
+        serving = serving_sample(args.serve_shots, (lo, hi), args.serve_load, 7, min(8, args.serve_shots))
+        log(f"served {serving['served_shots_per_hour']} shots/h vs sequential "
+            f"{serving['sequential_plain_shots_per_hour']}; p50 {serving['p50_latency_s']} s p99 "
+            f"{serving['p99_latency_s']} s")
+
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: {args.cpu_steps} steps")
@@ -179,6 +297,8 @@ def main():
                    "cells": cells, "storage": args.storage, "kernel_path": "plain" if args.plain else "zmarch",
                    "seismograms_finite": finite, "device_ms": round(run_ms, 2)},
         "roofline": roof,
+        "reduced_storage": reduced,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

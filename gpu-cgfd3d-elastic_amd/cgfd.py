@@ -33,6 +33,8 @@ EXPORTS = [
     "cgfd_media_layered", "cgfd_media_homogeneous", "cgfd_rho_to_slow", "cgfd_pml_len_dh",
     "cgfd_pml_profile", "cgfd_free_surface_matrices", "cgfd_wavelet", "cgfd_angle2moment",
     "cgfd_source_point", "cgfd_receiver_index", "cgfd_storage_scales",
+    "cgfd_server_create", "cgfd_server_destroy", "cgfd_server_add_model", "cgfd_server_submit",
+    "cgfd_server_wait", "cgfd_server_result", "cgfd_server_get_stats",
 ]
 
 
@@ -64,6 +66,30 @@ class SourceDesc(C.Structure):
 
 class CgfdError(RuntimeError):
     pass
+
+
+class ServerDesc(C.Structure):
+    _fields_ = [("hbm_budget", C.c_longlong), ("max_batch", C.c_int), ("chunk_steps", C.c_int)]
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("ni", C.c_int), ("nj", C.c_int), ("nk", C.c_int), ("storage", C.c_int), ("dtype", C.c_int),
+                ("metric", C.c_void_p), ("lam", C.c_void_p), ("mu", C.c_void_p), ("slw", C.c_void_p),
+                ("free_top", C.c_int), ("matVx2Vz", C.c_void_p), ("matVy2Vz", C.c_void_p),
+                ("pml_nlay", C.c_int * 6), ("pml_A", C.c_void_p * 6), ("pml_B", C.c_void_p * 6),
+                ("pml_D", C.c_void_p * 6), ("scale_exp2", C.c_void_p), ("dt", C.c_double),
+                ("max_nrecv", C.c_int), ("max_nt", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("model", C.c_int), ("nt_total", C.c_int), ("src", C.c_void_p), ("nrecv", C.c_int),
+                ("recv_indx8", C.c_void_p), ("recv_dtype", C.c_int), ("recv_frac3", C.c_void_p)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("admitted", C.c_longlong), ("completed", C.c_longlong),
+                ("rejected", C.c_longlong), ("groups", C.c_int), ("active_shots", C.c_int),
+                ("queued", C.c_int), ("hbm_in_use", C.c_longlong), ("hbm_budget", C.c_longlong)]
 
 
 def _source_desc(src):
@@ -140,6 +166,13 @@ def load(path=LIB_PATH):
         "cgfd_source_point": ([i, i, i, d, d, d, i, d, d, d, i, d, d, d, vp, vp, vp, vp, vp, vp, vp], i),
         "cgfd_receiver_index": ([i, i, i, d, d, d, i, vp, vp], i),
         "cgfd_storage_scales": ([i, i, i, i, vp, vp, vp, vp, C.POINTER(SourceDesc), d, vp], i),
+        "cgfd_server_create": ([C.POINTER(ServerDesc), C.POINTER(vp)], i),
+        "cgfd_server_destroy": ([vp], None),
+        "cgfd_server_add_model": ([vp, C.POINTER(ModelDesc), C.POINTER(i)], i),
+        "cgfd_server_submit": ([vp, C.POINTER(Request), C.POINTER(ll)], i),
+        "cgfd_server_wait": ([vp, ll, d], i),
+        "cgfd_server_result": ([vp, ll, i, vp, C.POINTER(d), C.POINTER(d)], i),
+        "cgfd_server_get_stats": ([vp, C.POINTER(ServerStats)], i),
     })
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -445,3 +478,99 @@ def storage_scales(storage, ni, nj, nk, metric, lam, mu, slw, src, dt):
         dp = None
     _check(_lib.cgfd_storage_scales(storage, ni, nj, nk, *[_ptr(a) for a in arrs], dp, float(dt), _ptr(ex)))
     return [int(v) for v in ex]
+
+
+class Server:
+    """Continuous-batching shot server (cgfd_serve.h): models stay resident on
+    the device, requests on one model share a batched session."""
+
+    def __init__(self, hbm_budget=0, max_batch=8, chunk_steps=16):
+        load()
+        self.h = C.c_void_p()
+        d = ServerDesc(int(hbm_budget), int(max_batch), int(chunk_steps))
+        _check(_lib.cgfd_server_create(C.byref(d), C.byref(self.h)))
+        self._meta = {}
+
+    def close(self):
+        if self.h:
+            _lib.cgfd_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_model(self, ni, nj, nk, metric, lam, mu, slw, dt, storage=STORE_F32, free_top=False, mvx=None,
+                  mvy=None, pml=None, scale_exp2=None, max_nrecv=0, max_nt=1, dtype=np.float32):
+        """pml: {face: (nlay, A, B, D)}; arrays in the reference host layout."""
+        t = np.float64 if dtype == np.float64 else np.float32
+        keep = [_arr(metric, t), _arr(lam, t), _arr(mu, t), _arr(slw, t)]
+        md = ModelDesc()
+        md.ni, md.nj, md.nk, md.storage = ni, nj, nk, storage
+        md.dtype = DT_F64 if t == np.float64 else DT_F32
+        md.metric, md.lam, md.mu, md.slw = [_ptr(a).value for a in keep]
+        md.free_top = 1 if free_top else 0
+        if free_top:
+            keep += [_arr(mvx, t), _arr(mvy, t)]
+            md.matVx2Vz, md.matVy2Vz = _ptr(keep[-2]).value, _ptr(keep[-1]).value
+        for f, (nl, A, B, D) in (pml or {}).items():
+            arrs = [_arr(A, t), _arr(B, t), _arr(D, t)]
+            keep += arrs
+            md.pml_nlay[f] = nl
+            md.pml_A[f], md.pml_B[f], md.pml_D[f] = [_ptr(a).value for a in arrs]
+        if scale_exp2 is not None:
+            ex = np.ascontiguousarray(scale_exp2, dtype=np.int32)
+            keep.append(ex)
+            md.scale_exp2 = _ptr(ex).value
+        md.dt = float(dt)
+        md.max_nrecv, md.max_nt = int(max_nrecv), int(max_nt)
+        mid = C.c_int()
+        _check(_lib.cgfd_server_add_model(self.h, C.byref(md), C.byref(mid)))
+        return mid.value
+
+    def submit(self, model, nt_total, src, recv_indx, recv_frac):
+        rq = Request()
+        rq.model, rq.nt_total = model, int(nt_total)
+        keep = []
+        if src is not None and src["nsrc"] > 0:
+            d, k = _source_desc(src)
+            keep += [d, k]
+            rq.src = C.cast(C.pointer(d), C.c_void_p).value
+        ix = _arr(recv_indx, np.int64).reshape(-1, 8)
+        fr = _arr(recv_frac, np.float64).reshape(-1, 3)
+        rq.nrecv = ix.shape[0]
+        rq.recv_indx8, rq.recv_dtype, rq.recv_frac3 = _ptr(ix).value, DT_F64, _ptr(fr).value
+        t = C.c_longlong()
+        _check(_lib.cgfd_server_submit(self.h, C.byref(rq), C.byref(t)))
+        self._meta[t.value] = (ix.shape[0], int(nt_total))
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        """True when done, False on time-out; raises if the request was rejected."""
+        r = _lib.cgfd_server_wait(self.h, ticket, float(timeout))
+        if r == 1:
+            return False
+        _check(r)
+        return True
+
+    def result(self, ticket):
+        """(seismograms [nrecv][9][nt], seconds to admission, seconds to completion)."""
+        nr, nt = self._meta.pop(ticket)
+        out = np.zeros((nr, 9, nt))
+        ta, td = C.c_double(), C.c_double()
+        _check(_lib.cgfd_server_result(self.h, ticket, DT_F64, _ptr(out), C.byref(ta), C.byref(td)))
+        return out, ta.value, td.value
+
+    def stats(self):
+        st = ServerStats()
+        _check(_lib.cgfd_server_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+
+def footprint_bytes(ni, nj, nk, storage=STORE_F32, nshots=1, nrecv=0, nt_total=0):
+    """Device bytes a session of this shape holds (cgfd_footprint_bytes)."""
+    load()
+    d = Desc(ni, nj, nk, storage, nshots, 0)
+    return int(_lib.cgfd_footprint_bytes(C.byref(d), nrecv, nt_total))

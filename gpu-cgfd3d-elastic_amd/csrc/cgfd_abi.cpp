@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/cgfd_el_iso.h"
+#include "cgfd_internal.h"
 #include "el_iso_device.h"
 #include "el_iso_launch.h"
 
@@ -547,16 +548,30 @@ StageArgs<TS, TC> stage_args(cgfd_session *s, int slot_in, int slot_pre, int slo
   return a;
 }
 
-// Work split of one stage: the z-marching interior box, the PML face slabs
-// (z-marching PML variant) and, under a free surface, the top 4 rows
-// (per-point kernel, which also covers everything for tiny grids or the
-// plain method).
+// Work split of one stage: the z-marching interior box, then the boundary
+// shell cut into boxes by PML membership: boxes inside exactly one face (away
+// from a free surface) go to the slab kernel of that face's axis, the rest
+// (edges, corners, the 4 free-surface rows) to the general per-point kernel,
+// which also covers everything for tiny grids or the plain method.  With
+// CGFD_FLAG_PML_MARCH the face slabs are swept by the z-marching PML variant.
+struct BoxList {
+  int n = 0;
+  int box[kMaxBox][6];
+  long end[kMaxBox];
+  void add(int a0, int a1, int b0, int b1, int c0, int c1) {
+    if (a1 < a0 || b1 < b0 || c1 < c0) return;
+    const long prev = n ? end[n - 1] : 0;
+    box[n][0] = a0; box[n][1] = a1; box[n][2] = b0; box[n][3] = b1; box[n][4] = c0; box[n][5] = c1;
+    end[n] = prev + (long)(a1 - a0 + 1) * (b1 - b0 + 1) * (c1 - c0 + 1);
+    n++;
+  }
+};
+
 struct Shell {
   MarchSet inner;
   MarchSet pml;
-  int nbox;
-  int box[6][6];
-  long box_end[6];
+  BoxList point;
+  BoxList slab[3];
 };
 
 Shell make_shell(cgfd_session *s) {
@@ -569,50 +584,58 @@ Shell make_shell(cgfd_session *s) {
   const int ktopm = s->free_top ? nk - 5 : nk - 1;  // highest plane the z-march may sweep
   const int kb1 = ktopm - (s->free_top ? 0 : lay(5));
   const bool plain = (s->d.flags & CGFD_FLAG_PLAIN) != 0;
-  std::vector<std::array<int, 6>> pboxes;
-  auto add = [](MarchSet &m, int a0, int a1, int b0, int b1, int c0, int c1) {
+  auto addm = [](MarchSet &m, int a0, int a1, int b0, int b1, int c0, int c1) {
     if (a1 >= a0 && b1 >= b0 && c1 >= c0) {
       int q = m.nbox++;
       m.box[q][0] = a0; m.box[q][1] = a1; m.box[q][2] = b0; m.box[q][3] = b1; m.box[q][4] = c0; m.box[q][5] = c1;
     }
   };
   if (plain || ib1 < ib0 || jb1 < jb0 || kb1 < kb0) {
-    pboxes.push_back({0, ni - 1, 0, nj - 1, 0, nk - 1});
-  } else {
-    sh.inner.zchunk = s->tile_zchunk;
-    sh.inner.shape = s->tile_shape < 0 ? 0 : s->tile_shape;
-    add(sh.inner, ib0, ib1, jb0, jb1, kb0, kb1);
-    if (s->d.flags & CGFD_FLAG_PML_MARCH) {
-      sh.pml.zchunk = 24;
-      sh.pml.shape = 0;
-      add(sh.pml, 0, ib0 - 1, 0, nj - 1, 0, ktopm);
-      add(sh.pml, ib1 + 1, ni - 1, 0, nj - 1, 0, ktopm);
-      add(sh.pml, ib0, ib1, 0, jb0 - 1, 0, ktopm);
-      add(sh.pml, ib0, ib1, jb1 + 1, nj - 1, 0, ktopm);
-      add(sh.pml, ib0, ib1, jb0, jb1, 0, kb0 - 1);
-      add(sh.pml, ib0, ib1, jb0, jb1, kb1 + 1, ktopm);
-      if (ktopm < nk - 1) pboxes.push_back({0, ni - 1, 0, nj - 1, ktopm + 1, nk - 1});
-    } else {
-      auto addp = [&](int a0, int a1, int b0, int b1, int c0, int c1) {
-        if (a1 >= a0 && b1 >= b0 && c1 >= c0) pboxes.push_back({a0, a1, b0, b1, c0, c1});
-      };
-      addp(0, ib0 - 1, 0, nj - 1, 0, nk - 1);
-      addp(ib1 + 1, ni - 1, 0, nj - 1, 0, nk - 1);
-      addp(ib0, ib1, 0, jb0 - 1, 0, nk - 1);
-      addp(ib0, ib1, jb1 + 1, nj - 1, 0, nk - 1);
-      addp(ib0, ib1, jb0, jb1, 0, kb0 - 1);
-      addp(ib0, ib1, jb0, jb1, kb1 + 1, nk - 1);
-    }
+    sh.point.add(0, ni - 1, 0, nj - 1, 0, nk - 1);
+    return sh;
   }
-  sh.nbox = (int)pboxes.size();
-  long acc = 0;
-  for (int b = 0; b < sh.nbox; b++) {
-    for (int q = 0; q < 6; q++) sh.box[b][q] = pboxes[b][q];
-    acc += (long)(pboxes[b][1] - pboxes[b][0] + 1) * (pboxes[b][3] - pboxes[b][2] + 1) *
-           (pboxes[b][5] - pboxes[b][4] + 1);
-    sh.box_end[b] = acc;
+  sh.inner.zchunk = s->tile_zchunk;
+  sh.inner.shape = s->tile_shape < 0 ? 0 : s->tile_shape;
+  addm(sh.inner, ib0, ib1, jb0, jb1, kb0, kb1);
+  if (s->d.flags & CGFD_FLAG_PML_MARCH) {
+    sh.pml.zchunk = 24;
+    sh.pml.shape = 0;
+    addm(sh.pml, 0, ib0 - 1, 0, nj - 1, 0, ktopm);
+    addm(sh.pml, ib1 + 1, ni - 1, 0, nj - 1, 0, ktopm);
+    addm(sh.pml, ib0, ib1, 0, jb0 - 1, 0, ktopm);
+    addm(sh.pml, ib0, ib1, jb1 + 1, nj - 1, 0, ktopm);
+    addm(sh.pml, ib0, ib1, jb0, jb1, 0, kb0 - 1);
+    addm(sh.pml, ib0, ib1, jb0, jb1, kb1 + 1, ktopm);
+    sh.point.add(0, ni - 1, 0, nj - 1, ktopm + 1, nk - 1);
+    return sh;
   }
+  // per axis: [lower face | middle | upper face or free-surface rows]
+  const int ia[3][2] = {{0, ib0 - 1}, {ib0, ib1}, {ib1 + 1, ni - 1}};
+  const int ja[3][2] = {{0, jb0 - 1}, {jb0, jb1}, {jb1 + 1, nj - 1}};
+  const int ka[3][2] = {{0, kb0 - 1}, {kb0, kb1}, {kb1 + 1, nk - 1}};
+  for (int ck = 0; ck < 3; ck++)
+    for (int cj = 0; cj < 3; cj++)
+      for (int ci = 0; ci < 3; ci++) {
+        if (ci == 1 && cj == 1 && ck == 1) continue;
+        const int *I = ia[ci], *J = ja[cj], *K = ka[ck];
+        const bool free_rows = ck == 2 && s->free_top;
+        const int nd = (ci != 1) + (cj != 1) + (ck != 1);
+        if (free_rows || nd > 1) {
+          sh.point.add(I[0], I[1], J[0], J[1], K[0], K[1]);
+        } else {
+          const int fd = ci != 1 ? 0 : cj != 1 ? 1 : 2;
+          sh.slab[fd].add(I[0], I[1], J[0], J[1], K[0], K[1]);
+        }
+      }
   return sh;
+}
+
+template <typename TS, typename TC> void set_boxes(StageArgs<TS, TC> &a, const BoxList &l) {
+  a.nbox = l.n;
+  for (int b = 0; b < l.n; b++) {
+    for (int q = 0; q < 6; q++) a.box[b][q] = l.box[b][q];
+    a.box_end[b] = l.end[b];
+  }
 }
 
 // one stage over the whole grid: interior sweep (timed when requested), PML
@@ -630,12 +653,14 @@ int launch_stage(cgfd_session *s, const Shell &sh, StageArgs<TS, TC> &a, int dx,
     hipError_t e = launch_pml_march<TS, TC>(a, sh.pml, dx, dy, dz, nshots, s->st);
     if (e != hipSuccess) return fail(CGFD_E_HIP, "PML sweep kernel: %s", hipGetErrorString(e));
   }
-  if (sh.nbox) {
-    a.nbox = sh.nbox;
-    for (int b = 0; b < sh.nbox; b++) {
-      for (int q = 0; q < 6; q++) a.box[b][q] = sh.box[b][q];
-      a.box_end[b] = sh.box_end[b];
-    }
+  for (int fd = 0; fd < 3; fd++) {
+    if (!sh.slab[fd].n) continue;
+    set_boxes(a, sh.slab[fd]);
+    hipError_t e = launch_slab<TS, TC>(a, fd, dx, dy, dz, nshots, s->st);
+    if (e != hipSuccess) return fail(CGFD_E_HIP, "PML slab kernel: %s", hipGetErrorString(e));
+  }
+  if (sh.point.n) {
+    set_boxes(a, sh.point);
     hipError_t e = launch_point<TS, TC>(a, dx, dy, dz, nshots, s->st);
     if (e != hipSuccess) return fail(CGFD_E_HIP, "point kernel: %s", hipGetErrorString(e));
   }
@@ -905,6 +930,8 @@ template <typename TS, typename TC> int onestage_t(cgfd_session *s, int shot, in
 }
 
 }  // namespace
+
+void cgfd_internal_set_error(const char *msg) { g_err = msg ? msg : ""; }
 
 // =====================================================================================
 extern "C" {

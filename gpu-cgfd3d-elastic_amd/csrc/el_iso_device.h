@@ -90,6 +90,9 @@ template <typename TS, typename TM> struct FaceArgs {
   TS *a_in, *a_pre, *a_out, *a_end;  // shot-0 level bases, component stride fsize
 };
 
+// boxes one per-point launch may cover (the boundary shell splits into at most 26)
+constexpr int kMaxBox = 26;
+
 template <typename TS, typename TM> struct StageArgs {
   Layout L;
   const TM *met;   // NMET components, stride L.cs
@@ -105,8 +108,8 @@ template <typename TS, typename TM> struct StageArgs {
   TM ca, cb;
   float sc[kNC], isc[kNC];  // storage scale (value = stored*sc) and its inverse
   int nbox;
-  int box[6][6];            // i1,i2,j1,j2,k1,k2 inclusive, physical
-  long box_end[6];          // cumulative point counts
+  int box[kMaxBox][6];      // i1,i2,j1,j2,k1,k2 inclusive, physical
+  long box_end[kMaxBox];    // cumulative point counts
   const int *shot_active;   // per-shot 0/1, or null
 };
 

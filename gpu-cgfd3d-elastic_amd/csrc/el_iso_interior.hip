@@ -261,6 +261,284 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
   }
 }
 
+// ---- split-role variant ------------------------------------------------------------
+// A workgroup of 2 x TX x TY threads sweeps one TX x TY tile.  The first half
+// differentiates the six stress components and updates the velocity, the
+// second differentiates the three velocity components and updates the stress.
+// The halves share only the LDS array and the barrier; each carries the
+// register queues, prefetch buffers and coefficients of its own role, which
+// about halves the registers per thread and doubles the waves a CU keeps in
+// flight on this latency-bound sweep.  The arithmetic per point is that of
+// k_march (same derivative sums, same order).
+enum SplitRole { ROLE_V = 0, ROLE_T = 1 };
+
+template <int ROLE, int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, bool CART>
+__device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf, int gt, int i0, int j0, int bi1,
+                                            int bj1, int kbeg, int kend, int shot) {
+  constexpr int NT = TX * TY;
+  constexpr int NCG = ROLE == ROLE_V ? 6 : 3;   // components this role differentiates
+  constexpr int COFF = ROLE == ROLE_V ? 3 : 0;  // the first of them
+  constexpr int NOUT = ROLE == ROLE_V ? 3 : 6;  // components this role updates
+  constexpr int OOFF = ROLE == ROLE_V ? 0 : 3;
+  constexpr int NMD = ROLE == ROLE_V ? 1 : 2;   // media values: 1/rho | lambda, mu
+  constexpr int HXL = (DX == 0) ? 3 : 1;
+  constexpr int HYL = (DY == 0) ? 3 : 1;
+  constexpr int LW = TX + 4, LH = TY + 4;
+  constexpr int zlo = (DZ == 0) ? -3 : -1;
+  constexpr int zc0 = -zlo;
+  constexpr int NXH = 4 * TY, NYH = 4 * TX, NH = NXH + NYH;
+  constexpr int HPER = (NH + NT - 1) / NT;  // halo cells per thread and component
+  constexpr int LBUF = kNC * LH * LW;
+  const Layout &L = a.L;
+  const long cs = L.cs;
+  const int lx = gt % TX, ly = gt / TX;
+  const int i = i0 + lx, j = j0 + ly;
+  const bool owner = (i <= bi1) && (j <= bj1);
+  const int ic = i <= L.ni + 2 ? i : L.ni + 2, jc = j <= L.nj + 2 ? j : L.nj + 2;
+  const long so = (long)shot * a.shot_stride;
+  const unsigned upz = (unsigned)L.pz, upx = (unsigned)L.px;
+  const unsigned col = (unsigned)(jc + kNG) * upx + (unsigned)(ic + L.xa);
+  auto sc = [&](int c) -> TC { return St<TS>::scaled ? (TC)a.sc[c] : TC(1); };
+  auto gld = [&](const TS *base, int c, unsigned off) -> TC {
+    return (TC)St<TS>::ld(&at_b(base + so + c * cs, off * (unsigned)sizeof(TS))) * sc(c);
+  };
+
+  // the same halo cells for every component: one offset and one LDS slot per cell
+  int h_lds[HPER];
+  unsigned h_off[HPER];
+#pragma unroll
+  for (int p = 0; p < HPER; p++) {
+    const int e = gt + p * NT;
+    const int r = e < NH ? e : NH - 1;
+    int hi, hj, li, lj;
+    if (r < NXH) {
+      const int h = r % 4, row = r / 4;
+      li = (h < HXL) ? h : TX + h;
+      hi = i0 - HXL + li;
+      lj = row + HYL;
+      hj = j0 + row;
+    } else {
+      const int rr = r - NXH;
+      const int h = rr / TX, cc = rr % TX;
+      lj = (h < HYL) ? h : TY + h;
+      hj = j0 - HYL + lj;
+      li = cc + HXL;
+      hi = i0 + cc;
+    }
+    hi = hi <= L.ni + 2 ? hi : L.ni + 2;
+    hj = hj <= L.nj + 2 ? hj : L.nj + 2;
+    h_lds[p] = e < NH ? lj * LW + li : -1;
+    h_off[p] = (unsigned)(hj + kNG) * upx + (unsigned)(hi + L.xa);
+  }
+
+  TC q[NCG][5];
+#pragma unroll
+  for (int c = 0; c < NCG; c++)
+#pragma unroll
+    for (int s = 0; s < 4; s++) q[c][s + 1] = gld(a.w_in, COFF + c, (unsigned)(kbeg + zlo + s + kNG) * upz + col);
+
+  const int mode = a.mode;
+  // S0 reads the base of its outputs from PRE (= the stage input)
+  const bool need_end = mode == MODE_S12 || mode == MODE_S3;
+  const bool need_pre = mode == MODE_S12 || mode == MODE_S0;
+  // wavefield and coefficients of the next plane are fetched one plane ahead;
+  // the RK levels, needed only in the epilogue, are fetched for the current
+  // plane before its barrier
+  TC n_lead[NCG], n_hv[NCG][HPER], n_met[9 + NMD];
+  auto issue = [&](int kk) {
+    const unsigned pk = (unsigned)(kk + kNG) * upz;
+    const unsigned o = pk + col;
+#pragma unroll
+    for (int c = 0; c < NCG; c++) n_lead[c] = gld(a.w_in, COFF + c, (unsigned)(kk + zlo + 4 + kNG) * upz + col);
+#pragma unroll
+    for (int c = 0; c < NCG; c++)
+#pragma unroll
+      for (int p = 0; p < HPER; p++) n_hv[c][p] = gld(a.w_in, COFF + c, pk + h_off[p]);
+    const unsigned ob = o * (unsigned)sizeof(TC);
+#pragma unroll
+    for (int m = 0; m < 9; m++) n_met[m] = CART ? a.cmet[m] : at_b(a.met + (XIX + m) * cs, ob);
+    if (ROLE == ROLE_V) {
+      n_met[9] = at_b(a.med + 2 * cs, ob);
+    } else {
+      n_met[9] = at_b(a.med, ob);
+      n_met[10] = at_b(a.med + cs, ob);
+    }
+  };
+  issue(kbeg);
+
+  for (int k = kbeg; k <= kend; k++) {
+    const unsigned o = (unsigned)(k + kNG) * upz + col;
+    const unsigned qb = o * (unsigned)sizeof(TS);
+    TC hv[NCG][HPER], mt[9 + NMD], pre_v[NOUT], end_v[NOUT];
+#pragma unroll
+    for (int c = 0; c < NCG; c++) {
+#pragma unroll
+      for (int s = 0; s < 4; s++) q[c][s] = q[c][s + 1];
+      q[c][4] = n_lead[c];
+#pragma unroll
+      for (int p = 0; p < HPER; p++) hv[c][p] = n_hv[c][p];
+    }
+#pragma unroll
+    for (int m = 0; m < 9 + NMD; m++) mt[m] = n_met[m];
+    issue(k < kend ? k + 1 : kend);
+#pragma unroll
+    for (int c = 0; c < NOUT; c++) {
+      pre_v[c] = need_pre ? gld(a.w_pre, OOFF + c, o) : TC(0);
+      end_v[c] = need_end ? gld(a.w_end, OOFF + c, o) : TC(0);
+    }
+    const TC xix = mt[0], xiy = mt[1], xiz = mt[2], etx = mt[3], ety = mt[4], etz = mt[5];
+    const TC ztx = mt[6], zty = mt[7], ztz = mt[8];
+    const int buf = (k - kbeg) & 1;
+    TC *lb = ldsf + buf * LBUF;
+#pragma unroll
+    for (int c = 0; c < NCG; c++) {
+      lb[((COFF + c) * LH + ly + HYL) * LW + lx + HXL] = q[c][zc0];
+#pragma unroll
+      for (int p = 0; p < HPER; p++)
+        if (h_lds[p] >= 0) lb[(COFF + c) * LH * LW + h_lds[p]] = hv[c][p];
+    }
+    __syncthreads();
+
+    TC Lg[3][3];
+    TC dv[3] = {TC(0), TC(0), TC(0)};
+#pragma unroll
+    for (int cg = 0; cg < NCG; cg++) {
+      const int c = COFF + cg;
+      TC dx = TC(0), dy = TC(0), dz = TC(0);
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        dx += Drp<TC, DX>::c(n) * lb[(c * LH + ly + HYL) * LW + lx + n];
+        dy += Drp<TC, DY>::c(n) * lb[(c * LH + ly + n) * LW + lx + HXL];
+        dz += Drp<TC, DZ>::c(n) * q[cg][n];
+      }
+      const TC g0 = xix * dx + etx * dy + ztx * dz;
+      const TC g1 = xiy * dx + ety * dy + zty * dz;
+      const TC g2 = xiz * dx + etz * dy + ztz * dz;
+      if (c < 3) {
+        Lg[c][0] = g0; Lg[c][1] = g1; Lg[c][2] = g2;
+      } else if (c == TXX) {
+        dv[0] += g0;
+      } else if (c == TYY) {
+        dv[1] += g1;
+      } else if (c == TZZ) {
+        dv[2] += g2;
+      } else if (c == TYZ) {
+        dv[1] += g2; dv[2] += g1;
+      } else if (c == TXZ) {
+        dv[0] += g2; dv[2] += g0;
+      } else {
+        dv[0] += g1; dv[1] += g0;
+      }
+    }
+
+    if (owner) {
+      TC h[NOUT];
+      if constexpr (ROLE == ROLE_V) {
+        const TC slw = mt[9];
+        h[0] = slw * dv[0];
+        h[1] = slw * dv[1];
+        h[2] = slw * dv[2];
+      } else {
+        const TC lam = mt[9], mu = mt[10];
+        const TC lam2mu = lam + TC(2) * mu;
+        h[TXX - 3] = lam2mu * Lg[0][0] + lam * (Lg[1][1] + Lg[2][2]);
+        h[TYY - 3] = lam2mu * Lg[1][1] + lam * (Lg[0][0] + Lg[2][2]);
+        h[TZZ - 3] = lam2mu * Lg[2][2] + lam * (Lg[0][0] + Lg[1][1]);
+        h[TXY - 3] = mu * (Lg[0][1] + Lg[1][0]);
+        h[TXZ - 3] = mu * (Lg[0][2] + Lg[2][0]);
+        h[TYZ - 3] = mu * (Lg[1][2] + Lg[2][1]);
+      }
+#pragma unroll
+      for (int cg = 0; cg < NOUT; cg++) {
+        const int c = OOFF + cg;
+        TS *wout = a.w_out + so + c * cs;
+        TS *wend = a.w_end + so + c * cs;
+        const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
+        if (mode == MODE_S12) {
+          St<TS>::st(&at_b(wout, qb), (pre_v[cg] + a.ca * h[cg]) * is);
+          St<TS>::st(&at_b(wend, qb), (end_v[cg] + a.cb * h[cg]) * is);
+        } else if (mode == MODE_S0) {
+          St<TS>::st(&at_b(wout, qb), (pre_v[cg] + a.ca * h[cg]) * is);
+          St<TS>::st(&at_b(wend, qb), (pre_v[cg] + a.cb * h[cg]) * is);
+        } else if (mode == MODE_S3) {
+          St<TS>::st(&at_b(wend, qb), (end_v[cg] + a.cb * h[cg]) * is);
+        } else {
+          St<TS>::st(&at_b(wout, qb), h[cg] * is);
+        }
+      }
+    }
+  }
+}
+
+// MINW: minimum waves per SIMD, which caps the registers per thread (4 waves: 128)
+template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINW, bool CART>
+__global__ __launch_bounds__(2 * TX *TY, MINW) void k_march_split(StageArgs<TS, TC> a, MarchSet b, MarchGrid g) {
+  constexpr int NT = TX * TY;
+  __shared__ TC lds[2][kNC][TY + 4][TX + 4];
+  const int nwg = gridDim.x;
+  const int hw = blockIdx.x;
+  const int q8 = nwg / 8, r8 = nwg % 8;
+  const int xcd = hw % 8, m = hw / 8;
+  const int wg = xcd * q8 + (xcd < r8 ? xcd : r8) + m;
+  const int per_shot = g.wg_beg[b.nbox];
+  const int shot = wg / per_shot;
+  const int rw = wg - shot * per_shot;
+  int bx = 0;
+  while (bx < b.nbox - 1 && rw >= g.wg_beg[bx + 1]) bx++;
+  const int lw = rw - g.wg_beg[bx];
+  const int ntx = g.ntx[bx], nty = g.nty[bx];
+  const int tx_t = lw % ntx;
+  const int ty_t = (lw / ntx) % nty;
+  const int zc = lw / (ntx * nty);
+  if (a.shot_active && !a.shot_active[shot]) return;
+  const int i0 = b.box[bx][0] + tx_t * TX, j0 = b.box[bx][2] + ty_t * TY;
+  const int kbeg = b.box[bx][4] + zc * b.zchunk;
+  int kend = kbeg + b.zchunk - 1;
+  if (kend > b.box[bx][5]) kend = b.box[bx][5];
+  const int tid = threadIdx.x;
+  // wave-uniform role: NT is a multiple of the wavefront size
+  if (tid < NT)
+    split_sweep<ROLE_V, DX, DY, DZ, TS, TC, TX, TY, CART>(a, &lds[0][0][0][0], tid, i0, j0, b.box[bx][1],
+                                                           b.box[bx][3], kbeg, kend, shot);
+  else
+    split_sweep<ROLE_T, DX, DY, DZ, TS, TC, TX, TY, CART>(a, &lds[0][0][0][0], tid - NT, i0, j0, b.box[bx][1],
+                                                           b.box[bx][3], kbeg, kend, shot);
+}
+
+template <typename TS, typename TC, int TX, int TY, int MINW, bool CART>
+static hipError_t launch_split_t(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
+                                 hipStream_t st) {
+  static_assert((TX * TY) % 64 == 0, "roles must split at a wavefront boundary");
+  MarchGrid g{};
+  int acc = 0;
+  for (int q = 0; q < b.nbox; q++) {
+    g.wg_beg[q] = acc;
+    g.ntx[q] = (b.box[q][1] - b.box[q][0] + TX) / TX;
+    g.nty[q] = (b.box[q][3] - b.box[q][2] + TY) / TY;
+    g.nzc[q] = (b.box[q][5] - b.box[q][4] + b.zchunk) / b.zchunk;
+    acc += g.ntx[q] * g.nty[q] * g.nzc[q];
+  }
+  g.wg_beg[b.nbox] = acc;
+  const long nwg = (long)acc * nshots;
+  if (nwg == 0) return hipSuccess;
+  dim3 grid((unsigned)nwg), block(2 * TX * TY);
+  const int sel = dx * 4 + dy * 2 + dz;
+#define CGFD_L(X, Y, Z) \
+  hipLaunchKernelGGL((k_march_split<X, Y, Z, TS, TC, TX, TY, MINW, CART>), grid, block, 0, st, a, b, g)
+  switch (sel) {
+    case 0: CGFD_L(0, 0, 0); break;
+    case 1: CGFD_L(0, 0, 1); break;
+    case 2: CGFD_L(0, 1, 0); break;
+    case 3: CGFD_L(0, 1, 1); break;
+    case 4: CGFD_L(1, 0, 0); break;
+    case 5: CGFD_L(1, 0, 1); break;
+    case 6: CGFD_L(1, 1, 0); break;
+    default: CGFD_L(1, 1, 1); break;
+  }
+#undef CGFD_L
+  return hipGetLastError();
+}
+
 template <typename TS, typename TC, int TX, int TY, int MINB, bool PML, bool CART>
 static hipError_t launch_march_t(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz,
                                  int nshots, hipStream_t st) {
@@ -295,19 +573,36 @@ static hipError_t launch_march_t(const StageArgs<TS, TC> &a, const MarchSet &b, 
 }
 
 // tile shapes the autotuner chooses from (TX x TY threads)
-int interior_num_shapes() { return 4; }
+#if !defined(CGFD_TYPE) || CGFD_TYPE == 0
+// 0..3: one thread per column; 4..6: split roles, two threads per column
+int interior_num_shapes() { return 7; }
 
 void interior_shape(int id, int *tx, int *ty) {
-  static const int T[4][2] = {{64, 4}, {32, 8}, {64, 8}, {32, 16}};
-  id = id < 0 ? 0 : id > 3 ? 3 : id;
+  static const int T[7][2] = {{64, 4}, {32, 8}, {64, 8}, {32, 16}, {64, 8}, {32, 8}, {64, 4}};
+  id = id < 0 ? 0 : id > 6 ? 6 : id;
   *tx = T[id][0];
   *ty = T[id][1];
 }
+#endif
 
 template <typename TS, typename TC>
 hipError_t launch_interior(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
                            hipStream_t st) {
   if (b.nbox < 1) return hipSuccess;
+  if (b.shape >= 4) {
+    if (a.met_const) {
+      switch (b.shape) {
+        case 5: return launch_split_t<TS, TC, 32, 8, 4, true>(a, b, dx, dy, dz, nshots, st);
+        case 6: return launch_split_t<TS, TC, 64, 4, 4, true>(a, b, dx, dy, dz, nshots, st);
+        default: return launch_split_t<TS, TC, 64, 8, 1, true>(a, b, dx, dy, dz, nshots, st);
+      }
+    }
+    switch (b.shape) {
+      case 5: return launch_split_t<TS, TC, 32, 8, 4, false>(a, b, dx, dy, dz, nshots, st);
+      case 6: return launch_split_t<TS, TC, 64, 4, 4, false>(a, b, dx, dy, dz, nshots, st);
+      default: return launch_split_t<TS, TC, 64, 8, 1, false>(a, b, dx, dy, dz, nshots, st);
+    }
+  }
   if (a.met_const) {
     switch (b.shape) {
       case 1: return launch_march_t<TS, TC, 32, 8, 2, false, true>(a, b, dx, dy, dz, nshots, st);
@@ -338,10 +633,20 @@ hipError_t launch_pml_march(const StageArgs<TS, TC> &a, const MarchSet &b, int d
                                               hipStream_t);                                                   \
   template hipError_t launch_pml_march<TS, TC>(const StageArgs<TS, TC> &, const MarchSet &, int, int, int, int,  \
                                                hipStream_t);
+// CGFD_TYPE selects one storage type per object so the four compile in
+// parallel; without it every type is instantiated here.
+#if !defined(CGFD_TYPE) || CGFD_TYPE == 0
 CGFD_INST(float, float)
+#endif
+#if !defined(CGFD_TYPE) || CGFD_TYPE == 1
 CGFD_INST(double, double)
+#endif
+#if !defined(CGFD_TYPE) || CGFD_TYPE == 2
 CGFD_INST(__half, float)
+#endif
+#if !defined(CGFD_TYPE) || CGFD_TYPE == 3
 CGFD_INST(__hip_bfloat16, float)
+#endif
 #undef CGFD_INST
 
 }  // namespace cgfd

@@ -14,91 +14,103 @@
 
 namespace cgfd {
 
+// One face fi (its box must contain the point): Dd are the raw derivatives
+// along the face's axis and e1..e3 the matching metric row.
+template <typename TS, typename TC>
+__device__ __forceinline__ void pml_face(const StageArgs<TS, TC> &a, int fi, int shot, int i, int j, int k,
+                                         const TC *Dd, TC e1, TC e2, TC e3, TC ztx, TC zty, TC ztz, TC lam, TC mu,
+                                         TC lam2mu, TC slw, TC *h) {
+  const Layout &L = a.L;
+  const bool top = a.free_top != 0;
+  const int ktop = L.nk - 1;
+  auto S = [&](int c) -> TC { return St<TS>::scaled ? (TC)a.sc[c] : TC(1); };
+  const FaceArgs<TS, TC> &F = a.f[fi];
+  const int dim = fi >> 1;
+  const int ic = (dim == 0) ? i - F.i1 : (dim == 1) ? j - F.j1 : k - F.k1;
+  const unsigned ia = (unsigned)(((k - F.k1) * F.bj + (j - F.j1)) * F.bi + (i - F.i1));
+  const TC cD = F.D[ic], cA = F.A[ic], cB = F.B[ic];
+  const TC Bm1 = cB - TC(1);
+  TC R[kNC];
+  R[VX] = slw * (e1 * Dd[TXX] + e2 * Dd[TXY] + e3 * Dd[TXZ]);
+  R[VY] = slw * (e1 * Dd[TXY] + e2 * Dd[TYY] + e3 * Dd[TYZ]);
+  R[VZ] = slw * (e1 * Dd[TXZ] + e2 * Dd[TYZ] + e3 * Dd[TZZ]);
+  R[TXX] = lam2mu * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam * e3 * Dd[VZ];
+  R[TYY] = lam * e1 * Dd[VX] + lam2mu * e2 * Dd[VY] + lam * e3 * Dd[VZ];
+  R[TZZ] = lam * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam2mu * e3 * Dd[VZ];
+  R[TXY] = mu * (e2 * Dd[VX] + e1 * Dd[VY]);
+  R[TXZ] = mu * (e3 * Dd[VX] + e1 * Dd[VZ]);
+  R[TYZ] = mu * (e3 * Dd[VY] + e2 * Dd[VZ]);
+  const long fso = (long)shot * F.shot_stride;
+  const unsigned fs = (unsigned)F.fsize;
+  TC P[kNC], Ph[kNC], pre_a[kNC], end_a[kNC];
+#pragma unroll
+  for (int c = 0; c < kNC; c++) {
+    const unsigned qb = (c * fs + ia) * (unsigned)sizeof(TS);
+    P[c] = (TC)St<TS>::ld(&at_b(F.a_in + fso, qb)) * S(c);
+    if (a.mode == MODE_S12) pre_a[c] = (TC)St<TS>::ld(&at_b(F.a_pre + fso, qb)) * S(c);
+    if (a.mode == MODE_S12 || a.mode == MODE_S3) end_a[c] = (TC)St<TS>::ld(&at_b(F.a_end + fso, qb)) * S(c);
+  }
+#pragma unroll
+  for (int c = 0; c < kNC; c++) {
+    h[c] += Bm1 * R[c] - cB * P[c];
+    Ph[c] = cD * R[c] - cA * P[c];
+  }
+  if (top && dim < 2 && k == ktop) {
+    const long pl = (long)L.ni * L.nj, q = (long)j * L.ni + i;
+    const TC *A = a.mat + (dim == 0 ? 0 : 9 * pl) + q;
+    const TC dvx = A[0] * Dd[VX] + A[pl] * Dd[VY] + A[2 * pl] * Dd[VZ];
+    const TC dvy = A[3 * pl] * Dd[VX] + A[4 * pl] * Dd[VY] + A[5 * pl] * Dd[VZ];
+    const TC dvz = A[6 * pl] * Dd[VX] + A[7 * pl] * Dd[VY] + A[8 * pl] * Dd[VZ];
+    TC t[kNC];
+    t[TXX] = lam2mu * (ztx * dvx) + lam * (zty * dvy + ztz * dvz);
+    t[TYY] = lam2mu * (zty * dvy) + lam * (ztx * dvx + ztz * dvz);
+    t[TZZ] = lam2mu * (ztz * dvz) + lam * (ztx * dvx + zty * dvy);
+    t[TXY] = mu * (zty * dvx + ztx * dvy);
+    t[TXZ] = mu * (ztz * dvx + ztx * dvz);
+    t[TYZ] = mu * (ztz * dvy + zty * dvz);
+    // (coef_B - 1.0) * rhs accumulated in double (sv_curv_col_el_iso_gpu.cu:900-905)
+    const double bm1 = (double)cB - 1.0;
+#pragma unroll
+    for (int c = TXX; c < kNC; c++) {
+      h[c] = (TC)((double)h[c] + bm1 * (double)t[c]);
+      Ph[c] += cD * t[c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kNC; c++) {
+    const unsigned qb = (c * fs + ia) * (unsigned)sizeof(TS);
+    const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
+    if (a.mode == MODE_RHS) {
+      St<TS>::st(&at_b(F.a_out + fso, qb), Ph[c] * is);
+    } else if (a.mode == MODE_S0) {
+      St<TS>::st(&at_b(F.a_out + fso, qb), (P[c] + a.ca * Ph[c]) * is);
+      St<TS>::st(&at_b(F.a_end + fso, qb), (P[c] + a.cb * Ph[c]) * is);
+    } else if (a.mode == MODE_S12) {
+      St<TS>::st(&at_b(F.a_out + fso, qb), (pre_a[c] + a.ca * Ph[c]) * is);
+      St<TS>::st(&at_b(F.a_end + fso, qb), (end_a[c] + a.cb * Ph[c]) * is);
+    } else {
+      St<TS>::st(&at_b(F.a_end + fso, qb), (end_a[c] + a.cb * Ph[c]) * is);
+    }
+  }
+}
+
+// Every face box containing the point, in the reference's face order.
 template <typename TS, typename TC>
 __device__ __forceinline__ void pml_apply(const StageArgs<TS, TC> &a, int shot, int i, int j, int k, const TC *Dx,
                                           const TC *Dy, const TC *Dz, TC xix, TC xiy, TC xiz, TC etx, TC ety,
                                           TC etz, TC ztx, TC zty, TC ztz, TC lam, TC mu, TC lam2mu, TC slw,
                                           TC *h) {
-  const Layout &L = a.L;
-  const bool top = a.free_top != 0;
-  const int ktop = L.nk - 1;
-  auto S = [&](int c) -> TC { return St<TS>::scaled ? (TC)a.sc[c] : TC(1); };
 #pragma unroll
   for (int fi = 0; fi < 6; fi++) {
     const FaceArgs<TS, TC> &F = a.f[fi];
     if (!F.on) continue;
     if (i < F.i1 || i > F.i2 || j < F.j1 || j > F.j2 || k < F.k1 || k > F.k2) continue;
     const int dim = fi >> 1;
-    const int ic = (dim == 0) ? i - F.i1 : (dim == 1) ? j - F.j1 : k - F.k1;
-    const unsigned ia = (unsigned)(((k - F.k1) * F.bj + (j - F.j1)) * F.bi + (i - F.i1));
-    const TC cD = F.D[ic], cA = F.A[ic], cB = F.B[ic];
-    const TC Bm1 = cB - TC(1);
     const TC *Dd = (dim == 0) ? Dx : (dim == 1) ? Dy : Dz;
     const TC e1 = (dim == 0) ? xix : (dim == 1) ? etx : ztx;
     const TC e2 = (dim == 0) ? xiy : (dim == 1) ? ety : zty;
     const TC e3 = (dim == 0) ? xiz : (dim == 1) ? etz : ztz;
-    TC R[kNC];
-    R[VX] = slw * (e1 * Dd[TXX] + e2 * Dd[TXY] + e3 * Dd[TXZ]);
-    R[VY] = slw * (e1 * Dd[TXY] + e2 * Dd[TYY] + e3 * Dd[TYZ]);
-    R[VZ] = slw * (e1 * Dd[TXZ] + e2 * Dd[TYZ] + e3 * Dd[TZZ]);
-    R[TXX] = lam2mu * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam * e3 * Dd[VZ];
-    R[TYY] = lam * e1 * Dd[VX] + lam2mu * e2 * Dd[VY] + lam * e3 * Dd[VZ];
-    R[TZZ] = lam * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam2mu * e3 * Dd[VZ];
-    R[TXY] = mu * (e2 * Dd[VX] + e1 * Dd[VY]);
-    R[TXZ] = mu * (e3 * Dd[VX] + e1 * Dd[VZ]);
-    R[TYZ] = mu * (e3 * Dd[VY] + e2 * Dd[VZ]);
-    const long fso = (long)shot * F.shot_stride;
-    const unsigned fs = (unsigned)F.fsize;
-    TC P[kNC], Ph[kNC], pre_a[kNC], end_a[kNC];
-#pragma unroll
-    for (int c = 0; c < kNC; c++) {
-      const unsigned qb = (c * fs + ia) * (unsigned)sizeof(TS);
-      P[c] = (TC)St<TS>::ld(&at_b(F.a_in + fso, qb)) * S(c);
-      if (a.mode == MODE_S12) pre_a[c] = (TC)St<TS>::ld(&at_b(F.a_pre + fso, qb)) * S(c);
-      if (a.mode == MODE_S12 || a.mode == MODE_S3) end_a[c] = (TC)St<TS>::ld(&at_b(F.a_end + fso, qb)) * S(c);
-    }
-#pragma unroll
-    for (int c = 0; c < kNC; c++) {
-      h[c] += Bm1 * R[c] - cB * P[c];
-      Ph[c] = cD * R[c] - cA * P[c];
-    }
-    if (top && dim < 2 && k == ktop) {
-      const long pl = (long)L.ni * L.nj, q = (long)j * L.ni + i;
-      const TC *A = a.mat + (dim == 0 ? 0 : 9 * pl) + q;
-      const TC dvx = A[0] * Dd[VX] + A[pl] * Dd[VY] + A[2 * pl] * Dd[VZ];
-      const TC dvy = A[3 * pl] * Dd[VX] + A[4 * pl] * Dd[VY] + A[5 * pl] * Dd[VZ];
-      const TC dvz = A[6 * pl] * Dd[VX] + A[7 * pl] * Dd[VY] + A[8 * pl] * Dd[VZ];
-      TC t[kNC];
-      t[TXX] = lam2mu * (ztx * dvx) + lam * (zty * dvy + ztz * dvz);
-      t[TYY] = lam2mu * (zty * dvy) + lam * (ztx * dvx + ztz * dvz);
-      t[TZZ] = lam2mu * (ztz * dvz) + lam * (ztx * dvx + zty * dvy);
-      t[TXY] = mu * (zty * dvx + ztx * dvy);
-      t[TXZ] = mu * (ztz * dvx + ztx * dvz);
-      t[TYZ] = mu * (ztz * dvy + zty * dvz);
-      // (coef_B - 1.0) * rhs accumulated in double (sv_curv_col_el_iso_gpu.cu:900-905)
-      const double bm1 = (double)cB - 1.0;
-#pragma unroll
-      for (int c = TXX; c < kNC; c++) {
-        h[c] = (TC)((double)h[c] + bm1 * (double)t[c]);
-        Ph[c] += cD * t[c];
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < kNC; c++) {
-      const unsigned qb = (c * fs + ia) * (unsigned)sizeof(TS);
-      const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
-      if (a.mode == MODE_RHS) {
-        St<TS>::st(&at_b(F.a_out + fso, qb), Ph[c] * is);
-      } else if (a.mode == MODE_S0) {
-        St<TS>::st(&at_b(F.a_out + fso, qb), (P[c] + a.ca * Ph[c]) * is);
-        St<TS>::st(&at_b(F.a_end + fso, qb), (P[c] + a.cb * Ph[c]) * is);
-      } else if (a.mode == MODE_S12) {
-        St<TS>::st(&at_b(F.a_out + fso, qb), (pre_a[c] + a.ca * Ph[c]) * is);
-        St<TS>::st(&at_b(F.a_end + fso, qb), (end_a[c] + a.cb * Ph[c]) * is);
-      } else {
-        St<TS>::st(&at_b(F.a_end + fso, qb), (end_a[c] + a.cb * Ph[c]) * is);
-      }
-    }
+    pml_face<TS, TC>(a, fi, shot, i, j, k, Dd, e1, e2, e3, ztx, zty, ztz, lam, mu, lam2mu, slw, h);
   }
 }
 

@@ -108,3 +108,36 @@ def cells(m):
 def interior_cells(m, free_top=True):
     lay = PML["nlay"] + 1
     return (m["ni"] - 2 * lay) * (m["nj"] - 2 * lay) * (m["nk"] - lay - (4 if free_top else 0))
+
+
+def shot_requests(m, n, seed, nt_range=(500, 2000), f0_range=(1.0, 4.0), depth_m=(1000.0, 10000.0)):
+    """Seeded shot requests on a model: random epicentre inside the PML frame,
+    depth in depth_m clamped to [1 km, bottom - PML - 3 cells] (index units of
+    the top spacing), random moment tensor of scale 1e16 N m, Ricker f0 in
+    f0_range with t0 = 1.2 / f0, and a step count in nt_range."""
+    rng = np.random.default_rng(seed)
+    ni, nj, nk = m["ni"], m["nj"], m["nk"]
+    lay = PML["nlay"] + 1
+    z = m["z"]
+    dz = float(z[nk + 2, 3, 3] - z[nk + 1, 3, 3])  # top spacing
+    kmax = nk - 1 - lay - 3
+    out = []
+    for _ in range(n):
+        sx = float(rng.uniform(lay + 3, ni - lay - 4))
+        sy = float(rng.uniform(lay + 3, nj - lay - 4))
+        d = float(np.clip(rng.uniform(*depth_m) / dz, 1000.0 / dz, kmax))
+        f0 = float(rng.uniform(*f0_range))
+        nt = int(rng.integers(nt_range[0], nt_range[1] + 1))
+        m6 = rng.normal(size=6) * 1e16
+        src = _source(ni, nj, nk, sx, sy, d, m["dt"], f0, 1.2 / f0, m6, nt)
+        out.append(dict(src=src, nt=nt, f0=f0, depth_idx=d, epicentre=(sx, sy)))
+    return out
+
+
+def model_arrays(m, dtype=np.float32):
+    """Keyword arguments of cgfd.Server.add_model for a workload."""
+    f = dtype
+    return dict(metric=np.asarray(m["metric"], f), lam=np.asarray(m["lam"], f), mu=np.asarray(m["mu"], f),
+                slw=np.asarray(m["slw"], f), free_top=True, mvx=np.asarray(m["mvx"], f),
+                mvy=np.asarray(m["mvy"], f), pml={k: (nl, np.asarray(A, f), np.asarray(B, f), np.asarray(D, f))
+                                                  for k, (nl, A, B, D) in m["pml"].items()})

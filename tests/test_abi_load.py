@@ -53,3 +53,10 @@ def test_footprint_is_host_only_and_scales():
     assert b2 > 3 * b1 // 2 and b3 < b1
     bad = cgfd.Desc(0, 10, 10, 0, 1, 0)
     assert lib.cgfd_footprint_bytes(C.byref(bad), 0, 0) < 0
+
+
+def test_server_arguments_checked_before_device_use():
+    with pytest.raises(cgfd.CgfdError, match="max_batch"):
+        cgfd.Server(max_batch=0)
+    with pytest.raises(cgfd.CgfdError, match="chunk_steps"):
+        cgfd.Server(chunk_steps=12)
