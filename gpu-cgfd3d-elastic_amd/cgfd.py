@@ -34,7 +34,8 @@ EXPORTS = [
     "cgfd_pml_profile", "cgfd_free_surface_matrices", "cgfd_wavelet", "cgfd_angle2moment",
     "cgfd_source_point", "cgfd_receiver_index", "cgfd_storage_scales",
     "cgfd_server_create", "cgfd_server_destroy", "cgfd_server_add_model", "cgfd_server_submit",
-    "cgfd_server_wait", "cgfd_server_result", "cgfd_server_get_stats",
+    "cgfd_server_wait", "cgfd_server_result", "cgfd_server_get_stats", "cgfd_fetch_subvolume",
+    "cgfd_sac_write", "cgfd_station_sac", "cgfd_snap_open", "cgfd_snap_append", "cgfd_snap_close",
 ]
 
 
@@ -66,6 +67,12 @@ class SourceDesc(C.Structure):
 
 class CgfdError(RuntimeError):
     pass
+
+
+class SnapDesc(C.Structure):
+    _fields_ = [("ni", C.c_int), ("nj", C.c_int), ("nk", C.c_int), ("out_vel", C.c_int), ("out_stress", C.c_int),
+                ("out_strain", C.c_int), ("first_global", C.c_int * 3), ("first_local", C.c_int * 3),
+                ("stride", C.c_int * 3), ("topo", C.c_int * 3)]
 
 
 class ServerDesc(C.Structure):
@@ -166,6 +173,13 @@ def load(path=LIB_PATH):
         "cgfd_source_point": ([i, i, i, d, d, d, i, d, d, d, i, d, d, d, vp, vp, vp, vp, vp, vp, vp], i),
         "cgfd_receiver_index": ([i, i, i, d, d, d, i, vp, vp], i),
         "cgfd_storage_scales": ([i, i, i, i, vp, vp, vp, vp, C.POINTER(SourceDesc), d, vp], i),
+        "cgfd_fetch_subvolume": ([vp, i, i, C.c_uint, i, i, i, i, i, i, i, i, i, i, vp], i),
+        "cgfd_sac_write": ([C.c_char_p, vp, i, C.c_float, C.c_float, vp, vp, C.c_char_p, C.c_char_p], i),
+        "cgfd_station_sac": ([C.c_char_p, C.c_char_p, vp, i, C.c_float, C.c_float, C.c_float, C.c_float, i,
+                              C.c_float, C.c_float], i),
+        "cgfd_snap_open": ([C.c_char_p, C.POINTER(SnapDesc), C.POINTER(vp)], i),
+        "cgfd_snap_append": ([vp, C.c_float, vp, vp, vp], i),
+        "cgfd_snap_close": ([vp], i),
         "cgfd_server_create": ([C.POINTER(ServerDesc), C.POINTER(vp)], i),
         "cgfd_server_destroy": ([vp], None),
         "cgfd_server_add_model": ([vp, C.POINTER(ModelDesc), C.POINTER(i)], i),
@@ -313,6 +327,18 @@ class Session:
         a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         _check(_lib.cgfd_get_tile(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
         return dict(shape=a.value, tx=b.value, ty=c.value, zchunk=d.value)
+
+    def fetch_subvolume(self, shot, level, comps, start, count, stride=(1, 1, 1), dtype=np.float32):
+        """Components `comps` (indices 0..8) at start + n*stride (i, j, k order,
+        physical indices) as [len(comps)][nk][nj][ni]."""
+        mask = 0
+        for c in comps:
+            mask |= 1 << c
+        (i1, j1, k1), (ni, nj, nk), (di, dj, dk) = start, count, stride
+        out = np.zeros((len(comps), nk, nj, ni), dtype=dtype)
+        _check(_lib.cgfd_fetch_subvolume(self.h, shot, level, mask, i1, j1, k1, ni, nj, nk, di, dj, dk,
+                                         DT_F64 if dtype == np.float64 else DT_F32, _ptr(out)))
+        return out
 
     def set_timing(self, on):
         _check(_lib.cgfd_set_timing(self.h, 1 if on else 0))
@@ -574,3 +600,45 @@ def footprint_bytes(ni, nj, nk, storage=STORE_F32, nshots=1, nrecv=0, nt_total=0
     load()
     d = Desc(ni, nj, nk, storage, nshots, 0)
     return int(_lib.cgfd_footprint_bytes(C.byref(d), nrecv, nt_total))
+
+
+# ---------------------------------------------------------------- output files
+def sac_write(path, data, delta, b=0.0, ev=(0, 0, 0, 0), st=(0, 0, 0, 0), kstnm="-12345", kcmpnm="-12345"):
+    load()
+    d = _arr(data, np.float32)
+    e = _arr(ev, np.float32)
+    t = _arr(st, np.float32)
+    _check(_lib.cgfd_sac_write(str(path).encode(), _ptr(d), d.size, float(delta), float(b), _ptr(e), _ptr(t),
+                               kstnm.encode(), kcmpnm.encode()))
+
+
+def station_sac(out_dir, name, seismo, dt, x, y, z, strain=False, lam=0.0, mu=0.0):
+    """The 9 (+6 strain) SAC files of one receiver; seismo [9][nt]."""
+    load()
+    sz = _arr(seismo, np.float32).reshape(9, -1)
+    _check(_lib.cgfd_station_sac(str(out_dir).encode(), name.encode(), _ptr(sz), sz.shape[1], float(dt), float(x),
+                                 float(y), float(z), 1 if strain else 0, float(lam), float(mu)))
+
+
+class Snapshot:
+    """NetCDF-3 snapshot file with the reference's names and attributes."""
+
+    def __init__(self, path, ni, nj, nk, vel=True, stress=True, strain=False, first_global=(0, 0, 0),
+                 first_local=(3, 3, 3), stride=(1, 1, 1), topo=(0, 0, 0)):
+        load()
+        d = SnapDesc(ni, nj, nk, int(vel), int(stress), int(strain), (C.c_int * 3)(*first_global),
+                     (C.c_int * 3)(*first_local), (C.c_int * 3)(*stride), (C.c_int * 3)(*topo))
+        self.h = C.c_void_p()
+        self.n = ni * nj * nk
+        _check(_lib.cgfd_snap_open(str(path).encode(), C.byref(d), C.byref(self.h)))
+
+    def append(self, time, w9, lam=None, mu=None):
+        w = _arr(w9, np.float32)
+        la = _arr(lam, np.float32) if lam is not None else None
+        m = _arr(mu, np.float32) if mu is not None else None
+        _check(_lib.cgfd_snap_append(self.h, float(time), _ptr(w), _ptr(la), _ptr(m)))
+
+    def close(self):
+        if self.h:
+            _check(_lib.cgfd_snap_close(self.h))
+            self.h = C.c_void_p()

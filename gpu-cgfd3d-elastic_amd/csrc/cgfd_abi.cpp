@@ -197,6 +197,8 @@ struct cgfd_session {
     int slot0 = -1;
   } graph[2];
   std::vector<char> rec_changed;  // per shot: receivers replaced since the last rebuild
+  void *gather = nullptr;         // device buffer of cgfd_fetch_subvolume
+  size_t gather_bytes = 0;
   double last_run_ms = 0, last_int_avg_ms = 0;
   int last_int_n = 0;
   std::vector<void *> allocs;
@@ -1277,6 +1279,58 @@ int cgfd_fetch_level(cgfd_session *s, int shot, int level, int dtype, void *w9) 
           else ((float *)w9)[q] = (float)x;
         }
   return CGFD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+template <typename TS, typename TC>
+int subvolume_t(cgfd_session *s, int shot, int level, unsigned mask, int i1, int j1, int k1, int ni, int nj, int nk,
+                int di, int dj, int dk, int dtype, void *out) {
+  const size_t n = (size_t)ni * nj * nk;
+  if (s->gather_bytes < n * sizeof(TC)) {
+    if (s->gather) dfree(s, s->gather);
+    s->gather_bytes = 0;
+    int r = dalloc(s, &s->gather, n * sizeof(TC));
+    if (r) return r;
+    s->gather_bytes = n * sizeof(TC);
+  }
+  std::vector<TC> h(n);
+  size_t q = 0;
+  for (int c = 0; c < kNC; c++) {
+    if (!(mask >> c & 1u)) continue;
+    const TS *w = (const TS *)wav_level(s, shot, s->slot[level]) + (size_t)c * s->L.cs;
+    const TC sc = St<TS>::scaled ? (TC)s->sc[c] : TC(1);
+    hipError_t e = launch_gather<TS, TC>(w, s->L, i1, j1, k1, ni, nj, nk, di, dj, dk, sc, (TC *)s->gather, s->st);
+    if (e != hipSuccess) return fail(CGFD_E_HIP, "gather kernel: %s", hipGetErrorString(e));
+    HIPCHK(d2h(s, h.data(), s->gather, n * sizeof(TC)));
+    for (size_t p = 0; p < n; p++, q++) {
+      if (dtype == CGFD_DT_F64) ((double *)out)[q] = (double)h[p];
+      else ((float *)out)[q] = (float)h[p];
+    }
+  }
+  return CGFD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int cgfd_fetch_subvolume(cgfd_session *s, int shot, int level, unsigned cmp_mask, int i1, int j1, int k1, int ni,
+                         int nj, int nk, int di, int dj, int dk, int dtype, void *out) {
+  if (!s || shot < 0 || shot >= s->d.nshots || level < 0 || level > 3 || !out)
+    return fail(CGFD_E_ARG, "bad fetch_subvolume arguments");
+  if (ni < 1 || nj < 1 || nk < 1 || di < 1 || dj < 1 || dk < 1 || i1 < 0 || j1 < 0 || k1 < 0 ||
+      i1 + (long)(ni - 1) * di > s->d.ni - 1 || j1 + (long)(nj - 1) * dj > s->d.nj - 1 ||
+      k1 + (long)(nk - 1) * dk > s->d.nk - 1)
+    return fail(CGFD_E_ARG, "sub-volume outside the grid");
+  if ((cmp_mask & 0x1ffu) == 0 || (cmp_mask & ~0x1ffu)) return fail(CGFD_E_ARG, "component mask %#x", cmp_mask);
+  HIPCHK(hipStreamSynchronize(s->st));
+  switch (s->d.storage) {
+    case CGFD_STORE_F32: return subvolume_t<float, float>(s, shot, level, cmp_mask, i1, j1, k1, ni, nj, nk, di, dj, dk, dtype, out);
+    case CGFD_STORE_F64: return subvolume_t<double, double>(s, shot, level, cmp_mask, i1, j1, k1, ni, nj, nk, di, dj, dk, dtype, out);
+    case CGFD_STORE_F16: return subvolume_t<__half, float>(s, shot, level, cmp_mask, i1, j1, k1, ni, nj, nk, di, dj, dk, dtype, out);
+    default: return subvolume_t<__hip_bfloat16, float>(s, shot, level, cmp_mask, i1, j1, k1, ni, nj, nk, di, dj, dk, dtype, out);
+  }
 }
 
 int cgfd_fetch_pml_aux(cgfd_session *s, int shot, int face, int level, int dtype, void *out) {

@@ -17,6 +17,12 @@ template <typename TS, typename TC> hipError_t launch_recv(const RecvArgs<TS, TC
 
 hipError_t launch_clock_advance(int *clock, hipStream_t st);
 
+// one component's points (i1 + n*di, j1 + m*dj, k1 + l*dk), physical indices,
+// into out [nk][nj][ni], multiplied by scale
+template <typename TS, typename TC>
+hipError_t launch_gather(const TS *w, const Layout &L, int i1, int j1, int k1, int ni, int nj, int nk, int di,
+                         int dj, int dk, TC scale, TC *out, hipStream_t st);
+
 template <typename T> hipError_t launch_fill(T *p, long n, T v, hipStream_t st);
 
 // Boxes swept by the z-marching kernel ([i0,i1]x[j0,j1]x[k0,k1], physical,

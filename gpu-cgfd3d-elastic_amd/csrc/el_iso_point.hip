@@ -386,7 +386,30 @@ template <typename T> __global__ __launch_bounds__(256) void k_fill(T *p, long n
   for (; i < n; i += stride) p[i] = v;
 }
 
+// ---- strided sub-volume gather (io_snap_pack_buff, io_funcs.cu:1568-1596) ----
+template <typename TS, typename TC>
+__global__ __launch_bounds__(256) void k_gather(const TS *w, Layout L, int i1, int j1, int k1, int ni, int nj, int nk,
+                                                int di, int dj, int dk, TC scale, TC *out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)ni * nj * nk;
+  if (t >= n) return;
+  const int i = (int)(t % ni);
+  const long r = t / ni;
+  const int j = (int)(r % nj), k = (int)(r / nj);
+  out[t] = (TC)St<TS>::ld(w + L.off(i1 + i * di, j1 + j * dj, k1 + k * dk)) * scale;
+}
+
 // ---- launchers ------------------------------------------------------------------
+template <typename TS, typename TC>
+hipError_t launch_gather(const TS *w, const Layout &L, int i1, int j1, int k1, int ni, int nj, int nk, int di,
+                         int dj, int dk, TC scale, TC *out, hipStream_t st) {
+  const long n = (long)ni * nj * nk;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_gather<TS, TC>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w, L, i1, j1, k1, ni,
+                     nj, nk, di, dj, dk, scale, out);
+  return hipGetLastError();
+}
+
 template <typename TS, typename TC>
 hipError_t launch_point(const StageArgs<TS, TC> &a, int dx, int dy, int dz, int nshots, hipStream_t st) {
   const long n = a.nbox ? a.box_end[a.nbox - 1] : 0;
@@ -462,6 +485,8 @@ template <typename T> hipError_t launch_fill(T *p, long n, T v, hipStream_t st) 
 #define CGFD_INST(TS, TC)                                                                                   \
   template hipError_t launch_point<TS, TC>(const StageArgs<TS, TC> &, int, int, int, int, hipStream_t);    \
   template hipError_t launch_slab<TS, TC>(const StageArgs<TS, TC> &, int, int, int, int, int, hipStream_t); \
+  template hipError_t launch_gather<TS, TC>(const TS *, const Layout &, int, int, int, int, int, int, int, int,   \
+                                            int, TC, TC *, hipStream_t);                                        \
   template hipError_t launch_source<TS, TC>(const SrcArgs<TS, TC> &, hipStream_t);                          \
   template hipError_t launch_recv<TS, TC>(const RecvArgs<TS, TC> &, hipStream_t);
 // CGFD_TYPE selects one storage type per object so the four compile in

@@ -170,6 +170,15 @@ int cgfd_read_seismo(cgfd_session *s, int shot, int dtype, void *out);
 /* One level of one shot back to a host array in reference layout (ghosts zero). */
 int cgfd_fetch_level(cgfd_session *s, int shot, int level, int dtype, void *w9);
 
+/* Strided sub-volume of one level, gathered on the device, for snapshots and
+ * slices (io_snap_pack_buff, src/forward/io_funcs.cu:1568-1596): the points
+ * (i1 + a*di, j1 + b*dj, k1 + c*dk), a < ni, b < nj, c < nk, in physical
+ * 0-based indices, of every component whose bit is set in cmp_mask (bit c =
+ * component c in the order Vx,Vy,Vz,Txx,Tyy,Tzz,Tyz,Txz,Txy), written to out
+ * as [selected component][nk][nj][ni] in dtype.  level: PRE/TMPA/TMPB/END. */
+int cgfd_fetch_subvolume(cgfd_session *s, int shot, int level, unsigned cmp_mask, int i1, int j1, int k1, int ni,
+                         int nj, int nk, int di, int dj, int dk, int dtype, void *out);
+
 /* PML memory of one face: level = PRE/TMPA/TMPB/END or RHS; [9][face box]
  * with the face box indexed iz*(nj*ni)+iy*ni+ix (sv_curv_col_el_iso_gpu.cu:775). */
 int cgfd_fetch_pml_aux(cgfd_session *s, int shot, int face, int level, int dtype, void *out);

@@ -1,0 +1,108 @@
+"""Output formats: SAC traces and NetCDF-3 snapshots, read back independently.
+
+The SAC header is decoded from the published v6 word layout; snapshots are
+read with scipy's NetCDF-3 reader.  Host only.
+"""
+import struct
+
+import numpy as np
+import pytest
+from scipy.io import netcdf_file
+
+import cgfd
+
+
+def _sac(path):
+    raw = open(path, "rb").read()
+    hf = np.frombuffer(raw[:280], dtype=np.float32)
+    hi = np.frombuffer(raw[280:440], dtype=np.int32)
+    hk = raw[440:632]
+    data = np.frombuffer(raw[632:], dtype=np.float32)
+    return hf, hi, hk, data
+
+
+def test_sac_header_and_data(tmp_path):
+    x = np.sin(np.linspace(0, 3, 50)).astype(np.float32)
+    p = tmp_path / "a.sac"
+    cgfd.sac_write(p, x, 0.005, b=0.25, ev=(1, 2, 3, 4), st=(5, 6, 7, 8), kstnm="STA1", kcmpnm="Vz")
+    hf, hi, hk, data = _sac(p)
+    assert len(open(p, "rb").read()) == 632 + 4 * 50
+    assert np.array_equal(data, x)
+    assert hf[0] == np.float32(0.005) and hf[5] == np.float32(0.25)
+    assert hf[6] == np.float32(0.25) + np.float32(0.005) * np.float32(49)
+    assert list(hf[35:39]) == [1, 2, 3, 4] and list(hf[31:35]) == [5, 6, 7, 8]
+    assert hf[57] == 0 and hf[58] == 0 and hf[1] == -12345.0  # cmpaz, cmpinc set; depmin undefined
+    assert hi[6] == 6 and hi[9] == 50 and hi[15] == 1 and hi[35] == 1 and hi[0] == -12345
+    assert hk[0:4] == b"STA1" and hk[160:162] == b"Vz"
+    assert hk[24:32] == b"-12345  "  # khole undefined
+
+
+def test_station_files_names_quirk_and_strain(tmp_path):
+    nt = 40
+    rng = np.random.default_rng(0)
+    seis = rng.normal(size=(9, nt)).astype(np.float32)
+    lam, mu = 3.2e10, 3.0e10
+    cgfd.station_sac(tmp_path, "r1", seis, 0.005, 100.0, 200.0, -50.0, strain=True, lam=lam, mu=mu)
+    names = ["Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Tyz", "Txz", "Txy"]
+    for c, n in enumerate(names):
+        hf, hi, _, data = _sac(tmp_path / f"r1.{n}.sac")
+        assert np.array_equal(data, seis[c])
+        assert hf[5] == 0.0 and hf[34] == np.float32(0.005)  # b = 0; station depth carries dt
+        assert list(hf[31:34]) == [100.0, 200.0, -50.0]
+    # strain from the isotropic compliance: E = (1/2mu) T - lam/(2mu(3lam+2mu)) tr(T) I
+    T = {n: seis[c].astype(np.float64) for c, n in enumerate(names)}
+    tr = T["Txx"] + T["Tyy"] + T["Tzz"]
+    want = {"Exx": T["Txx"] / (2 * mu) - lam / (2 * mu * (3 * lam + 2 * mu)) * tr,
+            "Eyy": T["Tyy"] / (2 * mu) - lam / (2 * mu * (3 * lam + 2 * mu)) * tr,
+            "Ezz": T["Tzz"] / (2 * mu) - lam / (2 * mu * (3 * lam + 2 * mu)) * tr,
+            "Eyz": T["Tyz"] / (2 * mu), "Exz": T["Txz"] / (2 * mu), "Exy": T["Txy"] / (2 * mu)}
+    for n, w in want.items():
+        _, _, _, data = _sac(tmp_path / f"r1.{n}.sac")
+        assert np.allclose(data, w, rtol=1e-5, atol=1e-6 * np.abs(w).max())
+
+
+def test_snapshot_netcdf_layout(tmp_path):
+    ni, nj, nk = 7, 5, 4
+    rng = np.random.default_rng(1)
+    p = tmp_path / "snap_px0_py0_pz0.nc"
+    sn = cgfd.Snapshot(p, ni, nj, nk, vel=True, stress=True, strain=True, first_global=(2, 3, 4),
+                       first_local=(5, 6, 7), stride=(2, 1, 3))
+    lam = rng.uniform(1e10, 3e10, size=(nk, nj, ni)).astype(np.float32)
+    mu = rng.uniform(1e10, 3e10, size=(nk, nj, ni)).astype(np.float32)
+    frames = [rng.normal(size=(9, nk, nj, ni)).astype(np.float32) for _ in range(3)]
+    for t, w in enumerate(frames):
+        sn.append(0.5 * t, w, lam, mu)
+    sn.close()
+    f = netcdf_file(str(p), "r", mmap=False)
+    assert f.dimensions["time"] is None and f.dimensions["i"] == ni and f.dimensions["k"] == nk
+    assert list(f._attributes["first_index_to_snapshot_output"]) == [2, 3, 4]
+    assert list(f._attributes["first_index_in_this_thread_with_ghosts"]) == [5, 6, 7]
+    assert list(f._attributes["index_stride_in_this_thread"]) == [2, 1, 3]
+    assert list(f._attributes["coords_of_mpi_topo"]) == [0, 0, 0]
+    assert np.array_equal(f.variables["time"][:], [0.0, 0.5, 1.0])
+    order = {"Vx": 0, "Vy": 1, "Vz": 2, "Txx": 3, "Tyy": 4, "Tzz": 5, "Tyz": 6, "Txz": 7, "Txy": 8}
+    for n, c in order.items():
+        v = f.variables[n]
+        assert v.dimensions == ("time", "k", "j", "i")
+        for t, w in enumerate(frames):
+            assert np.array_equal(v[t], w[c])
+    w = frames[1].astype(np.float64)
+    l, m = lam.astype(np.float64), mu.astype(np.float64)
+    tr = w[3] + w[4] + w[5]
+    k = l / (2 * m * (3 * l + 2 * m))
+    assert np.allclose(f.variables["Exx"][1], w[3] / (2 * m) - k * tr, rtol=1e-4, atol=1e-14)
+    assert np.allclose(f.variables["Eyz"][1], w[6] / (2 * m), rtol=1e-5)
+    assert np.allclose(f.variables["Exz"][1], w[7] / (2 * m), rtol=1e-5)
+    f.close()
+
+
+def test_snapshot_velocity_only(tmp_path):
+    p = tmp_path / "v.nc"
+    sn = cgfd.Snapshot(p, 3, 2, 2, vel=True, stress=False, strain=False)
+    w = np.arange(9 * 12, dtype=np.float32).reshape(9, 2, 2, 3)
+    sn.append(0.0, w)
+    sn.close()
+    f = netcdf_file(str(p), "r", mmap=False)
+    assert set(f.variables) == {"time", "Vx", "Vy", "Vz"}
+    assert np.array_equal(f.variables["Vz"][0], w[2])
+    f.close()
