@@ -36,6 +36,7 @@ EXPORTS = [
     "cgfd_server_create", "cgfd_server_destroy", "cgfd_server_add_model", "cgfd_server_submit",
     "cgfd_server_wait", "cgfd_server_result", "cgfd_server_get_stats", "cgfd_fetch_subvolume",
     "cgfd_sac_write", "cgfd_station_sac", "cgfd_snap_open", "cgfd_snap_append", "cgfd_snap_close",
+    "cgfd_set_peak_motion", "cgfd_read_peak_motion", "cgfd_peak_motion_write",
 ]
 
 
@@ -180,6 +181,9 @@ def load(path=LIB_PATH):
         "cgfd_snap_open": ([C.c_char_p, C.POINTER(SnapDesc), C.POINTER(vp)], i),
         "cgfd_snap_append": ([vp, C.c_float, vp, vp, vp], i),
         "cgfd_snap_close": ([vp], i),
+        "cgfd_set_peak_motion": ([vp, i], i),
+        "cgfd_read_peak_motion": ([vp, i, i, vp], i),
+        "cgfd_peak_motion_write": ([C.c_char_p, i, i, vp, vp, vp, vp], i),
         "cgfd_server_create": ([C.POINTER(ServerDesc), C.POINTER(vp)], i),
         "cgfd_server_destroy": ([vp], None),
         "cgfd_server_add_model": ([vp, C.POINTER(ModelDesc), C.POINTER(i)], i),
@@ -338,6 +342,15 @@ class Session:
         out = np.zeros((len(comps), nk, nj, ni), dtype=dtype)
         _check(_lib.cgfd_fetch_subvolume(self.h, shot, level, mask, i1, j1, k1, ni, nj, nk, di, dj, dk,
                                          DT_F64 if dtype == np.float64 else DT_F32, _ptr(out)))
+        return out
+
+    def set_peak_motion(self, on):
+        _check(_lib.cgfd_set_peak_motion(self.h, 1 if on else 0))
+
+    def read_peak_motion(self, shot, dtype=np.float64):
+        """[15][ny][nx] (PGV PGVh PGVx PGVy PGVz PGA ... PGDz), ghosts zero."""
+        out = np.zeros((15, self.ny, self.nx), dtype=dtype)
+        _check(_lib.cgfd_read_peak_motion(self.h, shot, DT_F64 if dtype == np.float64 else DT_F32, _ptr(out)))
         return out
 
     def set_timing(self, on):
@@ -642,3 +655,19 @@ class Snapshot:
         if self.h:
             _check(_lib.cgfd_snap_close(self.h))
             self.h = C.c_void_p()
+
+
+PEAK_NAMES = ["PGV", "PGVh", "PGVx", "PGVy", "PGVz", "PGA", "PGAh", "PGAx", "PGAy", "PGAz",
+              "PGD", "PGDh", "PGDx", "PGDy", "PGDz"]
+
+
+def peak_motion_write(path, pg, first_physical=(3, 3), count_physical=None, topo=(0, 0, 0)):
+    """PG_V_A_D file from maps [15][ny][nx]."""
+    load()
+    m = _arr(pg, np.float32)
+    ny, nx = m.shape[1], m.shape[2]
+    cp = count_physical if count_physical is not None else (nx - 6, ny - 6)
+    a = np.ascontiguousarray(first_physical, dtype=np.int32)
+    b = np.ascontiguousarray(cp, dtype=np.int32)
+    c = np.ascontiguousarray(topo, dtype=np.int32)
+    _check(_lib.cgfd_peak_motion_write(str(path).encode(), nx, ny, _ptr(m), _ptr(a), _ptr(b), _ptr(c)))

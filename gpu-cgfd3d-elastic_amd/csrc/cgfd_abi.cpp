@@ -198,6 +198,7 @@ struct cgfd_session {
   } graph[2];
   std::vector<char> rec_changed;  // per shot: receivers replaced since the last rebuild
   void *gather = nullptr;         // device buffer of cgfd_fetch_subvolume
+  void *peak = nullptr;           // peak ground motion maps, nshots x 18 x ni x nj (model type)
   size_t gather_bytes = 0;
   double last_run_ms = 0, last_int_avg_ms = 0;
   int last_int_n = 0;
@@ -795,6 +796,13 @@ int enqueue_steps(cgfd_session *s, const Shell &sh, int it0, int nsteps, TC dt, 
         if (e != hipSuccess) return fail(CGFD_E_HIP, "source kernel: %s", hipGetErrorString(e));
       }
     }
+    if (s->peak && s->free_top) {
+      // W^{n+1} sits in END before the swap (drv_rk_curv_col.cu:519-527)
+      const TS *wn = (const TS *)wav_level(s, 0, END), *wo = (const TS *)wav_level(s, 0, PRE);
+      hipError_t e = launch_peak<TS, TC>(wn, wo, s->shot_elems, s->L, (TC *)s->peak, dt, s->sc, s->shot_active,
+                                         s->d.nshots, s->st);
+      if (e != hipSuccess) return fail(CGFD_E_HIP, "peak motion kernel: %s", hipGetErrorString(e));
+    }
     hipError_t e = launch_clock_advance(s->clock, s->st);
     if (e != hipSuccess) return fail(CGFD_E_HIP, "clock kernel: %s", hipGetErrorString(e));
     std::swap(s->slot[0], s->slot[3]);  // pre <-> end (drv_rk_curv_col.cu:529-541)
@@ -1192,6 +1200,11 @@ int cgfd_reset_shot(cgfd_session *s, int shot) {
   if (!s || shot < 0 || shot >= s->d.nshots) return fail(CGFD_E_ARG, "bad shot");
   int r = fill_zero(s, wav_level(s, shot, 0), (size_t)s->shot_elems, s->es);
   if (r) return r;
+  if (s->peak) {
+    const size_t n = (size_t)18 * s->d.ni * s->d.nj;
+    r = fill_zero(s, (char *)s->peak + (size_t)shot * n * s->ms, n, s->ms);
+    if (r) return r;
+  }
   for (int f = 0; f < 6; f++) {
     FaceHost &F = s->face[f];
     if (!F.on) continue;
@@ -1380,6 +1393,48 @@ int cgfd_get_tile(cgfd_session *s, int *shape, int *tx, int *ty, int *zchunk) {
 int cgfd_set_timing(cgfd_session *s, int on) {
   if (!s) return fail(CGFD_E_ARG, "null session");
   s->timing = on != 0;
+  return CGFD_OK;
+}
+
+int cgfd_set_peak_motion(cgfd_session *s, int on) {
+  if (!s) return fail(CGFD_E_ARG, "null session");
+  invalidate(s);
+  if (!on) {
+    if (s->peak) dfree(s, s->peak);
+    return CGFD_OK;
+  }
+  if (s->peak) return CGFD_OK;
+  const size_t n = (size_t)s->d.nshots * 18 * s->d.ni * s->d.nj;
+  int r = dalloc(s, &s->peak, n * s->ms);
+  if (r) return r;
+  r = fill_zero(s, s->peak, n, s->ms);
+  if (r) return r;
+  HIPCHK(hipStreamSynchronize(s->st));
+  return CGFD_OK;
+}
+
+int cgfd_read_peak_motion(cgfd_session *s, int shot, int dtype, void *out) {
+  if (!s || shot < 0 || shot >= s->d.nshots || !out) return fail(CGFD_E_ARG, "bad read_peak_motion arguments");
+  if (!s->peak) return fail(CGFD_E_STATE, "peak motion is not enabled");
+  const int ni = s->d.ni, nj = s->d.nj, nx = ni + 2 * kNG, ny = nj + 2 * kNG;
+  const size_t n = (size_t)ni * nj;
+  std::vector<double> v(15 * n);
+  if (s->ms == 8) {
+    HIPCHK(d2h(s, v.data(), (double *)s->peak + (size_t)shot * 18 * n, 15 * n * 8));
+  } else {
+    std::vector<float> f(15 * n);
+    HIPCHK(d2h(s, f.data(), (float *)s->peak + (size_t)shot * 18 * n, 15 * n * 4));
+    for (size_t q = 0; q < f.size(); q++) v[q] = f[q];
+  }
+  for (int m = 0; m < 15; m++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        const bool in = i >= kNG && i < kNG + ni && j >= kNG && j < kNG + nj;
+        const double x = in ? v[(size_t)m * n + (size_t)(j - kNG) * ni + (i - kNG)] : 0.0;
+        const size_t q = ((size_t)m * ny + j) * nx + i;
+        if (dtype == CGFD_DT_F64) ((double *)out)[q] = x;
+        else ((float *)out)[q] = (float)x;
+      }
   return CGFD_OK;
 }
 

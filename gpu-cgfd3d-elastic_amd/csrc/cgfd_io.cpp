@@ -279,4 +279,58 @@ int cgfd_snap_close(cgfd_snap *sn) {
   return ok ? CGFD_OK : iofail(CGFD_E_ARG, "snapshot close failed");
 }
 
+int cgfd_peak_motion_write(const char *path, int nx, int ny, const float *pg, const int *first_physical,
+                           const int *count_physical, const int *topo) {
+  if (!path || !pg || nx < 1 || ny < 1 || !first_physical || !count_physical || !topo)
+    return iofail(CGFD_E_ARG, "cgfd_peak_motion_write: bad arguments");
+  static const char *names[15] = {"PGV", "PGVh", "PGVx", "PGVy", "PGVz", "PGA", "PGAh", "PGAx",
+                                  "PGAy", "PGAz", "PGD", "PGDh", "PGDx", "PGDy", "PGDz"};
+  const long long vsize = 4LL * nx * ny;
+  auto header = [&](uint32_t begin0) {
+    BE h;
+    h.b.insert(h.b.end(), {'C', 'D', 'F', 1});
+    h.u32(0);
+    h.u32(NC_DIMENSION);
+    h.u32(2);
+    h.name("j"); h.u32((uint32_t)ny);
+    h.name("i"); h.u32((uint32_t)nx);
+    h.u32(NC_ATTRIBUTE);
+    h.u32(3);
+    auto att = [&](const char *nm, const int *v, int n) {
+      h.name(nm);
+      h.u32(NC_INT);
+      h.u32((uint32_t)n);
+      for (int q = 0; q < n; q++) h.u32((uint32_t)v[q]);
+    };
+    att("global_index_of_first_physical_points", first_physical, 2);
+    att("count_index_of_physical_points", count_physical, 2);
+    att("coords_of_mpi_topo", topo, 3);
+    h.u32(NC_VARIABLE);
+    h.u32(15);
+    uint32_t off = begin0;
+    for (int v = 0; v < 15; v++) {
+      h.name(names[v]);
+      h.u32(2); h.u32(0); h.u32(1);
+      h.u32(0); h.u32(0);
+      h.u32(NC_FLOAT);
+      h.u32((uint32_t)vsize);
+      h.u32(off);
+      off += (uint32_t)vsize;
+    }
+    return h.b;
+  };
+  const std::vector<unsigned char> h0 = header(0);
+  if ((long long)h0.size() + 15 * vsize > 0x7fffffffLL) return iofail(CGFD_E_ARG, "peak motion maps too large");
+  const std::vector<unsigned char> h1 = header((uint32_t)h0.size());
+  BE d;
+  d.b.reserve((size_t)(15 * vsize));
+  for (long long q = 0; q < 15LL * nx * ny; q++) d.f32(pg[q]);
+  FILE *fp = std::fopen(path, "wb");
+  if (!fp) return iofail(CGFD_E_ARG, std::string("cannot open ") + path + ": " + std::strerror(errno));
+  bool ok = std::fwrite(h1.data(), 1, h1.size(), fp) == h1.size() &&
+            std::fwrite(d.b.data(), 1, d.b.size(), fp) == d.b.size();
+  ok = (std::fclose(fp) == 0) && ok;
+  return ok ? CGFD_OK : iofail(CGFD_E_ARG, std::string("write failed: ") + path);
+}
+
 }  // extern "C"

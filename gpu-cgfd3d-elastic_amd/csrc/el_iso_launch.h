@@ -17,6 +17,11 @@ template <typename TS, typename TC> hipError_t launch_recv(const RecvArgs<TS, TC
 
 hipError_t launch_clock_advance(int *clock, hipStream_t st);
 
+// peak ground motion maps of every shot after one step (free surface only)
+template <typename TS, typename TC>
+hipError_t launch_peak(const TS *w_new, const TS *w_old, long shot_stride, const Layout &L, TC *pg, TC dt,
+                       const float *sc, const int *shot_active, int nshots, hipStream_t st);
+
 // one component's points (i1 + n*di, j1 + m*dj, k1 + l*dk), physical indices,
 // into out [nk][nj][ni], multiplied by scale
 template <typename TS, typename TC>

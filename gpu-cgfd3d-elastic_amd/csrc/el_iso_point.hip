@@ -386,6 +386,43 @@ template <typename T> __global__ __launch_bounds__(256) void k_fill(T *p, long n
   for (; i < n; i += stride) p[i] = v;
 }
 
+// ---- peak ground motion on the free surface (PG_calcu_gpu, src/forward/wav_t.cu:281-360) ----
+// After each step, from W^{n+1} (w_new) and W^n (w_old) on the top plane:
+// running maxima of |V|, |V_h|, |V_x|, |V_y|, |V_z|, the same for the
+// acceleration |dV|/dt and for the displacement integrated with the trapezoid
+// rule (accumulated in double, as the reference's 0.5 literal promotes it).
+// Maps per shot: [15 maps + 3 displacements][nj][ni].
+template <typename TS, typename TC>
+__global__ __launch_bounds__(256) void k_peak(const TS *w_new, const TS *w_old, long shot_stride, Layout L, TC *pg,
+                                              TC dt, float sx, float sy, float sz, const int *shot_active) {
+  const int shot = blockIdx.y;
+  if (shot_active && !shot_active[shot]) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = L.ni * L.nj;
+  if (t >= n) return;
+  const int i = t % L.ni, j = t / L.ni;
+  const long o = L.off(i, j, L.nk - 1) + (long)shot * shot_stride;
+  const long cs = L.cs;
+  const TC vx1 = (TC)St<TS>::ld(w_new + o) * (TC)sx, vy1 = (TC)St<TS>::ld(w_new + cs + o) * (TC)sy;
+  const TC vz1 = (TC)St<TS>::ld(w_new + 2 * cs + o) * (TC)sz;
+  const TC vx0 = (TC)St<TS>::ld(w_old + o) * (TC)sx, vy0 = (TC)St<TS>::ld(w_old + cs + o) * (TC)sy;
+  const TC vz0 = (TC)St<TS>::ld(w_old + 2 * cs + o) * (TC)sz;
+  TC *m = pg + (long)shot * 18 * n + t;
+  const TC Ax = fabs((vx1 - vx0) / dt), Ay = fabs((vy1 - vy0) / dt), Az = fabs((vz1 - vz0) / dt);
+  TC Dx = m[15 * n], Dy = m[16 * n], Dz = m[17 * n];
+  Dx = (TC)((double)Dx + 0.5 * (double)(vx1 + vx0) * (double)dt);
+  Dy = (TC)((double)Dy + 0.5 * (double)(vy1 + vy0) * (double)dt);
+  Dz = (TC)((double)Dz + 0.5 * (double)(vz1 + vz0) * (double)dt);
+  m[15 * n] = Dx; m[16 * n] = Dy; m[17 * n] = Dz;
+  const TC V = sqrt(vx1 * vx1 + vy1 * vy1 + vz1 * vz1), Vh = sqrt(vx1 * vx1 + vy1 * vy1);
+  const TC A = sqrt(Ax * Ax + Ay * Ay + Az * Az), Ah = sqrt(Ax * Ax + Ay * Ay);
+  const TC D = sqrt(Dx * Dx + Dy * Dy + Dz * Dz), Dh = sqrt(Dx * Dx + Dy * Dy);
+  const TC val[15] = {V, Vh, fabs(vx1), fabs(vy1), fabs(vz1), A, Ah, Ax, Ay, Az, D, Dh, fabs(Dx), fabs(Dy), fabs(Dz)};
+#pragma unroll
+  for (int q = 0; q < 15; q++)
+    if (m[q * n] < val[q]) m[q * n] = val[q];
+}
+
 // ---- strided sub-volume gather (io_snap_pack_buff, io_funcs.cu:1568-1596) ----
 template <typename TS, typename TC>
 __global__ __launch_bounds__(256) void k_gather(const TS *w, Layout L, int i1, int j1, int k1, int ni, int nj, int nk,
@@ -400,6 +437,15 @@ __global__ __launch_bounds__(256) void k_gather(const TS *w, Layout L, int i1, i
 }
 
 // ---- launchers ------------------------------------------------------------------
+template <typename TS, typename TC>
+hipError_t launch_peak(const TS *w_new, const TS *w_old, long shot_stride, const Layout &L, TC *pg, TC dt,
+                       const float *sc, const int *shot_active, int nshots, hipStream_t st) {
+  const int n = L.ni * L.nj;
+  hipLaunchKernelGGL((k_peak<TS, TC>), dim3((unsigned)((n + 255) / 256), (unsigned)nshots), dim3(256), 0, st, w_new,
+                     w_old, shot_stride, L, pg, dt, sc[0], sc[1], sc[2], shot_active);
+  return hipGetLastError();
+}
+
 template <typename TS, typename TC>
 hipError_t launch_gather(const TS *w, const Layout &L, int i1, int j1, int k1, int ni, int nj, int nk, int di,
                          int dj, int dk, TC scale, TC *out, hipStream_t st) {
@@ -485,6 +531,8 @@ template <typename T> hipError_t launch_fill(T *p, long n, T v, hipStream_t st) 
 #define CGFD_INST(TS, TC)                                                                                   \
   template hipError_t launch_point<TS, TC>(const StageArgs<TS, TC> &, int, int, int, int, hipStream_t);    \
   template hipError_t launch_slab<TS, TC>(const StageArgs<TS, TC> &, int, int, int, int, int, hipStream_t); \
+  template hipError_t launch_peak<TS, TC>(const TS *, const TS *, long, const Layout &, TC *, TC, const float *, \
+                                          const int *, int, hipStream_t);                                      \
   template hipError_t launch_gather<TS, TC>(const TS *, const Layout &, int, int, int, int, int, int, int, int,   \
                                             int, TC, TC *, hipStream_t);                                        \
   template hipError_t launch_source<TS, TC>(const SrcArgs<TS, TC> &, hipStream_t);                          \

@@ -194,6 +194,18 @@ int cgfd_get_tile(cgfd_session *s, int *shape, int *tx, int *ty, int *zchunk);
  * launch.  Timed runs are enqueued launch by launch, without graphs. */
 int cgfd_set_timing(cgfd_session *s, int on);
 
+/* Peak ground motion on a free surface (PG_calcu_gpu,
+ * src/forward/wav_t.cu:281-360, called after every step at
+ * drv_rk_curv_col.cu:519-527): when enabled (and the top is free), every step
+ * updates per shot the running maxima PGV, PGVh, PGVx, PGVy, PGVz, PGA, PGAh,
+ * PGAx, PGAy, PGAz, PGD, PGDh, PGDx, PGDy, PGDz on the top plane, the
+ * acceleration being |V^{n+1}-V^n|/dt and the displacement the trapezoid sum
+ * of V.  Maps start at zero on enabling and on cgfd_reset_shot.
+ * cgfd_read_peak_motion returns them as [15][ny][nx] with ghosts (zero), the
+ * layout the reference writes (PG_slice_output, io_funcs.cu:2065-2125). */
+int cgfd_set_peak_motion(cgfd_session *s, int on);
+int cgfd_read_peak_motion(cgfd_session *s, int shot, int dtype, void *out);
+
 /* Run whole 8-step periods (starting at a step that is a multiple of 8) as
  * one captured hipGraph each (default on); no reference counterpart, the
  * reference launches and synchronises per kernel. */

@@ -57,6 +57,15 @@ int cgfd_snap_append(cgfd_snap *sn, float time, const float *w, const float *lam
 /* Writes the record count and closes the file. */
 int cgfd_snap_close(cgfd_snap *sn);
 
+/* Peak ground motion file PG_V_A_D_{coords}.nc (PG_slice_output,
+ * io_funcs.cu:2065-2125): dimensions j = ny, i = nx (with ghosts), float
+ * variables PGV PGVh PGVx PGVy PGVz PGA PGAh PGAx PGAy PGAz PGD PGDh PGDx PGDy
+ * PGDz from pg [15][ny][nx], global int attributes
+ * global_index_of_first_physical_points (2), count_index_of_physical_points
+ * (2), coords_of_mpi_topo (3). */
+int cgfd_peak_motion_write(const char *path, int nx, int ny, const float *pg, const int *first_physical,
+                           const int *count_physical, const int *topo);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,3 +106,18 @@ def test_snapshot_velocity_only(tmp_path):
     assert set(f.variables) == {"time", "Vx", "Vy", "Vz"}
     assert np.array_equal(f.variables["Vz"][0], w[2])
     f.close()
+
+
+def test_peak_motion_file(tmp_path):
+    rng = np.random.default_rng(2)
+    pg = rng.random((15, 9, 11)).astype(np.float32)
+    p = tmp_path / "PG_V_A_D_px0_py0.nc"
+    cgfd.peak_motion_write(p, pg, first_physical=(3, 3), count_physical=(5, 3), topo=(0, 0, 0))
+    f = netcdf_file(str(p), "r", mmap=False)
+    assert f.dimensions["j"] == 9 and f.dimensions["i"] == 11
+    assert list(f._attributes["global_index_of_first_physical_points"]) == [3, 3]
+    assert list(f._attributes["count_index_of_physical_points"]) == [5, 3]
+    for q, n in enumerate(cgfd.PEAK_NAMES):
+        assert f.variables[n].dimensions == ("j", "i")
+        assert np.array_equal(f.variables[n][:], pg[q])
+    f.close()
