@@ -186,10 +186,16 @@ def main():
         n = args.copy_calib * 1024 * 1024 // 4
         x = torch.ones(n, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        y.copy_(x)
+        e0.record()
         for _ in range(5):
             y.copy_(x)
+        e1.record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calib_bytes_each_way": n * 4, "copies": 5}), flush=True)
+        ms = e0.elapsed_time(e1) / 5
+        print(json.dumps({"copy_calib_bytes_each_way": n * 4, "copies": 6, "ms_per_copy": round(ms, 4),
+                          "copy_GBps_read_plus_write": round(2 * n * 4 / ms / 1e6, 1)}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "

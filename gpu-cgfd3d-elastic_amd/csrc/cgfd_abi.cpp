@@ -552,11 +552,11 @@ StageArgs<TS, TC> stage_args(cgfd_session *s, int slot_in, int slot_pre, int slo
 }
 
 // Work split of one stage: the z-marching interior box, then the boundary
-// shell cut into boxes by PML membership: boxes inside exactly one face (away
-// from a free surface) go to the slab kernel of that face's axis, the rest
-// (edges, corners, the 4 free-surface rows) to the general per-point kernel,
+// shell (PML face slabs and the 4 free-surface rows) in one per-point launch,
 // which also covers everything for tiny grids or the plain method.  With
 // CGFD_FLAG_PML_MARCH the face slabs are swept by the z-marching PML variant.
+// (Splitting the shell into per-axis single-face launches measured slower:
+// the x slabs are only nlay+1 points wide, and four launches add four tails.)
 struct BoxList {
   int n = 0;
   int box[kMaxBox][6];
@@ -574,7 +574,6 @@ struct Shell {
   MarchSet inner;
   MarchSet pml;
   BoxList point;
-  BoxList slab[3];
 };
 
 Shell make_shell(cgfd_session *s) {
@@ -612,24 +611,12 @@ Shell make_shell(cgfd_session *s) {
     sh.point.add(0, ni - 1, 0, nj - 1, ktopm + 1, nk - 1);
     return sh;
   }
-  // per axis: [lower face | middle | upper face or free-surface rows]
-  const int ia[3][2] = {{0, ib0 - 1}, {ib0, ib1}, {ib1 + 1, ni - 1}};
-  const int ja[3][2] = {{0, jb0 - 1}, {jb0, jb1}, {jb1 + 1, nj - 1}};
-  const int ka[3][2] = {{0, kb0 - 1}, {kb0, kb1}, {kb1 + 1, nk - 1}};
-  for (int ck = 0; ck < 3; ck++)
-    for (int cj = 0; cj < 3; cj++)
-      for (int ci = 0; ci < 3; ci++) {
-        if (ci == 1 && cj == 1 && ck == 1) continue;
-        const int *I = ia[ci], *J = ja[cj], *K = ka[ck];
-        const bool free_rows = ck == 2 && s->free_top;
-        const int nd = (ci != 1) + (cj != 1) + (ck != 1);
-        if (free_rows || nd > 1) {
-          sh.point.add(I[0], I[1], J[0], J[1], K[0], K[1]);
-        } else {
-          const int fd = ci != 1 ? 0 : cj != 1 ? 1 : 2;
-          sh.slab[fd].add(I[0], I[1], J[0], J[1], K[0], K[1]);
-        }
-      }
+  sh.point.add(0, ib0 - 1, 0, nj - 1, 0, nk - 1);
+  sh.point.add(ib1 + 1, ni - 1, 0, nj - 1, 0, nk - 1);
+  sh.point.add(ib0, ib1, 0, jb0 - 1, 0, nk - 1);
+  sh.point.add(ib0, ib1, jb1 + 1, nj - 1, 0, nk - 1);
+  sh.point.add(ib0, ib1, jb0, jb1, 0, kb0 - 1);
+  sh.point.add(ib0, ib1, jb0, jb1, kb1 + 1, nk - 1);
   return sh;
 }
 
@@ -655,12 +642,6 @@ int launch_stage(cgfd_session *s, const Shell &sh, StageArgs<TS, TC> &a, int dx,
   if (sh.pml.nbox) {
     hipError_t e = launch_pml_march<TS, TC>(a, sh.pml, dx, dy, dz, nshots, s->st);
     if (e != hipSuccess) return fail(CGFD_E_HIP, "PML sweep kernel: %s", hipGetErrorString(e));
-  }
-  for (int fd = 0; fd < 3; fd++) {
-    if (!sh.slab[fd].n) continue;
-    set_boxes(a, sh.slab[fd]);
-    hipError_t e = launch_slab<TS, TC>(a, fd, dx, dy, dz, nshots, s->st);
-    if (e != hipSuccess) return fail(CGFD_E_HIP, "PML slab kernel: %s", hipGetErrorString(e));
   }
   if (sh.point.n) {
     set_boxes(a, sh.point);

@@ -18,26 +18,53 @@ enum Mode { MODE_RHS = 0, MODE_S0 = 1, MODE_S12 = 2, MODE_S3 = 3 };
 
 // ---- storage traits ---------------------------------------------------------
 template <typename TS> struct St;
+// ld/st: ordinary accesses; ld_nt/st_nt: non-temporal (streamed once, kept
+// out of the way of the wavefield planes that the stencil re-reads from L2)
 template <> struct St<float> {
   static constexpr bool scaled = false;
   __device__ __forceinline__ static float ld(const float *p) { return *p; }
   __device__ __forceinline__ static void st(float *p, float v) { *p = v; }
+  __device__ __forceinline__ static float ld_nt(const float *p) { return __builtin_nontemporal_load(p); }
+  __device__ __forceinline__ static void st_nt(float *p, float v) { __builtin_nontemporal_store(v, p); }
 };
 template <> struct St<double> {
   static constexpr bool scaled = false;
   __device__ __forceinline__ static double ld(const double *p) { return *p; }
   __device__ __forceinline__ static void st(double *p, double v) { *p = v; }
+  __device__ __forceinline__ static double ld_nt(const double *p) { return __builtin_nontemporal_load(p); }
+  __device__ __forceinline__ static void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
 };
 template <> struct St<__half> {
   static constexpr bool scaled = true;
   __device__ __forceinline__ static float ld(const __half *p) { return __half2float(*p); }
   __device__ __forceinline__ static void st(__half *p, float v) { *p = __float2half(v); }
+  __device__ __forceinline__ static float ld_nt(const __half *p) {
+    const unsigned short u = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+    return __half2float(__ushort_as_half(u));
+  }
+  __device__ __forceinline__ static void st_nt(__half *p, float v) {
+    __builtin_nontemporal_store(__half_as_ushort(__float2half(v)), reinterpret_cast<unsigned short *>(p));
+  }
 };
 template <> struct St<__hip_bfloat16> {
   static constexpr bool scaled = true;
   __device__ __forceinline__ static float ld(const __hip_bfloat16 *p) { return __bfloat162float(*p); }
   __device__ __forceinline__ static void st(__hip_bfloat16 *p, float v) { *p = __float2bfloat16(v); }
+  __device__ __forceinline__ static float ld_nt(const __hip_bfloat16 *p) {
+    // bfloat16 is the upper half of a float
+    const unsigned short u = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+    return __uint_as_float((unsigned)u << 16);
+  }
+  __device__ __forceinline__ static void st_nt(__hip_bfloat16 *p, float v) {
+    const __hip_bfloat16 b = __float2bfloat16(v);
+    unsigned short u;
+    __builtin_memcpy(&u, &b, 2);
+    __builtin_nontemporal_store(u, reinterpret_cast<unsigned short *>(p));
+  }
 };
+
+// non-temporal load of a model coefficient
+template <typename T> __device__ __forceinline__ T ld_nt(const T &r) { return __builtin_nontemporal_load(&r); }
 
 // Load/store through a uniform base pointer and a 32-bit byte offset: the
 // compiler then addresses with a scalar base plus one 32-bit VGPR offset

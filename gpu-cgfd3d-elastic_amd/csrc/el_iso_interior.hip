@@ -144,9 +144,9 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
     for (int p = 0; p < NPER; p++) n_hv[p] = gld(h_c[p], pk + h_off[p]);
     const unsigned ob = o * (unsigned)sizeof(TC);
 #pragma unroll
-    for (int m = 0; m < 9; m++) n_met[m] = CART ? a.cmet[m] : at_b(a.met + (XIX + m) * cs, ob);
+    for (int m = 0; m < 9; m++) n_met[m] = CART ? a.cmet[m] : (at_b(a.met + (XIX + m) * cs, ob));
 #pragma unroll
-    for (int m = 0; m < 3; m++) n_met[9 + m] = at_b(a.med + m * cs, ob);
+    for (int m = 0; m < 3; m++) n_met[9 + m] = (at_b(a.med + m * cs, ob));
     const unsigned qb = o * (unsigned)sizeof(TS);
     if (need_end) {
 #pragma unroll
@@ -356,12 +356,12 @@ __device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf
       for (int p = 0; p < HPER; p++) n_hv[c][p] = gld(a.w_in, COFF + c, pk + h_off[p]);
     const unsigned ob = o * (unsigned)sizeof(TC);
 #pragma unroll
-    for (int m = 0; m < 9; m++) n_met[m] = CART ? a.cmet[m] : at_b(a.met + (XIX + m) * cs, ob);
+    for (int m = 0; m < 9; m++) n_met[m] = CART ? a.cmet[m] : (at_b(a.met + (XIX + m) * cs, ob));
     if (ROLE == ROLE_V) {
-      n_met[9] = at_b(a.med + 2 * cs, ob);
+      n_met[9] = (at_b(a.med + 2 * cs, ob));
     } else {
-      n_met[9] = at_b(a.med, ob);
-      n_met[10] = at_b(a.med + cs, ob);
+      n_met[9] = (at_b(a.med, ob));
+      n_met[10] = (at_b(a.med + cs, ob));
     }
   };
   issue(kbeg);
@@ -383,8 +383,8 @@ __device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf
     issue(k < kend ? k + 1 : kend);
 #pragma unroll
     for (int c = 0; c < NOUT; c++) {
-      pre_v[c] = need_pre ? gld(a.w_pre, OOFF + c, o) : TC(0);
-      end_v[c] = need_end ? gld(a.w_end, OOFF + c, o) : TC(0);
+      pre_v[c] = need_pre ? (TC)St<TS>::ld(&at_b(a.w_pre + so + (OOFF + c) * cs, qb)) * sc(OOFF + c) : TC(0);
+      end_v[c] = need_end ? (TC)St<TS>::ld(&at_b(a.w_end + so + (OOFF + c) * cs, qb)) * sc(OOFF + c) : TC(0);
     }
     const TC xix = mt[0], xiy = mt[1], xiz = mt[2], etx = mt[3], ety = mt[4], etz = mt[5];
     const TC ztx = mt[6], zty = mt[7], ztz = mt[8];

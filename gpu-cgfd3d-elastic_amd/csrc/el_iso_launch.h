@@ -7,10 +7,6 @@ namespace cgfd {
 template <typename TS, typename TC>
 hipError_t launch_point(const StageArgs<TS, TC> &a, int dx, int dy, int dz, int nshots, hipStream_t st);
 
-// single-face PML slab points of axis fd (0 x, 1 y, 2 z), boxes in a.box
-template <typename TS, typename TC>
-hipError_t launch_slab(const StageArgs<TS, TC> &a, int fd, int dx, int dy, int dz, int nshots, hipStream_t st);
-
 template <typename TS, typename TC> hipError_t launch_source(const SrcArgs<TS, TC> &a, hipStream_t st);
 
 template <typename TS, typename TC> hipError_t launch_recv(const RecvArgs<TS, TC> &a, hipStream_t st);

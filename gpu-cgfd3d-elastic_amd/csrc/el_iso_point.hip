@@ -210,86 +210,6 @@ __global__ __launch_bounds__(256) void k_point(StageArgs<TS, TC> a) {
   point_update<DX, DY, DZ, TS, TC>(a, shot, i, j, k);
 }
 
-// ---- single-face PML slabs ----------------------------------------------------------
-// Points inside exactly one PML face box of axis FD and away from a free
-// surface: the inner right-hand side plus that face's ADE terms.  Only the
-// derivatives along FD outlive the inner sum, which keeps the register count
-// (and so the waves in flight) well above the general per-point kernel's.
-template <int DX, int DY, int DZ, typename TS, typename TC, int FD>
-__global__ __launch_bounds__(256) void k_slab(StageArgs<TS, TC> a) {
-  const int shot = blockIdx.y;
-  if (a.shot_active && !a.shot_active[shot]) return;
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int i, j, k;
-  if (!box_point(a, gid, i, j, k)) return;
-  const Layout &L = a.L;
-  const int o = (int)L.off(i, j, k);
-  const long so = (long)shot * a.shot_stride;
-  const TS *win = a.w_in + so;
-  const long cs = L.cs;
-  auto W = [&](int c, long d) -> TC {
-    return (TC)St<TS>::ld(&at_b(win + c * cs, (unsigned)(o + (int)d) * (unsigned)sizeof(TS)));
-  };
-  auto S = [&](int c) -> TC { return St<TS>::scaled ? (TC)a.sc[c] : TC(1); };
-  const unsigned ob = (unsigned)o * (unsigned)sizeof(TC);
-  auto Mt = [&](int m) -> TC { return at_b(a.met + m * cs, ob); };
-  const TC xix = Mt(XIX), xiy = Mt(XIY), xiz = Mt(XIZ);
-  const TC etx = Mt(ETX), ety = Mt(ETY), etz = Mt(ETZ);
-  const TC ztx = Mt(ZTX), zty = Mt(ZTY), ztz = Mt(ZTZ);
-  const TC lam = at_b(a.med, ob), mu = at_b(a.med + cs, ob), slw = at_b(a.med + 2 * cs, ob);
-  const TC lam2mu = lam + TC(2) * mu;
-  TC Dd[kNC];
-  // velocity gradient (Lg[a][b] = dV_a/dx_b) and stress divergence, folded
-  // component by component as in the z-marching kernel
-  TC Lg[3][3];
-  TC dv[3] = {TC(0), TC(0), TC(0)};
-#pragma unroll
-  for (int c = 0; c < kNC; c++) {
-    auto f = [&](long d) { return W(c, d); };
-    TC dx = drp5<DX, TC>(f, 1), dy = drp5<DY, TC>(f, L.px), dz = drp5<DZ, TC>(f, L.pz);
-    if (St<TS>::scaled) { dx *= S(c); dy *= S(c); dz *= S(c); }
-    Dd[c] = FD == 0 ? dx : FD == 1 ? dy : dz;
-    const TC g0 = xix * dx + etx * dy + ztx * dz;
-    const TC g1 = xiy * dx + ety * dy + zty * dz;
-    const TC g2 = xiz * dx + etz * dy + ztz * dz;
-    if (c < 3) {
-      Lg[c][0] = g0; Lg[c][1] = g1; Lg[c][2] = g2;
-    } else if (c == TXX) {
-      dv[0] += g0;
-    } else if (c == TYY) {
-      dv[1] += g1;
-    } else if (c == TZZ) {
-      dv[2] += g2;
-    } else if (c == TYZ) {
-      dv[1] += g2; dv[2] += g1;
-    } else if (c == TXZ) {
-      dv[0] += g2; dv[2] += g0;
-    } else {
-      dv[0] += g1; dv[1] += g0;
-    }
-  }
-  TC h[kNC];
-  h[VX] = slw * dv[0];
-  h[VY] = slw * dv[1];
-  h[VZ] = slw * dv[2];
-  h[TXX] = lam2mu * Lg[0][0] + lam * (Lg[1][1] + Lg[2][2]);
-  h[TYY] = lam2mu * Lg[1][1] + lam * (Lg[0][0] + Lg[2][2]);
-  h[TZZ] = lam2mu * Lg[2][2] + lam * (Lg[0][0] + Lg[1][1]);
-  h[TXY] = mu * (Lg[0][1] + Lg[1][0]);
-  h[TXZ] = mu * (Lg[0][2] + Lg[2][0]);
-  h[TYZ] = mu * (Lg[1][2] + Lg[2][1]);
-  // the slab lies in the lower or the upper face of its axis
-  const FaceArgs<TS, TC> &Flo = a.f[2 * FD];
-  const int pos = FD == 0 ? i : FD == 1 ? j : k;
-  const int hi_lo = FD == 0 ? Flo.i2 : FD == 1 ? Flo.j2 : Flo.k2;
-  const int fi = (Flo.on && pos <= hi_lo) ? 2 * FD : 2 * FD + 1;
-  const TC e1 = FD == 0 ? xix : FD == 1 ? etx : ztx;
-  const TC e2 = FD == 0 ? xiy : FD == 1 ? ety : zty;
-  const TC e3 = FD == 0 ? xiz : FD == 1 ? etz : ztz;
-  pml_face<TS, TC>(a, fi, shot, i, j, k, Dd, e1, e2, e3, ztx, zty, ztz, lam, mu, lam2mu, slw, h);
-  rk_store<TS, TC>(a, so, o, h);
-}
-
 // ---- source injection (sv_curv_col_el_iso_rhs_src_gpu, sv_curv_col_el_gpu.cu:227-301) ----
 // One thread per distinct grid point; contributions of the shot's sources are
 // summed in source order and applied to the RK outputs of this stage.
@@ -475,32 +395,6 @@ hipError_t launch_point(const StageArgs<TS, TC> &a, int dx, int dy, int dz, int 
   return hipGetLastError();
 }
 
-template <typename TS, typename TC, int FD>
-static void launch_slab_fd(const StageArgs<TS, TC> &a, int sel, dim3 grid, dim3 block, hipStream_t st) {
-  switch (sel) {
-    case 0: hipLaunchKernelGGL((k_slab<0, 0, 0, TS, TC, FD>), grid, block, 0, st, a); break;
-    case 1: hipLaunchKernelGGL((k_slab<0, 0, 1, TS, TC, FD>), grid, block, 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_slab<0, 1, 0, TS, TC, FD>), grid, block, 0, st, a); break;
-    case 3: hipLaunchKernelGGL((k_slab<0, 1, 1, TS, TC, FD>), grid, block, 0, st, a); break;
-    case 4: hipLaunchKernelGGL((k_slab<1, 0, 0, TS, TC, FD>), grid, block, 0, st, a); break;
-    case 5: hipLaunchKernelGGL((k_slab<1, 0, 1, TS, TC, FD>), grid, block, 0, st, a); break;
-    case 6: hipLaunchKernelGGL((k_slab<1, 1, 0, TS, TC, FD>), grid, block, 0, st, a); break;
-    default: hipLaunchKernelGGL((k_slab<1, 1, 1, TS, TC, FD>), grid, block, 0, st, a); break;
-  }
-}
-
-template <typename TS, typename TC>
-hipError_t launch_slab(const StageArgs<TS, TC> &a, int fd, int dx, int dy, int dz, int nshots, hipStream_t st) {
-  const long n = a.nbox ? a.box_end[a.nbox - 1] : 0;
-  if (n == 0) return hipSuccess;
-  dim3 grid((unsigned)((n + 255) / 256), (unsigned)nshots), block(256);
-  const int sel = dx * 4 + dy * 2 + dz;
-  if (fd == 0) launch_slab_fd<TS, TC, 0>(a, sel, grid, block, st);
-  else if (fd == 1) launch_slab_fd<TS, TC, 1>(a, sel, grid, block, st);
-  else launch_slab_fd<TS, TC, 2>(a, sel, grid, block, st);
-  return hipGetLastError();
-}
-
 template <typename TS, typename TC> hipError_t launch_source(const SrcArgs<TS, TC> &a, hipStream_t st) {
   if (a.npt == 0) return hipSuccess;
   hipLaunchKernelGGL((k_source<TS, TC>), dim3((a.npt + 255) / 256), dim3(256), 0, st, a);
@@ -530,7 +424,6 @@ template <typename T> hipError_t launch_fill(T *p, long n, T v, hipStream_t st) 
 
 #define CGFD_INST(TS, TC)                                                                                   \
   template hipError_t launch_point<TS, TC>(const StageArgs<TS, TC> &, int, int, int, int, hipStream_t);    \
-  template hipError_t launch_slab<TS, TC>(const StageArgs<TS, TC> &, int, int, int, int, int, hipStream_t); \
   template hipError_t launch_peak<TS, TC>(const TS *, const TS *, long, const Layout &, TC *, TC, const float *, \
                                           const int *, int, hipStream_t);                                      \
   template hipError_t launch_gather<TS, TC>(const TS *, const Layout &, int, int, int, int, int, int, int, int,   \
