@@ -112,7 +112,7 @@ def serving_sample(n, nt_range, load, seed, nseq):
     # served: FP32 and FP16 models resident, continuous batching
     ex = np.max([cgfd.storage_scales(cgfd.STORE_F16, ni, nj, nk, m["metric"], m["lam"], m["mu"], m["slw"],
                                      r["src"], dt) for r, h in zip(reqs, fp16) if h] or [[0] * 9], axis=0)
-    srv = cgfd.Server(max_batch=16, chunk_steps=16)
+    srv = cgfd.Server(max_batch=32, chunk_steps=16)
     nt_max = nt_range[1]
     m32 = srv.add_model(ni, nj, nk, dt=dt, storage=cgfd.STORE_F32, max_nrecv=nrec, max_nt=nt_max, **arrs)
     m16 = srv.add_model(ni, nj, nk, dt=dt, storage=cgfd.STORE_F16, scale_exp2=ex, max_nrecv=nrec, max_nt=nt_max,
@@ -142,6 +142,9 @@ def serving_sample(n, nt_range, load, seed, nseq):
     end = time.perf_counter()
     stats = srv.stats()
     srv.close()
+    # throughput of the stream: completions over the span from the first
+    # submission to the last completion (arrivals outpace the server, so the
+    # span measures its capacity, tail included)
     served_rate = n / (end - start)
     lat = np.array(lat)
     return dict(
@@ -174,9 +177,9 @@ def main():
                     help="steps of the per-launch event timing pass (0: skip)")
     ap.add_argument("--reduced-steps", type=int, default=200,
                     help="timed steps of the FP16/BF16-storage runs of the same workload (0: skip)")
-    ap.add_argument("--serve-shots", type=int, default=32, help="served-stream sample size (0: skip)")
+    ap.add_argument("--serve-shots", type=int, default=128, help="served-stream sample size (0: skip)")
     ap.add_argument("--serve-steps", default="120,480", help="step range of the served requests")
-    ap.add_argument("--serve-load", type=float, default=4.0, help="arrival rate / sequential plain rate")
+    ap.add_argument("--serve-load", type=float, default=8.0, help="arrival rate / sequential plain rate")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()

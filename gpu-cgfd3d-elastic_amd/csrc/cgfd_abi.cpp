@@ -651,10 +651,14 @@ int launch_stage(cgfd_session *s, const Shell &sh, StageArgs<TS, TC> &a, int dx,
   return CGFD_OK;
 }
 
-// Tile autotuner: time one stage-0 interior sweep per (shape, z chunk) and
-// keep the fastest.  Stage 0 only writes the TMPA and END levels, which every
-// step rewrites, so tuning between steps leaves the state intact.  Results
-// are cached per (grid, storage, shots) for the life of the process.
+// Tile autotuner: per (shape, z chunk), time the interior sweeps of one full
+// step's stage sequence (S0, S12, S12, S3 with the operator directions of
+// pair 0) and keep the fastest; the stages differ in traffic (39 to 57 words
+// per cell), so tuning on one mode alone can pick a tile that loses on the
+// others.  The RK coefficients are zero, and every level the sweeps write
+// (TMPA, TMPB, END) is rewritten by the next step before it is read, so
+// tuning between steps leaves the state intact.  Results are cached per
+// (grid, storage, shots) for the life of the process.
 struct TileKey {
   int ni, nj, nk, storage, nshots;
   bool operator<(const TileKey &o) const {
@@ -677,19 +681,25 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
     g_tile_cache[key] = {s->tile_shape, s->tile_zchunk};
     return CGFD_OK;
   }
-  StageArgs<TS, TC> a = stage_args<TS, TC>(s, s->slot[0], s->slot[0], s->slot[1], s->slot[3], false);
-  a.mode = MODE_S0;
-  a.ca = TC(0);
-  a.cb = TC(0);
+  const int PRE = s->slot[0], TA = s->slot[1], TB = s->slot[2], END = s->slot[3];
+  const int sin[4] = {PRE, TA, TB, TA}, sout[4] = {TA, TB, TA, TB};
+  const int smode[4] = {MODE_S0, MODE_S12, MODE_S12, MODE_S3};
+  StageArgs<TS, TC> sa[4];
+  for (int q = 0; q < 4; q++) {
+    sa[q] = stage_args<TS, TC>(s, sin[q], PRE, sout[q], END, false);
+    sa[q].mode = smode[q];
+    sa[q].ca = TC(0);
+    sa[q].cb = TC(0);
+  }
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   const int nz = sh.inner.box[0][5] - sh.inner.box[0][4] + 1;
-  const int zcands[3] = {24, 48, 96};
+  const int zcands[4] = {16, 24, 32, 48};
   float best = 1e30f;
-  int bs = 0, bz = 48;
+  int bs = 0, bz = 24;
   for (int shape = 0; shape < interior_num_shapes(); shape++) {
-    for (int zi = 0; zi < 3; zi++) {
+    for (int zi = 0; zi < 4; zi++) {
       int zc = std::min(zcands[zi], nz);
       if (zi > 0 && zc == std::min(zcands[zi - 1], nz)) continue;
       MarchSet ib = sh.inner;
@@ -698,8 +708,11 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
       float tmin = 1e30f;
       for (int rep = 0; rep < 3; rep++) {
         HIPCHK(hipEventRecord(e0, s->st));
-        hipError_t e = launch_interior<TS, TC>(a, ib, 0, 0, 0, s->d.nshots, s->st);
-        if (e != hipSuccess) return fail(CGFD_E_HIP, "autotune launch: %s", hipGetErrorString(e));
+        for (int q = 0; q < 4; q++) {
+          const int d = (kFlags[0][0] + q) % 2;
+          hipError_t e = launch_interior<TS, TC>(sa[q], ib, d, d, d, s->d.nshots, s->st);
+          if (e != hipSuccess) return fail(CGFD_E_HIP, "autotune launch: %s", hipGetErrorString(e));
+        }
         HIPCHK(hipEventRecord(e1, s->st));
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0;

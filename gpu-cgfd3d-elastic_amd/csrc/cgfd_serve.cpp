@@ -317,11 +317,19 @@ void cgfd_server::run() {
       std::lock_guard<std::mutex> lk(mu);
       for (long long t : admitted) pending.erase(std::find(pending.begin(), pending.end(), t));
     }
-    // launch the next chunk of every idle group with shots on board
+    // launch the next chunk of every idle group with shots on board: up to
+    // the first shot that finishes, at most chunk_steps while the group has
+    // free slots (new arrivals wait at most that long) and 8 x chunk_steps
+    // when it is full
     for (auto &g : groups) {
       if (g->inflight || g->active() == 0) continue;
       const Model &m = *models[g->model];
-      if (cgfd_run_async(g->s, g->clock, d.chunk_steps, m.d.dt) != CGFD_OK) {
+      int left = 1 << 30;
+      for (const Slot &sl : g->slot)
+        if (sl.busy) left = std::min(left, sl.start + sl.nt - g->clock);
+      int len = (std::max(left, 1) + 7) / 8 * 8;
+      len = std::min(len, g->active() < g->cap ? d.chunk_steps : 8 * d.chunk_steps);
+      if (cgfd_run_async(g->s, g->clock, len, m.d.dt) != CGFD_OK) {
         // fail every shot of the group
         std::lock_guard<std::mutex> lk(mu);
         for (Slot &sl : g->slot) {
@@ -338,7 +346,7 @@ void cgfd_server::run() {
         cv_done.notify_all();
         continue;
       }
-      g->clock += d.chunk_steps;
+      g->clock += len;
       g->inflight = true;
       g->last_used = Clock::now();
     }

@@ -30,7 +30,9 @@ typedef struct cgfd_server cgfd_server;
 typedef struct {
   long long hbm_budget; /* device bytes the server may hold; 0: 90% of the free HBM at creation */
   int max_batch;        /* shots per group, 1..64 */
-  int chunk_steps;      /* steps between admission points, a positive multiple of 8 */
+  int chunk_steps;      /* a positive multiple of 8: the longest a group with free slots runs between
+                           admission points; a full group runs up to its first finishing shot, at
+                           most 8 x chunk_steps */
 } cgfd_server_desc;
 
 /* A model: grid, medium, boundaries and time step, arrays in the reference
