@@ -28,7 +28,7 @@ EXPORTS = [
     "cgfd_el_iso_onestage", "cgfd_read_seismo", "cgfd_fetch_level",
     "cgfd_fetch_pml_aux", "cgfd_set_timing", "cgfd_last_run_timing",
     "cgfd_set_tile", "cgfd_get_tile", "cgfd_set_graph", "cgfd_set_shot",
-    "cgfd_run_async", "cgfd_poll", "cgfd_wait",
+    "cgfd_run_async", "cgfd_poll", "cgfd_wait", "cgfd_reserve_receivers",
     "cgfd_grid_cartesian", "cgfd_grid_topography", "cgfd_metric_calc", "cgfd_dt_estimate",
     "cgfd_media_layered", "cgfd_media_homogeneous", "cgfd_rho_to_slow", "cgfd_pml_len_dh",
     "cgfd_pml_profile", "cgfd_free_surface_matrices", "cgfd_wavelet", "cgfd_angle2moment",
@@ -152,6 +152,7 @@ def load(path=LIB_PATH):
         "cgfd_set_shot": ([vp, i, i, i], i),
         "cgfd_run_async": ([vp, i, i, C.c_double], i),
         "cgfd_poll": ([vp], i),
+        "cgfd_reserve_receivers": ([vp, i, i], i),
         "cgfd_wait": ([vp], i),
         "cgfd_set_tile": ([vp, i, i], i),
         "cgfd_get_tile": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)], i),
@@ -355,6 +356,9 @@ class Session:
 
     def set_timing(self, on):
         _check(_lib.cgfd_set_timing(self.h, 1 if on else 0))
+
+    def reserve_receivers(self, max_nrecv, max_nt):
+        _check(_lib.cgfd_reserve_receivers(self.h, max_nrecv, max_nt))
 
     def set_graph(self, on):
         _check(_lib.cgfd_set_graph(self.h, 1 if on else 0))

@@ -159,10 +159,9 @@ struct cgfd_session {
   float sc[9], isc[9];
   std::vector<SrcHost> src;
   std::vector<RecvHost> rec;
-  // device source CSR
+  // device source CSR (grow-only buffers: capacities in bytes)
   int s_npt = 0;
-  void *s_buf = nullptr;
-  size_t s_bytes = 0;
+  size_t s_cap[11] = {0};
   int *s_pt_shot = nullptr, *s_pt_beg = nullptr, *s_ct_src = nullptr, *s_itb = nullptr, *s_ite = nullptr,
       *s_kind = nullptr;
   long *s_pt_off = nullptr, *s_tab = nullptr;
@@ -175,7 +174,9 @@ struct cgfd_session {
   long *r_off = nullptr, *r_out = nullptr;
   void *r_frac = nullptr, *seismo = nullptr;
   long seismo_elems = 0;
-  std::vector<long> seismo_off;  // per shot
+  std::vector<long> seismo_off;  // per shot: shot * seismo_stride
+  long seismo_stride = 0;        // elements reserved per shot
+  size_t r_cap[5] = {0};
   bool rec_dirty = false;
   int *clock = nullptr, *shot_start = nullptr, *shot_active = nullptr;
   // interior tile choice (-1: not tuned yet)
@@ -225,6 +226,20 @@ void dfree(cgfd_session *s, void *&p) {
   if (it != s->allocs.end()) s->allocs.erase(it);
   (void)hipFree(p);
   p = nullptr;
+}
+
+// Device buffer of at least `need` bytes, reallocated only when it grows, so
+// that rebuilding small tables between runs does not free memory (a free
+// waits for the whole device, including other sessions' work).
+int ensure(cgfd_session *s, void **p, size_t *cap, size_t need) {
+  if (*p && need <= *cap) return CGFD_OK;
+  if (*p) dfree(s, *p);
+  *cap = 0;
+  const size_t want = std::max<size_t>(need + need / 2, 256);
+  int r = dalloc(s, p, want);
+  if (r) return r;
+  *cap = want;
+  return CGFD_OK;
 }
 
 // stream-ordered copies; the session stream never waits on other streams
@@ -361,14 +376,6 @@ int fill_zero(cgfd_session *s, void *p, size_t elems, size_t esz) {
 
 // ---- rebuild device source CSR over all shots ----
 template <typename TM> int build_sources(cgfd_session *s) {
-  for (void *p : {(void *)s->s_pt_shot, (void *)s->s_pt_beg, (void *)s->s_ct_src, (void *)s->s_itb,
-                  (void *)s->s_ite, (void *)s->s_kind, (void *)s->s_pt_off, (void *)s->s_tab, s->s_ct_coef, s->s_F,
-                  s->s_M}) {
-    if (p) { void *q = p; dfree(s, q); }
-  }
-  s->s_pt_shot = s->s_pt_beg = s->s_ct_src = s->s_itb = s->s_ite = s->s_kind = nullptr;
-  s->s_pt_off = s->s_tab = nullptr;
-  s->s_ct_coef = s->s_F = s->s_M = nullptr;
   s->s_npt = 0;
   std::vector<int> itb, ite, kind;
   std::vector<long> tab;
@@ -399,8 +406,8 @@ template <typename TM> int build_sources(cgfd_session *s) {
     }
   }
   const int npt = (int)pts.size();
-  s->s_npt = npt;
   s->s_ntab = ntab;
+  invalidate(s);
   if (npt == 0) { s->src_dirty = false; return CGFD_OK; }
   std::vector<int> pt_shot, pt_beg, ct_src;
   std::vector<long> pt_off;
@@ -412,8 +419,9 @@ template <typename TM> int build_sources(cgfd_session *s) {
     for (auto &c : kv.second) { ct_src.push_back(c.first); ct_coef.push_back((TM)c.second); }
   }
   pt_beg.push_back((int)ct_src.size());
+  int slot = 0;
   auto up = [&](void **dst, const void *src, size_t bytes) -> int {
-    int r = dalloc(s, dst, bytes);
+    int r = ensure(s, dst, &s->s_cap[slot++], bytes);
     if (r) return r;
     if (bytes) HIPCHK(h2d(s, *dst, src, bytes));
     return CGFD_OK;
@@ -433,76 +441,79 @@ template <typename TM> int build_sources(cgfd_session *s) {
   for (int c = 0; c < 6; c++) Mall.insert(Mall.end(), M[c].begin(), M[c].end());
   if ((r = up(&s->s_F, Fall.data(), Fall.size() * sizeof(TM)))) return r;
   if ((r = up(&s->s_M, Mall.data(), Mall.size() * sizeof(TM)))) return r;
+  s->s_npt = npt;
   s->src_dirty = false;
-  invalidate(s);
   return CGFD_OK;
 }
 
-// Rebuild the receiver tables of all shots.  Seismograms of shots whose
-// receivers did not change are carried over, so a shot can be given new
-// receivers while the others keep what they recorded.
+// Rebuild the receiver tables of all shots.  Each shot owns a fixed-stride
+// seismogram region; shots whose receivers did not change keep what they
+// recorded, and while the stride suffices nothing is reallocated, so a shot
+// can be given new receivers while the others keep running.
 template <typename TC> int build_receivers(cgfd_session *s) {
-  for (void *p : {(void *)s->r_shot, (void *)s->r_nt, (void *)s->r_off, (void *)s->r_out, s->r_frac}) {
-    if (p) { void *q = p; dfree(s, q); }
-  }
-  s->r_shot = s->r_nt = nullptr;
-  s->r_off = s->r_out = nullptr;
-  s->r_frac = nullptr;
-  void *old_seismo = s->seismo;
-  const std::vector<long> old_off = s->seismo_off;
-  s->seismo = nullptr;
   std::vector<int> shot_of, nt_of;
   std::vector<long> off, out;
   std::vector<TC> frac;
-  long total = 0;
-  s->seismo_off.assign(s->rec.size(), 0);
+  long stride = 0;
+  for (const RecvHost &h : s->rec) stride = std::max(stride, (long)h.nrecv * kNC * h.nt);
+  const bool grow = stride > s->seismo_stride || !s->seismo;
+  const long new_stride = grow ? std::max(stride, s->seismo_stride) : s->seismo_stride;
   for (int shot = 0; shot < (int)s->rec.size(); shot++) {
     const RecvHost &h = s->rec[shot];
-    s->seismo_off[shot] = total;
+    long pos = (long)shot * new_stride;
     for (int r = 0; r < h.nrecv; r++) {
       shot_of.push_back(shot);
       nt_of.push_back(h.nt);
       for (int q = 0; q < 8; q++) off.push_back(ref_to_dev(s->L, h.indx8[(size_t)r * 8 + q]));
       for (int q = 0; q < 3; q++) frac.push_back((TC)h.frac[(size_t)r * 3 + q]);
-      out.push_back(total);
-      total += (long)kNC * h.nt;
+      out.push_back(pos);
+      pos += (long)kNC * h.nt;
     }
   }
   s->r_n = (int)shot_of.size();
-  s->seismo_elems = total;
   invalidate(s);
-  if (s->r_n == 0) {
-    if (old_seismo) dfree(s, old_seismo);
-    s->rec_dirty = false;
-    std::fill(s->rec_changed.begin(), s->rec_changed.end(), 0);
-    return CGFD_OK;
+  int r;
+  if (new_stride > 0 && grow) {
+    void *old = s->seismo;
+    const long old_stride = s->seismo_stride;
+    s->seismo = nullptr;
+    if ((r = dalloc(s, &s->seismo, (size_t)s->d.nshots * new_stride * sizeof(TC)))) return r;
+    if ((r = fill_zero(s, s->seismo, (size_t)s->d.nshots * new_stride, sizeof(TC)))) return r;
+    if (old) {
+      for (int shot = 0; shot < (int)s->rec.size(); shot++) {
+        const size_t n = (size_t)s->rec[shot].nrecv * kNC * s->rec[shot].nt;
+        if (s->rec_changed[shot] || n == 0) continue;
+        HIPCHK(hipMemcpyAsync((TC *)s->seismo + (size_t)shot * new_stride, (const TC *)old + (size_t)shot * old_stride,
+                              n * sizeof(TC), hipMemcpyDeviceToDevice, s->st));
+      }
+      HIPCHK(hipStreamSynchronize(s->st));
+      dfree(s, old);
+    }
+    s->seismo_stride = new_stride;
+  } else if (new_stride > 0) {
+    for (int shot = 0; shot < (int)s->rec.size(); shot++)
+      if (s->rec_changed[shot] && (r = fill_zero(s, (TC *)s->seismo + (size_t)shot * new_stride,
+                                                  (size_t)new_stride, sizeof(TC))))
+        return r;
   }
+  s->seismo_off.assign(s->rec.size(), 0);
+  for (int shot = 0; shot < (int)s->rec.size(); shot++) s->seismo_off[shot] = (long)shot * s->seismo_stride;
+  s->seismo_elems = (long)s->d.nshots * s->seismo_stride;
+  std::fill(s->rec_changed.begin(), s->rec_changed.end(), 0);
+  s->rec_dirty = false;
+  if (s->r_n == 0) return CGFD_OK;
+  int slot = 0;
   auto up = [&](void **dst, const void *src, size_t bytes) -> int {
-    int r = dalloc(s, dst, bytes);
-    if (r) return r;
+    int rr = ensure(s, dst, &s->r_cap[slot++], bytes);
+    if (rr) return rr;
     HIPCHK(h2d(s, *dst, src, bytes));
     return CGFD_OK;
   };
-  int r;
   if ((r = up((void **)&s->r_shot, shot_of.data(), shot_of.size() * 4))) return r;
   if ((r = up((void **)&s->r_nt, nt_of.data(), nt_of.size() * 4))) return r;
   if ((r = up((void **)&s->r_off, off.data(), off.size() * 8))) return r;
   if ((r = up((void **)&s->r_out, out.data(), out.size() * 8))) return r;
   if ((r = up(&s->r_frac, frac.data(), frac.size() * sizeof(TC)))) return r;
-  if ((r = dalloc(s, &s->seismo, (size_t)total * sizeof(TC)))) return r;
-  if ((r = fill_zero(s, s->seismo, (size_t)total, sizeof(TC)))) return r;
-  if (old_seismo) {
-    for (int shot = 0; shot < (int)s->rec.size(); shot++) {
-      const size_t n = (size_t)s->rec[shot].nrecv * kNC * s->rec[shot].nt;
-      if (s->rec_changed[shot] || n == 0) continue;
-      HIPCHK(hipMemcpyAsync((TC *)s->seismo + s->seismo_off[shot], (const TC *)old_seismo + old_off[shot],
-                            n * sizeof(TC), hipMemcpyDeviceToDevice, s->st));
-    }
-    HIPCHK(hipStreamSynchronize(s->st));
-    dfree(s, old_seismo);
-  }
-  std::fill(s->rec_changed.begin(), s->rec_changed.end(), 0);
-  s->rec_dirty = false;
   return CGFD_OK;
 }
 
@@ -1430,6 +1441,30 @@ int cgfd_read_peak_motion(cgfd_session *s, int shot, int dtype, void *out) {
         else ((float *)out)[q] = (float)x;
       }
   return CGFD_OK;
+}
+
+int cgfd_reserve_receivers(cgfd_session *s, int max_nrecv, int max_nt) {
+  if (!s || max_nrecv < 0 || max_nt < 0) return fail(CGFD_E_ARG, "bad reserve_receivers arguments");
+  const long want = (long)max_nrecv * kNC * max_nt;
+  if (want <= s->seismo_stride) return CGFD_OK;
+  void *old = s->seismo;
+  const long old_stride = s->seismo_stride;
+  void *p = nullptr;
+  int r = dalloc(s, &p, (size_t)s->d.nshots * want * s->ms);
+  if (r) return r;
+  if ((r = fill_zero(s, p, (size_t)s->d.nshots * want, s->ms))) return r;
+  if (old) {
+    for (int shot = 0; shot < s->d.nshots; shot++)
+      HIPCHK(hipMemcpyAsync((char *)p + (size_t)shot * want * s->ms, (const char *)old + (size_t)shot * old_stride * s->ms,
+                            (size_t)old_stride * s->ms, hipMemcpyDeviceToDevice, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    dfree(s, old);
+  }
+  s->seismo = p;
+  s->seismo_stride = want;
+  // receiver tables point into the seismogram buffer by offset: rebuild them
+  s->rec_dirty = true;
+  return s->ms == 8 ? build_receivers<double>(s) : build_receivers<float>(s);
 }
 
 int cgfd_set_graph(cgfd_session *s, int on) {

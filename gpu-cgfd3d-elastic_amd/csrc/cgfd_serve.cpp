@@ -183,6 +183,11 @@ Group *cgfd_server::make_group(int model, std::string &why) {
     return nullptr;
   }
   for (int q = 0; q < cap; q++) cgfd_set_shot(s, q, 0, 0);
+  if (cgfd_reserve_receivers(s, m.d.max_nrecv, m.d.max_nt) != CGFD_OK) {
+    why = cgfd_last_error();
+    cgfd_destroy(s);
+    return nullptr;
+  }
   auto g = std::make_unique<Group>();
   g->model = model;
   g->s = s;

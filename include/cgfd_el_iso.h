@@ -211,6 +211,12 @@ int cgfd_read_peak_motion(cgfd_session *s, int shot, int dtype, void *out);
  * reference launches and synchronises per kernel. */
 int cgfd_set_graph(cgfd_session *s, int on);
 
+/* Reserve seismogram memory for up to max_nrecv receivers over max_nt steps
+ * per shot, so that later cgfd_upload_receivers within these bounds never
+ * reallocate device memory (no reference counterpart; used when shots are
+ * replaced while others run). */
+int cgfd_reserve_receivers(cgfd_session *s, int max_nrecv, int max_nt);
+
 /* Continuous batching of shots (no reference counterpart).  A shot with
  * active = 1 advances with every later run, its own step number being the
  * session step minus it_start; its source and receivers use that number, and

@@ -46,7 +46,8 @@ def test_graph_periods_equal_eager_launches():
     b.close()
 
 
-def test_shot_admitted_mid_run_matches_standalone():
+@pytest.mark.parametrize("reserve", [False, True], ids=["rebuild", "reserved"])
+def test_shot_admitted_mid_run_matches_standalone(reserve):
     A = _model((12, 14, 20), 40)
     B = _model((25, 20, 15), 24)
     sa = _sess(A)
@@ -55,6 +56,8 @@ def test_shot_admitted_mid_run_matches_standalone():
     sb.run(0, B.nt_total, B.dt)
 
     m = _sess(A, nshots=3)
+    if reserve:
+        m.reserve_receivers(len(A.recv_indx), 64)
     m.set_shot(1, False)
     m.set_shot(2, False)
     m.run(0, 16, A.dt)
