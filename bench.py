@@ -171,6 +171,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain", action="store_true", help="per-point kernel everywhere")
+    ap.add_argument("--pml-march", action="store_true", help="z-marching sweep over the PML slabs")
     ap.add_argument("--tile", default="", metavar="SHAPE,ZCHUNK",
                     help="pin the interior tile instead of autotuning (profiling runs)")
     ap.add_argument("--timed-launch-steps", type=int, default=8,
@@ -213,8 +214,8 @@ def main():
         raise SystemExit(f"dt {m['dt']} exceeds the stability limit {m['dtmax']}")
 
     nt_total = args.warmup + args.steps
-    sess = cgfd.Session(m["ni"], m["nj"], m["nk"], storage=STORE[args.storage],
-                        flags=cgfd.FLAG_PLAIN if args.plain else 0)
+    flags = (cgfd.FLAG_PLAIN if args.plain else 0) | (cgfd.FLAG_PML_MARCH if args.pml_march else 0)
+    sess = cgfd.Session(m["ni"], m["nj"], m["nk"], storage=STORE[args.storage], flags=flags)
     nt_total += args.timed_launch_steps
     workloads.upload(sess, m, nt_total=nt_total)
     if args.tile:
@@ -303,7 +304,8 @@ def main():
         "config": {"workload": f"{m['name']}: {m['ni']}x{m['nj']}x{m['nk']} el_iso RK4, "
                                f"{'curvilinear topography, 3 perturbed layers, double couple, 400 receivers' if m['name'] == 'topo' else 'Cartesian homogeneous, explosion, 20 receivers'}, "
                                f"free surface + 10-layer CFS-PML, dt {m['dt']}",
-                   "cells": cells, "storage": args.storage, "kernel_path": "plain" if args.plain else "zmarch",
+                   "cells": cells, "storage": args.storage,
+                   "kernel_path": "plain" if args.plain else ("zmarch+pml-march" if args.pml_march else "zmarch"),
                    "seismograms_finite": finite, "device_ms": round(run_ms, 2)},
         "roofline": roof,
         "reduced_storage": reduced,

@@ -20,17 +20,15 @@
 // On a Cartesian grid every metric component is one constant (CART = true):
 // the nine metric values come in as arguments instead of nine HBM streams.
 //
-// The interior variant (PML = false) sweeps the box outside every PML face
-// box; the PML variant also keeps the raw derivatives and applies the CFS-PML
-// ADE terms and memory-variable update to the face slabs.  Points within 4
-// rows of a free surface go to the per-point kernel.
+// k_march sweeps the box outside every PML face box; the split-role variant
+// below also serves the PML face slabs (CGFD_FLAG_PML_MARCH).  Points within
+// 4 rows of a free surface go to the per-point kernel.
 #include "el_iso_device.h"
 #include "el_iso_launch.h"
-#include "el_iso_pml.h"
 
 namespace cgfd {
 
-template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINB, bool PML, bool CART>
+template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINB, bool CART>
 __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, MarchSet b, MarchGrid g) {
   constexpr int NT = TX * TY;
   constexpr int HXL = (DX == 0) ? 3 : 1;
@@ -191,7 +189,6 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
     // (V) or the stress divergence (T): 12 accumulators instead of 27 values
     TC Lg[3][3];                       // Lg[a][b] = dV_a / dx_b
     TC dv[3] = {TC(0), TC(0), TC(0)};  // dT_ab / dx_b
-    TC Dxr[PML ? kNC : 1], Dyr[PML ? kNC : 1], Dzr[PML ? kNC : 1];  // raw derivatives for the PML terms
 #pragma unroll
     for (int c = 0; c < kNC; c++) {
       TC dx = TC(0), dy = TC(0), dz = TC(0);
@@ -201,7 +198,6 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
         dy += Drp<TC, DY>::c(n) * lds[buf][c][ly + n][lx + HXL];
         dz += Drp<TC, DZ>::c(n) * q[c][n];
       }
-      if constexpr (PML) { Dxr[c] = dx; Dyr[c] = dy; Dzr[c] = dz; }
       const TC g0 = xix * dx + etx * dy + ztx * dz;
       const TC g1 = xiy * dx + ety * dy + zty * dz;
       const TC g2 = xiz * dx + etz * dy + ztz * dz;
@@ -234,9 +230,6 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
       h[TXY] = mu * (Lg[0][1] + Lg[1][0]);
       h[TXZ] = mu * (Lg[0][2] + Lg[2][0]);
       h[TYZ] = mu * (Lg[1][2] + Lg[2][1]);
-      if constexpr (PML)
-        pml_apply<TS, TC>(a, shot, i, j, k, Dxr, Dyr, Dzr, xix, xiy, xiz, etx, ety, etz, ztx, zty, ztz, lam, mu,
-                          lam2mu, slw, h);
 #pragma unroll
       for (int c = 0; c < kNC; c++) {
         TS *wout = a.w_out + so + c * cs;
@@ -272,7 +265,76 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
 // k_march (same derivative sums, same order).
 enum SplitRole { ROLE_V = 0, ROLE_T = 1 };
 
-template <int ROLE, int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, bool CART>
+// CFS-PML terms of one role at one point, for every face box containing it
+// (the reference's face order, sv_curv_col_el_iso_gpu.cu:613-1147): R = the
+// part of the role's right-hand side along the face's axis from the raw
+// derivatives Dr[axis][component], rhs += (B-1) R - B Phi, Phi' = D R - A Phi,
+// and the RK update of the role's memory variables.  Points of the sweep lie
+// below the free-surface rows, so the surface term of the x/y faces never
+// applies here.
+template <int ROLE, typename TS, typename TC, int NCG>
+__device__ __forceinline__ void pml_role(const StageArgs<TS, TC> &a, int shot, int i, int j, int k,
+                                         const TC (&Dr)[3][NCG], const TC *mt, TC *h) {
+  constexpr int NOUT = ROLE == ROLE_V ? 3 : 6;
+  constexpr int OOFF = ROLE == ROLE_V ? 0 : 3;
+  auto S = [&](int c) -> TC { return St<TS>::scaled ? (TC)a.sc[c] : TC(1); };
+#pragma unroll
+  for (int fi = 0; fi < 6; fi++) {
+    const FaceArgs<TS, TC> &F = a.f[fi];
+    if (!F.on) continue;
+    if (i < F.i1 || i > F.i2 || j < F.j1 || j > F.j2 || k < F.k1 || k > F.k2) continue;
+    const int dim = fi >> 1;
+    const TC *Dd = Dr[dim];
+    const TC e1 = mt[3 * dim], e2 = mt[3 * dim + 1], e3 = mt[3 * dim + 2];
+    TC R[NOUT];
+    if constexpr (ROLE == ROLE_V) {
+      const TC slw = mt[9];
+      R[0] = slw * (e1 * Dd[TXX - 3] + e2 * Dd[TXY - 3] + e3 * Dd[TXZ - 3]);
+      R[1] = slw * (e1 * Dd[TXY - 3] + e2 * Dd[TYY - 3] + e3 * Dd[TYZ - 3]);
+      R[2] = slw * (e1 * Dd[TXZ - 3] + e2 * Dd[TYZ - 3] + e3 * Dd[TZZ - 3]);
+    } else {
+      const TC lam = mt[9], mu = mt[10];
+      const TC lam2mu = lam + TC(2) * mu;
+      R[TXX - 3] = lam2mu * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam * e3 * Dd[VZ];
+      R[TYY - 3] = lam * e1 * Dd[VX] + lam2mu * e2 * Dd[VY] + lam * e3 * Dd[VZ];
+      R[TZZ - 3] = lam * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam2mu * e3 * Dd[VZ];
+      R[TXY - 3] = mu * (e2 * Dd[VX] + e1 * Dd[VY]);
+      R[TXZ - 3] = mu * (e3 * Dd[VX] + e1 * Dd[VZ]);
+      R[TYZ - 3] = mu * (e3 * Dd[VY] + e2 * Dd[VZ]);
+    }
+    const int ic = (dim == 0) ? i - F.i1 : (dim == 1) ? j - F.j1 : k - F.k1;
+    const unsigned ia = (unsigned)(((k - F.k1) * F.bj + (j - F.j1)) * F.bi + (i - F.i1));
+    const TC cD = F.D[ic], cA = F.A[ic], cB = F.B[ic];
+    const TC Bm1 = cB - TC(1);
+    const long fso = (long)shot * F.shot_stride;
+    const unsigned fs = (unsigned)F.fsize;
+#pragma unroll
+    for (int cg = 0; cg < NOUT; cg++) {
+      const int c = OOFF + cg;
+      const unsigned qb = (c * fs + ia) * (unsigned)sizeof(TS);
+      const TC P = (TC)St<TS>::ld(&at_b(F.a_in + fso, qb)) * S(c);
+      h[cg] += Bm1 * R[cg] - cB * P;
+      const TC Ph = cD * R[cg] - cA * P;
+      const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
+      if (a.mode == MODE_RHS) {
+        St<TS>::st(&at_b(F.a_out + fso, qb), Ph * is);
+      } else if (a.mode == MODE_S0) {
+        St<TS>::st(&at_b(F.a_out + fso, qb), (P + a.ca * Ph) * is);
+        St<TS>::st(&at_b(F.a_end + fso, qb), (P + a.cb * Ph) * is);
+      } else if (a.mode == MODE_S12) {
+        const TC pre = (TC)St<TS>::ld(&at_b(F.a_pre + fso, qb)) * S(c);
+        const TC en = (TC)St<TS>::ld(&at_b(F.a_end + fso, qb)) * S(c);
+        St<TS>::st(&at_b(F.a_out + fso, qb), (pre + a.ca * Ph) * is);
+        St<TS>::st(&at_b(F.a_end + fso, qb), (en + a.cb * Ph) * is);
+      } else {
+        const TC en = (TC)St<TS>::ld(&at_b(F.a_end + fso, qb)) * S(c);
+        St<TS>::st(&at_b(F.a_end + fso, qb), (en + a.cb * Ph) * is);
+      }
+    }
+  }
+}
+
+template <int ROLE, int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, bool CART, bool PML>
 __device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf, int gt, int i0, int j0, int bi1,
                                             int bj1, int kbeg, int kend, int shot) {
   constexpr int NT = TX * TY;
@@ -401,6 +463,7 @@ __device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf
 
     TC Lg[3][3];
     TC dv[3] = {TC(0), TC(0), TC(0)};
+    TC Dr[3][PML ? NCG : 1];  // raw derivatives along x, y, z for the PML terms
 #pragma unroll
     for (int cg = 0; cg < NCG; cg++) {
       const int c = COFF + cg;
@@ -411,6 +474,7 @@ __device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf
         dy += Drp<TC, DY>::c(n) * lb[(c * LH + ly + n) * LW + lx + HXL];
         dz += Drp<TC, DZ>::c(n) * q[cg][n];
       }
+      if constexpr (PML) { Dr[0][cg] = dx; Dr[1][cg] = dy; Dr[2][cg] = dz; }
       const TC g0 = xix * dx + etx * dy + ztx * dz;
       const TC g1 = xiy * dx + ety * dy + zty * dz;
       const TC g2 = xiz * dx + etz * dy + ztz * dz;
@@ -448,6 +512,7 @@ __device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf
         h[TXZ - 3] = mu * (Lg[0][2] + Lg[2][0]);
         h[TYZ - 3] = mu * (Lg[1][2] + Lg[2][1]);
       }
+      if constexpr (PML) pml_role<ROLE, TS, TC, NCG>(a, shot, i, j, k, Dr, mt, h);
 #pragma unroll
       for (int cg = 0; cg < NOUT; cg++) {
         const int c = OOFF + cg;
@@ -471,7 +536,7 @@ __device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf
 }
 
 // MINW: minimum waves per SIMD, which caps the registers per thread (4 waves: 128)
-template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINW, bool CART>
+template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINW, bool CART, bool PML>
 __global__ __launch_bounds__(2 * TX *TY, MINW) void k_march_split(StageArgs<TS, TC> a, MarchSet b, MarchGrid g) {
   constexpr int NT = TX * TY;
   __shared__ TC lds[2][kNC][TY + 4][TX + 4];
@@ -498,14 +563,14 @@ __global__ __launch_bounds__(2 * TX *TY, MINW) void k_march_split(StageArgs<TS, 
   const int tid = threadIdx.x;
   // wave-uniform role: NT is a multiple of the wavefront size
   if (tid < NT)
-    split_sweep<ROLE_V, DX, DY, DZ, TS, TC, TX, TY, CART>(a, &lds[0][0][0][0], tid, i0, j0, b.box[bx][1],
+    split_sweep<ROLE_V, DX, DY, DZ, TS, TC, TX, TY, CART, PML>(a, &lds[0][0][0][0], tid, i0, j0, b.box[bx][1],
                                                            b.box[bx][3], kbeg, kend, shot);
   else
-    split_sweep<ROLE_T, DX, DY, DZ, TS, TC, TX, TY, CART>(a, &lds[0][0][0][0], tid - NT, i0, j0, b.box[bx][1],
+    split_sweep<ROLE_T, DX, DY, DZ, TS, TC, TX, TY, CART, PML>(a, &lds[0][0][0][0], tid - NT, i0, j0, b.box[bx][1],
                                                            b.box[bx][3], kbeg, kend, shot);
 }
 
-template <typename TS, typename TC, int TX, int TY, int MINW, bool CART>
+template <typename TS, typename TC, int TX, int TY, int MINW, bool CART, bool PML>
 static hipError_t launch_split_t(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
                                  hipStream_t st) {
   static_assert((TX * TY) % 64 == 0, "roles must split at a wavefront boundary");
@@ -524,7 +589,7 @@ static hipError_t launch_split_t(const StageArgs<TS, TC> &a, const MarchSet &b, 
   dim3 grid((unsigned)nwg), block(2 * TX * TY);
   const int sel = dx * 4 + dy * 2 + dz;
 #define CGFD_L(X, Y, Z) \
-  hipLaunchKernelGGL((k_march_split<X, Y, Z, TS, TC, TX, TY, MINW, CART>), grid, block, 0, st, a, b, g)
+  hipLaunchKernelGGL((k_march_split<X, Y, Z, TS, TC, TX, TY, MINW, CART, PML>), grid, block, 0, st, a, b, g)
   switch (sel) {
     case 0: CGFD_L(0, 0, 0); break;
     case 1: CGFD_L(0, 0, 1); break;
@@ -539,7 +604,7 @@ static hipError_t launch_split_t(const StageArgs<TS, TC> &a, const MarchSet &b, 
   return hipGetLastError();
 }
 
-template <typename TS, typename TC, int TX, int TY, int MINB, bool PML, bool CART>
+template <typename TS, typename TC, int TX, int TY, int MINB, bool CART>
 static hipError_t launch_march_t(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz,
                                  int nshots, hipStream_t st) {
   MarchGrid g{};
@@ -557,7 +622,7 @@ static hipError_t launch_march_t(const StageArgs<TS, TC> &a, const MarchSet &b, 
   dim3 grid((unsigned)nwg), block(TX * TY);
   const int sel = dx * 4 + dy * 2 + dz;
 #define CGFD_L(X, Y, Z) \
-  hipLaunchKernelGGL((k_march<X, Y, Z, TS, TC, TX, TY, MINB, PML, CART>), grid, block, 0, st, a, b, g)
+  hipLaunchKernelGGL((k_march<X, Y, Z, TS, TC, TX, TY, MINB, CART>), grid, block, 0, st, a, b, g)
   switch (sel) {
     case 0: CGFD_L(0, 0, 0); break;
     case 1: CGFD_L(0, 0, 1); break;
@@ -592,40 +657,44 @@ hipError_t launch_interior(const StageArgs<TS, TC> &a, const MarchSet &b, int dx
   if (b.shape >= 4) {
     if (a.met_const) {
       switch (b.shape) {
-        case 5: return launch_split_t<TS, TC, 32, 8, 4, true>(a, b, dx, dy, dz, nshots, st);
-        case 6: return launch_split_t<TS, TC, 64, 4, 4, true>(a, b, dx, dy, dz, nshots, st);
-        default: return launch_split_t<TS, TC, 64, 8, 1, true>(a, b, dx, dy, dz, nshots, st);
+        case 5: return launch_split_t<TS, TC, 32, 8, 4, true, false>(a, b, dx, dy, dz, nshots, st);
+        case 6: return launch_split_t<TS, TC, 64, 4, 4, true, false>(a, b, dx, dy, dz, nshots, st);
+        default: return launch_split_t<TS, TC, 64, 8, 1, true, false>(a, b, dx, dy, dz, nshots, st);
       }
     }
     switch (b.shape) {
-      case 5: return launch_split_t<TS, TC, 32, 8, 4, false>(a, b, dx, dy, dz, nshots, st);
-      case 6: return launch_split_t<TS, TC, 64, 4, 4, false>(a, b, dx, dy, dz, nshots, st);
-      default: return launch_split_t<TS, TC, 64, 8, 1, false>(a, b, dx, dy, dz, nshots, st);
+      case 5: return launch_split_t<TS, TC, 32, 8, 4, false, false>(a, b, dx, dy, dz, nshots, st);
+      case 6: return launch_split_t<TS, TC, 64, 4, 4, false, false>(a, b, dx, dy, dz, nshots, st);
+      default: return launch_split_t<TS, TC, 64, 8, 1, false, false>(a, b, dx, dy, dz, nshots, st);
     }
   }
   if (a.met_const) {
     switch (b.shape) {
-      case 1: return launch_march_t<TS, TC, 32, 8, 2, false, true>(a, b, dx, dy, dz, nshots, st);
-      case 2: return launch_march_t<TS, TC, 64, 8, 1, false, true>(a, b, dx, dy, dz, nshots, st);
-      case 3: return launch_march_t<TS, TC, 32, 16, 1, false, true>(a, b, dx, dy, dz, nshots, st);
-      default: return launch_march_t<TS, TC, 64, 4, 2, false, true>(a, b, dx, dy, dz, nshots, st);
+      case 1: return launch_march_t<TS, TC, 32, 8, 2, true>(a, b, dx, dy, dz, nshots, st);
+      case 2: return launch_march_t<TS, TC, 64, 8, 1, true>(a, b, dx, dy, dz, nshots, st);
+      case 3: return launch_march_t<TS, TC, 32, 16, 1, true>(a, b, dx, dy, dz, nshots, st);
+      default: return launch_march_t<TS, TC, 64, 4, 2, true>(a, b, dx, dy, dz, nshots, st);
     }
   }
   switch (b.shape) {
-    case 1: return launch_march_t<TS, TC, 32, 8, 2, false, false>(a, b, dx, dy, dz, nshots, st);
-    case 2: return launch_march_t<TS, TC, 64, 8, 1, false, false>(a, b, dx, dy, dz, nshots, st);
-    case 3: return launch_march_t<TS, TC, 32, 16, 1, false, false>(a, b, dx, dy, dz, nshots, st);
-    default: return launch_march_t<TS, TC, 64, 4, 2, false, false>(a, b, dx, dy, dz, nshots, st);
+    case 1: return launch_march_t<TS, TC, 32, 8, 2, false>(a, b, dx, dy, dz, nshots, st);
+    case 2: return launch_march_t<TS, TC, 64, 8, 1, false>(a, b, dx, dy, dz, nshots, st);
+    case 3: return launch_march_t<TS, TC, 32, 16, 1, false>(a, b, dx, dy, dz, nshots, st);
+    default: return launch_march_t<TS, TC, 64, 4, 2, false>(a, b, dx, dy, dz, nshots, st);
   }
 }
 
 // PML slabs are thin (nlay+1 points) across one axis: a square 16 x 16 tile
-// keeps most lanes busy on x, y and z slabs alike
+// keeps most lanes busy on x, y and z slabs alike.  Measured slower than the
+// per-point shell kernel on 10-layer slabs (a third of the lanes idle, and the
+// bottom slab too thin for the z queue to pay), hence opt-in.
 template <typename TS, typename TC>
 hipError_t launch_pml_march(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
                             hipStream_t st) {
   if (b.nbox < 1) return hipSuccess;
-  return launch_march_t<TS, TC, 16, 16, 1, true, false>(a, b, dx, dy, dz, nshots, st);
+  // split roles keep the PML state per role (about 150 registers each)
+  if (a.met_const) return launch_split_t<TS, TC, 16, 16, 2, true, true>(a, b, dx, dy, dz, nshots, st);
+  return launch_split_t<TS, TC, 16, 16, 2, false, true>(a, b, dx, dy, dz, nshots, st);
 }
 
 #define CGFD_INST(TS, TC)                                                                                     \
