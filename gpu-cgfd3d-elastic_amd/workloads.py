@@ -49,18 +49,22 @@ def _source(ni, nj, nk, sx, sy, depth_idx, dt, f0, t0, m6, nt_total, wavelet_tst
                 ext_num=np.array([s["ext_num"]]), ext_indx=s["ext_indx"], ext_coef=s["ext_coef"], F=None, M=M)
 
 
-def small(seed=1, nt=500):
-    ni, nj, nk, h = 100, 100, 60, 100.0
+def small(seed=1, nt=500, ni=100, nj=100, nk=60):
+    """The 100x100x60 explosion model; other sizes keep its physics, with the
+    source at the horizontal centre and 3 km deep."""
+    h = 100.0
     x, y, z = cgfd.grid_cartesian(ni, nj, nk, h, 0.0, h, 0.0, h, -(nk - 1) * h)
     metric = cgfd.metric_calc(ni, nj, nk, x, y, z)
     lam, mu, rho = cgfd.media_homogeneous(ni, nj, nk, 6000.0, 3464.0, 2670.0)
-    m = dict(name="small", ni=ni, nj=nj, nk=nk, dt=0.005, nt_total=nt + 1, x=x, y=y, z=z, metric=metric,
+    m = dict(name="small" if (ni, nj, nk) == (100, 100, 60) else f"explosion{ni}x{nj}x{nk}", ni=ni, nj=nj, nk=nk,
+             dt=0.005, nt_total=nt + 1, x=x, y=y, z=z, metric=metric,
              lam=lam, mu=mu, rho=rho)
     m["dtmax"] = cgfd.dt_estimate(ni, nj, nk, x, y, z, lam, mu, rho)
     m["slw"] = cgfd.rho_to_slow(rho)
     m["mvx"], m["mvy"] = cgfd.free_surface_matrices(ni, nj, nk, metric, lam, mu)
     m["pml"] = _pml_faces(m, ni, nj, nk, x, y, z, 6000.0)
-    m["src"] = _source(ni, nj, nk, 50.0, 50.0, 30.0, m["dt"], 2.0, 0.6, [1e16, 1e16, 1e16, 0, 0, 0], m["nt_total"])
+    m["src"] = _source(ni, nj, nk, ni / 2, nj / 2, 30.0, m["dt"], 2.0, 0.6, [1e16, 1e16, 1e16, 0, 0, 0],
+                       m["nt_total"])
     m["recv_indx"], m["recv_frac"] = _surface_receivers(ni, nj, nk, 20, seed, PML["nlay"] + 1)
     return m
 
