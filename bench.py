@@ -161,6 +161,57 @@ def serving_sample(n, nt_range, load, seed, nseq):
         groups=stats["groups"], rejected=stats["rejected"])
 
 
+def batch_sample(n, nt_range, seed, nseq, storage):
+    """n shot requests on the 100x100x60 medium submitted at once to the
+    server (one batched group, launches per RK stage shared by all its
+    shots, 8-step periods replayed as graphs); shots/hour against the same
+    shots run one after another in an FP32 single-shot session."""
+    m = workloads.small()
+    reqs = workloads.shot_requests(m, n, seed, nt_range=nt_range)
+    ni, nj, nk, dt = m["ni"], m["nj"], m["nk"], m["dt"]
+    nrec = len(m["recv_indx"])
+    sess = cgfd.Session(ni, nj, nk)
+    workloads.upload(sess, m, nt_total=8)
+    sess.run(0, 8, dt)
+    t0 = time.perf_counter()
+    steps_seq = 0
+    for q in range(nseq):
+        r = reqs[q]
+        sess.reset_shot(0)
+        sess.upload_source(0, r["src"])
+        sess.upload_receivers(0, m["recv_indx"], np.asarray(m["recv_frac"], np.float32), r["nt"])
+        sess.run(0, r["nt"], dt)
+        sess.read_seismo(0)
+        steps_seq += r["nt"]
+    t_seq = (time.perf_counter() - t0) * sum(r["nt"] for r in reqs) / steps_seq
+    sess.close()
+    st = STORE[storage]
+    ex = None
+    if st == cgfd.STORE_F16:
+        ex = np.max([cgfd.storage_scales(st, ni, nj, nk, m["metric"], m["lam"], m["mu"], m["slw"], r["src"], dt)
+                     for r in reqs], axis=0)
+    srv = cgfd.Server(max_batch=n, chunk_steps=64)
+    mid = srv.add_model(ni, nj, nk, dt=dt, storage=st, scale_exp2=ex, max_nrecv=nrec, max_nt=nt_range[1],
+                        **workloads.model_arrays(m))
+    t = srv.submit(mid, 16, reqs[0]["src"], m["recv_indx"], m["recv_frac"])  # warm: autotune, capture
+    srv.wait(t)
+    srv.result(t)
+    start = time.perf_counter()
+    tickets = [srv.submit(mid, r["nt"], r["src"], m["recv_indx"], m["recv_frac"]) for r in reqs]
+    lat = []
+    for t in tickets:
+        srv.wait(t)
+        lat.append(srv.result(t)[2])
+    end = time.perf_counter()
+    srv.close()
+    rate, seq = n / (end - start), n / t_seq
+    return dict(workload=f"{n} shot requests submitted at once on the 100x100x60 medium, steps U{list(nt_range)}, "
+                         f"f0 U[1,4] Hz, {storage} storage, one batched group",
+                shots_per_hour=round(rate * 3600, 1), sequential_plain_shots_per_hour=round(seq * 3600, 1),
+                speedup=round(rate / seq, 3), p50_latency_s=round(float(np.percentile(lat, 50)), 3),
+                p99_latency_s=round(float(np.percentile(lat, 99)), 3))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +230,8 @@ def main():
     ap.add_argument("--reduced-steps", type=int, default=200,
                     help="timed steps of the FP16/BF16-storage runs of the same workload (0: skip)")
     ap.add_argument("--serve-shots", type=int, default=128, help="served-stream sample size (0: skip)")
+    ap.add_argument("--batch-shots", type=int, default=64, help="batched-shots sample size (0: skip)")
+    ap.add_argument("--batch-steps", default="500,2000", help="step range of the batched shots")
     ap.add_argument("--serve-steps", default="120,480", help="step range of the served requests")
     ap.add_argument("--serve-load", type=float, default=8.0, help="arrival rate / sequential plain rate")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -291,6 +344,16 @@ def main():
             f"{serving['sequential_plain_shots_per_hour']}; p50 {serving['p50_latency_s']} s p99 "
             f"{serving['p99_latency_s']} s")
 
+    batched = None
+    if args.batch_shots > 0:
+        lo, hi = (int(v) for v in args.batch_steps.split(","))
+        batched = {}
+        for st in ("f32", "f16"):
+            log(f"batched shots: {args.batch_shots} x {st}")
+            batched[st] = batch_sample(args.batch_shots, (lo, hi), 11, 4, st)
+            log(f"  {batched[st]['shots_per_hour']} shots/h vs sequential "
+                f"{batched[st]['sequential_plain_shots_per_hour']} ({batched[st]['speedup']}x)")
+
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: {args.cpu_steps} steps")
@@ -310,6 +373,7 @@ def main():
         "roofline": roof,
         "reduced_storage": reduced,
         "serving": serving,
+        "batched_shots": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
