@@ -34,6 +34,9 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic HBM bytes per interior cell and RK stage of the fused dataflow:
 # mean of 39/57/57/39 words (wavefield in, coefficients, RK levels out)
 ALGO_BYTES = {"f32": 192.0, "f16": 120.0, "bf16": 120.0, "f64": 384.0}
+# per (point, PML face) pair in the sweep: nine memory variables, mean of
+# 27/45/45/27 words (memory in, RK levels in and out)
+AUX_BYTES = {"f32": 144.0, "f16": 72.0, "bf16": 72.0, "f64": 288.0}
 STORE = {"f32": cgfd.STORE_F32, "f16": cgfd.STORE_F16, "bf16": cgfd.STORE_BF16, "f64": cgfd.STORE_F64}
 
 
@@ -62,13 +65,14 @@ def cpu_baseline(m, steps):
                        f"unfused reference stage order, {sec:.1f} s loop")
 
 
-def measured_traffic(workload, storage):
-    """HBM bytes per interior launch from the committed counter summary, if any.
+def measured_traffic(workload, storage, split):
+    """HBM bytes per sweep launch from the committed counter summary, if any.
 
-    The summary holds FETCH_SIZE + WRITE_SIZE per dispatch of the interior
-    kernel on this workload, scaled by the same counters on a copy kernel of
-    known size."""
-    path = os.path.join(ROOT, "profiles", f"k_march_{workload}_{storage}_hbm_counters.txt")
+    The summary holds FETCH_SIZE + WRITE_SIZE per dispatch of the z-marching
+    kernel on this workload and work split, scaled by the same counters on a
+    copy kernel of known size."""
+    name = "planes" if split == cgfd.SPLIT_PLANES else "shell"
+    path = os.path.join(ROOT, "profiles", f"k_march_{workload}_{storage}_{name}_hbm_counters.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -222,7 +226,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain", action="store_true", help="per-point kernel everywhere")
-    ap.add_argument("--pml-march", action="store_true", help="z-marching sweep over the PML slabs")
+    ap.add_argument("--pml-point", action="store_true",
+                    help="PML slabs by the per-point kernel (the sweep keeps to the box inside them)")
     ap.add_argument("--tile", default="", metavar="SHAPE,ZCHUNK",
                     help="pin the interior tile instead of autotuning (profiling runs)")
     ap.add_argument("--timed-launch-steps", type=int, default=8,
@@ -267,10 +272,14 @@ def main():
         raise SystemExit(f"dt {m['dt']} exceeds the stability limit {m['dtmax']}")
 
     nt_total = args.warmup + args.steps
-    flags = (cgfd.FLAG_PLAIN if args.plain else 0) | (cgfd.FLAG_PML_MARCH if args.pml_march else 0)
+    flags = (cgfd.FLAG_PLAIN if args.plain else 0) | (cgfd.FLAG_PML_POINT if args.pml_point else 0)
     sess = cgfd.Session(m["ni"], m["nj"], m["nk"], storage=STORE[args.storage], flags=flags)
     nt_total += args.timed_launch_steps
     workloads.upload(sess, m, nt_total=nt_total)
+    if args.storage == "f16":
+        # FP16 needs power-of-two scales to keep stresses in range (BF16 has FP32's exponent range)
+        sess.set_storage_scale(cgfd.storage_scales(cgfd.STORE_F16, m["ni"], m["nj"], m["nk"], m["metric"], m["lam"],
+                                                   m["mu"], m["slw"], m["src"], m["dt"]))
     if args.tile:
         sh, zc = (int(v) for v in args.tile.split(","))
         sess.set_tile(sh, zc)
@@ -293,6 +302,7 @@ def main():
     seis = sess.read_seismo(0)
     finite = bool(np.isfinite(seis).all())
     tile = sess.get_tile()
+    split = sess.get_split()
     sess.close()
     wall = t2 - t1
     cells = workloads.cells(m)
@@ -301,14 +311,21 @@ def main():
 
     roof = None
     if n_int > 0 and int_avg_ms > 0:
-        icells = workloads.interior_cells(m)
-        ach = ALGO_BYTES[args.storage] * icells / (int_avg_ms * 1e-3) / 1e9
+        if split == cgfd.SPLIT_SHELL:
+            icells, pairs = workloads.interior_cells(m), 0
+        else:
+            icells, pairs = workloads.sweep_cells(m), workloads.sweep_pml_pairs(m)
+        algo = ALGO_BYTES[args.storage] * icells + AUX_BYTES[args.storage] * pairs
+        ach = algo / (int_avg_ms * 1e-3) / 1e9
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_march (interior z-marching sweep)", interior_cells=icells, avg_launch_ms=round(int_avg_ms, 5),
+                    kernel="k_march (z-marching sweep" + (", inside the PML slabs)" if split == cgfd.SPLIT_SHELL
+                                                          else ", whole planes with the PML slabs)"),
+                    sweep_cells=icells, pml_face_pairs=pairs, algorithmic_bytes_per_launch=algo,
+                    avg_launch_ms=round(int_avg_ms, 5),
                     tile=f"{tile['tx']}x{tile['ty']} threads, {tile['zchunk']} planes",
-                    interior_share_of_step=round(4 * int_avg_ms / step_ms_timed, 4))
-        traffic = measured_traffic(m["name"], args.storage)
+                    sweep_share_of_step=round(4 * int_avg_ms / step_ms_timed, 4))
+        traffic = None if args.plain else measured_traffic(m["name"], args.storage, split)
         if traffic:
             roof["traffic"] = traffic
 
@@ -368,7 +385,8 @@ def main():
                                f"{'curvilinear topography, 3 perturbed layers, double couple, 400 receivers' if m['name'] == 'topo' else 'Cartesian homogeneous, explosion, 20 receivers'}, "
                                f"free surface + 10-layer CFS-PML, dt {m['dt']}",
                    "cells": cells, "storage": args.storage,
-                   "kernel_path": "plain" if args.plain else ("zmarch+pml-march" if args.pml_march else "zmarch"),
+                   "kernel_path": "plain" if args.plain else ("zmarch planes" if split == cgfd.SPLIT_PLANES
+                                                                 else "zmarch + point shell"),
                    "seismograms_finite": finite, "device_ms": round(run_ms, 2)},
         "roofline": roof,
         "reduced_storage": reduced,

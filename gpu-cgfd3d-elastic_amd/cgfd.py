@@ -17,7 +17,9 @@ STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
 DT_F32, DT_F64 = 0, 1
 LEVEL_PRE, LEVEL_TMPA, LEVEL_TMPB, LEVEL_END, LEVEL_RHS = 0, 1, 2, 3, 4
 FLAG_PLAIN = 1
-FLAG_PML_MARCH = 2
+FLAG_PML_POINT = 2
+SPLIT_SHELL = 0
+SPLIT_PLANES = 1
 
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = [
@@ -27,7 +29,7 @@ EXPORTS = [
     "cgfd_upload_receivers", "cgfd_set_level", "cgfd_reset_shot", "cgfd_run",
     "cgfd_el_iso_onestage", "cgfd_read_seismo", "cgfd_fetch_level",
     "cgfd_fetch_pml_aux", "cgfd_set_timing", "cgfd_last_run_timing",
-    "cgfd_set_tile", "cgfd_get_tile", "cgfd_set_graph", "cgfd_set_shot",
+    "cgfd_set_tile", "cgfd_get_tile", "cgfd_get_split", "cgfd_set_graph", "cgfd_set_shot",
     "cgfd_run_async", "cgfd_poll", "cgfd_wait", "cgfd_reserve_receivers",
     "cgfd_grid_cartesian", "cgfd_grid_topography", "cgfd_metric_calc", "cgfd_dt_estimate",
     "cgfd_media_layered", "cgfd_media_homogeneous", "cgfd_rho_to_slow", "cgfd_pml_len_dh",
@@ -156,6 +158,7 @@ def load(path=LIB_PATH):
         "cgfd_wait": ([vp], i),
         "cgfd_set_tile": ([vp, i, i], i),
         "cgfd_get_tile": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)], i),
+        "cgfd_get_split": ([vp, C.POINTER(i)], i),
         "cgfd_last_run_timing": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)], i),
     }
     d = C.c_double
@@ -327,6 +330,12 @@ class Session:
 
     def set_tile(self, shape, zchunk=48):
         _check(_lib.cgfd_set_tile(self.h, shape, zchunk))
+
+    def get_split(self):
+        """SPLIT_PLANES or SPLIT_SHELL: how stages divide between the kernels."""
+        v = C.c_int()
+        _check(_lib.cgfd_get_split(self.h, C.byref(v)))
+        return v.value
 
     def get_tile(self):
         a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()

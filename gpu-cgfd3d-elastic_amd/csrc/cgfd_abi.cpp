@@ -182,6 +182,7 @@ struct cgfd_session {
   // interior tile choice (-1: not tuned yet)
   int tile_shape = -1, tile_zchunk = 48;
   bool tile_fixed = false;
+  int split = -1;  // CGFD_SPLIT_*, chosen by the autotuner (-1: not yet: planes)
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev;
@@ -562,12 +563,15 @@ StageArgs<TS, TC> stage_args(cgfd_session *s, int slot_in, int slot_pre, int slo
   return a;
 }
 
-// Work split of one stage: the z-marching interior box, then the boundary
-// shell (PML face slabs and the 4 free-surface rows) in one per-point launch,
-// which also covers everything for tiny grids or the plain method.  With
-// CGFD_FLAG_PML_MARCH the face slabs are swept by the z-marching PML variant.
-// (Splitting the shell into per-axis single-face launches measured slower:
-// the x slabs are only nlay+1 points wide, and four launches add four tails.)
+// Work split of one stage.  Default: the z-marching kernel sweeps whole x-y
+// planes from the bottom up to the last plane below the free-surface rows,
+// applying the CFS-PML terms inside the face boxes, and one per-point launch
+// takes the 4 free-surface rows.  With CGFD_FLAG_PML_POINT the sweep keeps to
+// the box inside every face box and the per-point launch takes the whole
+// shell (face slabs and surface rows); the per-point launch also covers
+// everything for tiny grids or the plain method.  (Splitting that shell into
+// per-axis single-face launches measured slower: the x slabs are only nlay+1
+// points wide, and four launches add four tails.)
 struct BoxList {
   int n = 0;
   int box[kMaxBox][6];
@@ -583,7 +587,6 @@ struct BoxList {
 
 struct Shell {
   MarchSet inner;
-  MarchSet pml;
   BoxList point;
 };
 
@@ -609,19 +612,14 @@ Shell make_shell(cgfd_session *s) {
   }
   sh.inner.zchunk = s->tile_zchunk;
   sh.inner.shape = s->tile_shape < 0 ? 0 : s->tile_shape;
-  addm(sh.inner, ib0, ib1, jb0, jb1, kb0, kb1);
-  if (s->d.flags & CGFD_FLAG_PML_MARCH) {
-    sh.pml.zchunk = 24;
-    sh.pml.shape = 0;
-    addm(sh.pml, 0, ib0 - 1, 0, nj - 1, 0, ktopm);
-    addm(sh.pml, ib1 + 1, ni - 1, 0, nj - 1, 0, ktopm);
-    addm(sh.pml, ib0, ib1, 0, jb0 - 1, 0, ktopm);
-    addm(sh.pml, ib0, ib1, jb1 + 1, nj - 1, 0, ktopm);
-    addm(sh.pml, ib0, ib1, jb0, jb1, 0, kb0 - 1);
-    addm(sh.pml, ib0, ib1, jb0, jb1, kb1 + 1, ktopm);
+  const bool any_face = ib0 > 0 || ib1 < ni - 1 || jb0 > 0 || jb1 < nj - 1 || kb0 > 0 || kb1 < ktopm;
+  if (any_face && !(s->d.flags & CGFD_FLAG_PML_POINT) && s->split != CGFD_SPLIT_SHELL) {
+    sh.inner.pml = 1;
+    addm(sh.inner, 0, ni - 1, 0, nj - 1, 0, ktopm);
     sh.point.add(0, ni - 1, 0, nj - 1, ktopm + 1, nk - 1);
     return sh;
   }
+  addm(sh.inner, ib0, ib1, jb0, jb1, kb0, kb1);
   sh.point.add(0, ib0 - 1, 0, nj - 1, 0, nk - 1);
   sh.point.add(ib1 + 1, ni - 1, 0, nj - 1, 0, nk - 1);
   sh.point.add(ib0, ib1, 0, jb0 - 1, 0, nk - 1);
@@ -639,8 +637,8 @@ template <typename TS, typename TC> void set_boxes(StageArgs<TS, TC> &a, const B
   }
 }
 
-// one stage over the whole grid: interior sweep (timed when requested), PML
-// slabs, free-surface rows
+// one stage over the whole grid: z-marching sweep (timed when requested), then
+// the per-point boxes
 template <typename TS, typename TC>
 int launch_stage(cgfd_session *s, const Shell &sh, StageArgs<TS, TC> &a, int dx, int dy, int dz, int nshots,
                  hipEvent_t ev0, hipEvent_t ev1) {
@@ -649,10 +647,6 @@ int launch_stage(cgfd_session *s, const Shell &sh, StageArgs<TS, TC> &a, int dx,
     hipError_t e = launch_interior<TS, TC>(a, sh.inner, dx, dy, dz, nshots, s->st);
     if (e != hipSuccess) return fail(CGFD_E_HIP, "interior kernel: %s", hipGetErrorString(e));
     if (ev1) HIPCHK(hipEventRecord(ev1, s->st));
-  }
-  if (sh.pml.nbox) {
-    hipError_t e = launch_pml_march<TS, TC>(a, sh.pml, dx, dy, dz, nshots, s->st);
-    if (e != hipSuccess) return fail(CGFD_E_HIP, "PML sweep kernel: %s", hipGetErrorString(e));
   }
   if (sh.point.n) {
     set_boxes(a, sh.point);
@@ -669,27 +663,33 @@ int launch_stage(cgfd_session *s, const Shell &sh, StageArgs<TS, TC> &a, int dx,
 // others.  The RK coefficients are zero, and every level the sweeps write
 // (TMPA, TMPB, END) is rewritten by the next step before it is read, so
 // tuning between steps leaves the state intact.  Results are cached per
-// (grid, storage, shots) for the life of the process.
+// (grid, storage, shots, work split) for the life of the process.
 struct TileKey {
-  int ni, nj, nk, storage, nshots;
+  int ni, nj, nk, storage, nshots, flags, met_const;
   bool operator<(const TileKey &o) const {
-    return std::tie(ni, nj, nk, storage, nshots) < std::tie(o.ni, o.nj, o.nk, o.storage, o.nshots);
+    return std::tie(ni, nj, nk, storage, nshots, flags, met_const) <
+           std::tie(o.ni, o.nj, o.nk, o.storage, o.nshots, o.flags, o.met_const);
   }
 };
-std::map<TileKey, std::pair<int, int>> g_tile_cache;
+struct TileChoice {
+  int shape, zchunk, split;
+};
+std::map<TileKey, TileChoice> g_tile_cache;
 
 template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
-  const TileKey key{s->d.ni, s->d.nj, s->d.nk, s->d.storage, s->d.nshots};
+  const TileKey key{s->d.ni, s->d.nj, s->d.nk, s->d.storage, s->d.nshots, s->d.flags, s->met_const ? 1 : 0};
   auto hit = g_tile_cache.find(key);
   if (hit != g_tile_cache.end()) {
-    s->tile_shape = hit->second.first;
-    s->tile_zchunk = hit->second.second;
+    s->tile_shape = hit->second.shape;
+    s->tile_zchunk = hit->second.zchunk;
+    s->split = hit->second.split;
     return CGFD_OK;
   }
   s->tile_shape = 0;
-  Shell sh = make_shell(s);
-  if (!sh.inner.nbox) {
-    g_tile_cache[key] = {s->tile_shape, s->tile_zchunk};
+  s->split = CGFD_SPLIT_PLANES;
+  const Shell planes = make_shell(s);
+  if (!planes.inner.nbox) {
+    g_tile_cache[key] = {s->tile_shape, s->tile_zchunk, s->split};
     return CGFD_OK;
   }
   const int PRE = s->slot[0], TA = s->slot[1], TB = s->slot[2], END = s->slot[3];
@@ -705,39 +705,74 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  const int nz = sh.inner.box[0][5] - sh.inner.box[0][4] + 1;
-  const int zcands[4] = {16, 24, 32, 48};
-  float best = 1e30f;
-  int bs = 0, bz = 24;
-  for (int shape = 0; shape < interior_num_shapes(); shape++) {
-    for (int zi = 0; zi < 4; zi++) {
-      int zc = std::min(zcands[zi], nz);
-      if (zi > 0 && zc == std::min(zcands[zi - 1], nz)) continue;
-      MarchSet ib = sh.inner;
-      ib.shape = shape;
-      ib.zchunk = zc;
-      float tmin = 1e30f;
-      for (int rep = 0; rep < 3; rep++) {
-        HIPCHK(hipEventRecord(e0, s->st));
-        for (int q = 0; q < 4; q++) {
-          const int d = (kFlags[0][0] + q) % 2;
-          hipError_t e = launch_interior<TS, TC>(sa[q], ib, d, d, d, s->d.nshots, s->st);
-          if (e != hipSuccess) return fail(CGFD_E_HIP, "autotune launch: %s", hipGetErrorString(e));
-        }
-        HIPCHK(hipEventRecord(e1, s->st));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        if (rep > 0) tmin = std::min(tmin, ms);
+  // minimum over reps 1..2 of one full step's four stages of `launch`
+  auto timed = [&](auto launch, float &tmin) -> int {
+    tmin = 1e30f;
+    for (int rep = 0; rep < 3; rep++) {
+      HIPCHK(hipEventRecord(e0, s->st));
+      for (int q = 0; q < 4; q++) {
+        const int d = (kFlags[0][0] + q) % 2;
+        hipError_t e = launch(q, d);
+        if (e != hipSuccess) return fail(CGFD_E_HIP, "autotune launch: %s", hipGetErrorString(e));
       }
-      if (tmin < best) { best = tmin; bs = shape; bz = zc; }
+      HIPCHK(hipEventRecord(e1, s->st));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (rep > 0) tmin = std::min(tmin, ms);
     }
+    return CGFD_OK;
+  };
+  // Work splits: whole-plane sweep (PML terms inside) and, when the grid has
+  // PML faces, the sweep inside the face boxes with the per-point kernel on
+  // the slabs.  Each is timed with its best tile plus its per-point launch.
+  const bool shell_too = planes.inner.pml && !(s->d.flags & CGFD_FLAG_PML_POINT);
+  float best_total = 1e30f;
+  int bs = 0, bz = 24, bsplit = CGFD_SPLIT_PLANES;
+  for (int split = CGFD_SPLIT_PLANES; split >= (shell_too ? CGFD_SPLIT_SHELL : CGFD_SPLIT_PLANES); split--) {
+    s->split = split;
+    const Shell sh = make_shell(s);
+    const int nz = sh.inner.box[0][5] - sh.inner.box[0][4] + 1;
+    const int zcands[4] = {16, 24, 32, 48};
+    float best = 1e30f;
+    int ts = 0, tz = 24;
+    for (int shape = 0; shape < interior_num_shapes(); shape++) {
+      // split-role and two-column tiles have no PML form (they would rerun
+      // 64 x 8); two-column tiles exist for 16-bit storage only
+      if (sh.inner.pml && shape >= 4) continue;
+      if (shape >= 7 && sizeof(TS) != 2) continue;
+      for (int zi = 0; zi < 4; zi++) {
+        int zc = std::min(zcands[zi], nz);
+        if (zi > 0 && zc == std::min(zcands[zi - 1], nz)) continue;
+        MarchSet ib = sh.inner;
+        ib.shape = shape;
+        ib.zchunk = zc;
+        float t;
+        int r = timed([&](int q, int d) { return launch_interior<TS, TC>(sa[q], ib, d, d, d, s->d.nshots, s->st); },
+                      t);
+        if (r) return r;
+        if (t < best) { best = t; ts = shape; tz = zc; }
+      }
+    }
+    float tp = 0;
+    if (sh.point.n) {
+      int r = timed(
+          [&](int q, int d) {
+            StageArgs<TS, TC> pa = sa[q];
+            set_boxes(pa, sh.point);
+            return launch_point<TS, TC>(pa, d, d, d, s->d.nshots, s->st);
+          },
+          tp);
+      if (r) return r;
+    }
+    if (best + tp < best_total) { best_total = best + tp; bs = ts; bz = tz; bsplit = split; }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   s->tile_shape = bs;
   s->tile_zchunk = bz;
-  g_tile_cache[key] = {bs, bz};
+  s->split = bsplit;
+  g_tile_cache[key] = {bs, bz, bsplit};
   invalidate(s);
   return CGFD_OK;
 }
@@ -1384,6 +1419,12 @@ int cgfd_set_tile(cgfd_session *s, int shape, int zchunk) {
   return CGFD_OK;
 }
 
+int cgfd_get_split(cgfd_session *s, int *split) {
+  if (!s) return fail(CGFD_E_ARG, "null session");
+  if (split) *split = (s->d.flags & CGFD_FLAG_PML_POINT) || s->split == CGFD_SPLIT_SHELL ? CGFD_SPLIT_SHELL
+                                                                                        : CGFD_SPLIT_PLANES;
+  return CGFD_OK;
+}
 int cgfd_get_tile(cgfd_session *s, int *shape, int *tx, int *ty, int *zchunk) {
   if (!s) return fail(CGFD_E_ARG, "null session");
   int sh = s->tile_shape < 0 ? 0 : s->tile_shape, a = 0, b = 0;

@@ -20,15 +20,23 @@
 // On a Cartesian grid every metric component is one constant (CART = true):
 // the nine metric values come in as arguments instead of nine HBM streams.
 //
-// k_march sweeps the box outside every PML face box; the split-role variant
-// below also serves the PML face slabs (CGFD_FLAG_PML_MARCH).  Points within
-// 4 rows of a free surface go to the per-point kernel.
+// The PML variant (PML = true) sweeps whole x-y planes, PML face slabs
+// included: a point inside a face box recomputes the nine derivatives along
+// that face's axis from the same LDS tile or z queue (bit-identical to the
+// sums above it) and applies the CFS-PML ADE terms and memory-variable
+// update.  y and z faces are uniform across a wavefront (a wave spans one
+// or two rows of one plane); only the tiles at the two x ends diverge.  The
+// interior variant (PML = false) sweeps a box outside every face box.
+// Points within 4 rows of a free surface go to the per-point kernel.
 #include "el_iso_device.h"
 #include "el_iso_launch.h"
+#include "el_iso_pml.h"
+
+#include <type_traits>
 
 namespace cgfd {
 
-template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINB, bool CART>
+template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINB, bool CART, bool PML>
 __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, MarchSet b, MarchGrid g) {
   constexpr int NT = TX * TY;
   constexpr int HXL = (DX == 0) ? 3 : 1;
@@ -72,6 +80,18 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
   int kend = kbeg + b.zchunk - 1;
   if (kend > bk1) kend = bk1;
   const bool owner = (i <= bi1) && (j <= bj1);
+  // face boxes this tile's columns and z chunk meet (uniform bit mask): the
+  // PML block runs only on planes that meet one
+  unsigned tile_faces = 0;
+  if constexpr (PML) {
+    const int ti1 = i0 + TX - 1 < bi1 ? i0 + TX - 1 : bi1, tj1 = j0 + TY - 1 < bj1 ? j0 + TY - 1 : bj1;
+#pragma unroll
+    for (int fi = 0; fi < 6; fi++) {
+      const FaceArgs<TS, TC> &F = a.f[fi];
+      if (F.on && F.i1 <= ti1 && F.i2 >= i0 && F.j1 <= tj1 && F.j2 >= j0 && F.k1 <= kend && F.k2 >= kbeg)
+        tile_faces |= 1u << fi;
+    }
+  }
   // lanes past the last ghost column/row only feed LDS cells no owner reads;
   // clamping their address keeps every load unconditional and in bounds
   const int ic = i <= L.ni + 2 ? i : L.ni + 2, jc = j <= L.nj + 2 ? j : L.nj + 2;
@@ -230,6 +250,49 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
       h[TXY] = mu * (Lg[0][1] + Lg[1][0]);
       h[TXZ] = mu * (Lg[0][2] + Lg[2][0]);
       h[TYZ] = mu * (Lg[1][2] + Lg[2][1]);
+      unsigned plane_faces = 0;
+      if constexpr (PML) {
+#pragma unroll
+        for (int fi = 0; fi < 6; fi++)
+          if (((tile_faces >> fi) & 1u) && k >= a.f[fi].k1 && k <= a.f[fi].k2) plane_faces |= 1u << fi;
+      }
+      if (PML && plane_faces) {
+        // Every face box holding the point, in the reference's face order
+        // (sv_curv_col_el_iso_gpu.cu:613-1147).  The compiler-only memory
+        // clobber makes the derivative sums below read LDS again instead of
+        // keeping the 90 x/y tap values of the loop above live until here;
+        // the face loop stays rolled (one copy of its body).
+        __asm__ volatile("" ::: "memory");
+        // one body per face axis: straight-line derivative sums (LDS rows,
+        // LDS columns or the z queue)
+        auto face = [&](auto dimc, int fi) {
+          constexpr int DIM = decltype(dimc)::value;
+          TC Dd[kNC];
+#pragma unroll
+          for (int c = 0; c < kNC; c++) {
+            TC d = TC(0);
+#pragma unroll
+            for (int n = 0; n < 5; n++) {
+              if constexpr (DIM == 0) d += Drp<TC, DX>::c(n) * lds[buf][c][ly + HYL][lx + n];
+              else if constexpr (DIM == 1) d += Drp<TC, DY>::c(n) * lds[buf][c][ly + n][lx + HXL];
+              else d += Drp<TC, DZ>::c(n) * q[c][n];
+            }
+            Dd[c] = d;
+          }
+          const TC e1 = DIM == 0 ? xix : DIM == 1 ? etx : ztx;
+          const TC e2 = DIM == 0 ? xiy : DIM == 1 ? ety : zty;
+          const TC e3 = DIM == 0 ? xiz : DIM == 1 ? etz : ztz;
+          pml_face_march<DIM, TS, TC>(a, fi, shot, i, j, k, Dd, e1, e2, e3, lam, mu, lam2mu, slw, h);
+        };
+#pragma unroll 1
+        for (unsigned fm = plane_faces; fm; fm &= fm - 1) {
+          const int fi = __builtin_ctz(fm);  // uniform: scalar loads of the face's arguments
+          if (!pml_in_face(a, fi, i, j, k)) continue;
+          if (fi < 2) face(std::integral_constant<int, 0>{}, fi);
+          else if (fi < 4) face(std::integral_constant<int, 1>{}, fi);
+          else face(std::integral_constant<int, 2>{}, fi);
+        }
+      }
 #pragma unroll
       for (int c = 0; c < kNC; c++) {
         TS *wout = a.w_out + so + c * cs;
@@ -265,76 +328,7 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
 // k_march (same derivative sums, same order).
 enum SplitRole { ROLE_V = 0, ROLE_T = 1 };
 
-// CFS-PML terms of one role at one point, for every face box containing it
-// (the reference's face order, sv_curv_col_el_iso_gpu.cu:613-1147): R = the
-// part of the role's right-hand side along the face's axis from the raw
-// derivatives Dr[axis][component], rhs += (B-1) R - B Phi, Phi' = D R - A Phi,
-// and the RK update of the role's memory variables.  Points of the sweep lie
-// below the free-surface rows, so the surface term of the x/y faces never
-// applies here.
-template <int ROLE, typename TS, typename TC, int NCG>
-__device__ __forceinline__ void pml_role(const StageArgs<TS, TC> &a, int shot, int i, int j, int k,
-                                         const TC (&Dr)[3][NCG], const TC *mt, TC *h) {
-  constexpr int NOUT = ROLE == ROLE_V ? 3 : 6;
-  constexpr int OOFF = ROLE == ROLE_V ? 0 : 3;
-  auto S = [&](int c) -> TC { return St<TS>::scaled ? (TC)a.sc[c] : TC(1); };
-#pragma unroll
-  for (int fi = 0; fi < 6; fi++) {
-    const FaceArgs<TS, TC> &F = a.f[fi];
-    if (!F.on) continue;
-    if (i < F.i1 || i > F.i2 || j < F.j1 || j > F.j2 || k < F.k1 || k > F.k2) continue;
-    const int dim = fi >> 1;
-    const TC *Dd = Dr[dim];
-    const TC e1 = mt[3 * dim], e2 = mt[3 * dim + 1], e3 = mt[3 * dim + 2];
-    TC R[NOUT];
-    if constexpr (ROLE == ROLE_V) {
-      const TC slw = mt[9];
-      R[0] = slw * (e1 * Dd[TXX - 3] + e2 * Dd[TXY - 3] + e3 * Dd[TXZ - 3]);
-      R[1] = slw * (e1 * Dd[TXY - 3] + e2 * Dd[TYY - 3] + e3 * Dd[TYZ - 3]);
-      R[2] = slw * (e1 * Dd[TXZ - 3] + e2 * Dd[TYZ - 3] + e3 * Dd[TZZ - 3]);
-    } else {
-      const TC lam = mt[9], mu = mt[10];
-      const TC lam2mu = lam + TC(2) * mu;
-      R[TXX - 3] = lam2mu * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam * e3 * Dd[VZ];
-      R[TYY - 3] = lam * e1 * Dd[VX] + lam2mu * e2 * Dd[VY] + lam * e3 * Dd[VZ];
-      R[TZZ - 3] = lam * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam2mu * e3 * Dd[VZ];
-      R[TXY - 3] = mu * (e2 * Dd[VX] + e1 * Dd[VY]);
-      R[TXZ - 3] = mu * (e3 * Dd[VX] + e1 * Dd[VZ]);
-      R[TYZ - 3] = mu * (e3 * Dd[VY] + e2 * Dd[VZ]);
-    }
-    const int ic = (dim == 0) ? i - F.i1 : (dim == 1) ? j - F.j1 : k - F.k1;
-    const unsigned ia = (unsigned)(((k - F.k1) * F.bj + (j - F.j1)) * F.bi + (i - F.i1));
-    const TC cD = F.D[ic], cA = F.A[ic], cB = F.B[ic];
-    const TC Bm1 = cB - TC(1);
-    const long fso = (long)shot * F.shot_stride;
-    const unsigned fs = (unsigned)F.fsize;
-#pragma unroll
-    for (int cg = 0; cg < NOUT; cg++) {
-      const int c = OOFF + cg;
-      const unsigned qb = (c * fs + ia) * (unsigned)sizeof(TS);
-      const TC P = (TC)St<TS>::ld(&at_b(F.a_in + fso, qb)) * S(c);
-      h[cg] += Bm1 * R[cg] - cB * P;
-      const TC Ph = cD * R[cg] - cA * P;
-      const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
-      if (a.mode == MODE_RHS) {
-        St<TS>::st(&at_b(F.a_out + fso, qb), Ph * is);
-      } else if (a.mode == MODE_S0) {
-        St<TS>::st(&at_b(F.a_out + fso, qb), (P + a.ca * Ph) * is);
-        St<TS>::st(&at_b(F.a_end + fso, qb), (P + a.cb * Ph) * is);
-      } else if (a.mode == MODE_S12) {
-        const TC pre = (TC)St<TS>::ld(&at_b(F.a_pre + fso, qb)) * S(c);
-        const TC en = (TC)St<TS>::ld(&at_b(F.a_end + fso, qb)) * S(c);
-        St<TS>::st(&at_b(F.a_out + fso, qb), (pre + a.ca * Ph) * is);
-        St<TS>::st(&at_b(F.a_end + fso, qb), (en + a.cb * Ph) * is);
-      } else {
-        const TC en = (TC)St<TS>::ld(&at_b(F.a_end + fso, qb)) * S(c);
-        St<TS>::st(&at_b(F.a_end + fso, qb), (en + a.cb * Ph) * is);
-      }
-    }
-  }
-}
-
-template <int ROLE, int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, bool CART, bool PML>
+template <int ROLE, int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, bool CART>
 __device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf, int gt, int i0, int j0, int bi1,
                                             int bj1, int kbeg, int kend, int shot) {
   constexpr int NT = TX * TY;
@@ -463,7 +457,6 @@ __device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf
 
     TC Lg[3][3];
     TC dv[3] = {TC(0), TC(0), TC(0)};
-    TC Dr[3][PML ? NCG : 1];  // raw derivatives along x, y, z for the PML terms
 #pragma unroll
     for (int cg = 0; cg < NCG; cg++) {
       const int c = COFF + cg;
@@ -474,7 +467,6 @@ __device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf
         dy += Drp<TC, DY>::c(n) * lb[(c * LH + ly + n) * LW + lx + HXL];
         dz += Drp<TC, DZ>::c(n) * q[cg][n];
       }
-      if constexpr (PML) { Dr[0][cg] = dx; Dr[1][cg] = dy; Dr[2][cg] = dz; }
       const TC g0 = xix * dx + etx * dy + ztx * dz;
       const TC g1 = xiy * dx + ety * dy + zty * dz;
       const TC g2 = xiz * dx + etz * dy + ztz * dz;
@@ -512,7 +504,6 @@ __device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf
         h[TXZ - 3] = mu * (Lg[0][2] + Lg[2][0]);
         h[TYZ - 3] = mu * (Lg[1][2] + Lg[2][1]);
       }
-      if constexpr (PML) pml_role<ROLE, TS, TC, NCG>(a, shot, i, j, k, Dr, mt, h);
 #pragma unroll
       for (int cg = 0; cg < NOUT; cg++) {
         const int c = OOFF + cg;
@@ -536,7 +527,7 @@ __device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf
 }
 
 // MINW: minimum waves per SIMD, which caps the registers per thread (4 waves: 128)
-template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINW, bool CART, bool PML>
+template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINW, bool CART>
 __global__ __launch_bounds__(2 * TX *TY, MINW) void k_march_split(StageArgs<TS, TC> a, MarchSet b, MarchGrid g) {
   constexpr int NT = TX * TY;
   __shared__ TC lds[2][kNC][TY + 4][TX + 4];
@@ -563,14 +554,14 @@ __global__ __launch_bounds__(2 * TX *TY, MINW) void k_march_split(StageArgs<TS, 
   const int tid = threadIdx.x;
   // wave-uniform role: NT is a multiple of the wavefront size
   if (tid < NT)
-    split_sweep<ROLE_V, DX, DY, DZ, TS, TC, TX, TY, CART, PML>(a, &lds[0][0][0][0], tid, i0, j0, b.box[bx][1],
+    split_sweep<ROLE_V, DX, DY, DZ, TS, TC, TX, TY, CART>(a, &lds[0][0][0][0], tid, i0, j0, b.box[bx][1],
                                                            b.box[bx][3], kbeg, kend, shot);
   else
-    split_sweep<ROLE_T, DX, DY, DZ, TS, TC, TX, TY, CART, PML>(a, &lds[0][0][0][0], tid - NT, i0, j0, b.box[bx][1],
+    split_sweep<ROLE_T, DX, DY, DZ, TS, TC, TX, TY, CART>(a, &lds[0][0][0][0], tid - NT, i0, j0, b.box[bx][1],
                                                            b.box[bx][3], kbeg, kend, shot);
 }
 
-template <typename TS, typename TC, int TX, int TY, int MINW, bool CART, bool PML>
+template <typename TS, typename TC, int TX, int TY, int MINW, bool CART>
 static hipError_t launch_split_t(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
                                  hipStream_t st) {
   static_assert((TX * TY) % 64 == 0, "roles must split at a wavefront boundary");
@@ -589,7 +580,7 @@ static hipError_t launch_split_t(const StageArgs<TS, TC> &a, const MarchSet &b, 
   dim3 grid((unsigned)nwg), block(2 * TX * TY);
   const int sel = dx * 4 + dy * 2 + dz;
 #define CGFD_L(X, Y, Z) \
-  hipLaunchKernelGGL((k_march_split<X, Y, Z, TS, TC, TX, TY, MINW, CART, PML>), grid, block, 0, st, a, b, g)
+  hipLaunchKernelGGL((k_march_split<X, Y, Z, TS, TC, TX, TY, MINW, CART>), grid, block, 0, st, a, b, g)
   switch (sel) {
     case 0: CGFD_L(0, 0, 0); break;
     case 1: CGFD_L(0, 0, 1); break;
@@ -604,7 +595,7 @@ static hipError_t launch_split_t(const StageArgs<TS, TC> &a, const MarchSet &b, 
   return hipGetLastError();
 }
 
-template <typename TS, typename TC, int TX, int TY, int MINB, bool CART>
+template <typename TS, typename TC, int TX, int TY, int MINB, bool CART, bool PML>
 static hipError_t launch_march_t(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz,
                                  int nshots, hipStream_t st) {
   MarchGrid g{};
@@ -622,7 +613,7 @@ static hipError_t launch_march_t(const StageArgs<TS, TC> &a, const MarchSet &b, 
   dim3 grid((unsigned)nwg), block(TX * TY);
   const int sel = dx * 4 + dy * 2 + dz;
 #define CGFD_L(X, Y, Z) \
-  hipLaunchKernelGGL((k_march<X, Y, Z, TS, TC, TX, TY, MINB, CART>), grid, block, 0, st, a, b, g)
+  hipLaunchKernelGGL((k_march<X, Y, Z, TS, TC, TX, TY, MINB, CART, PML>), grid, block, 0, st, a, b, g)
   switch (sel) {
     case 0: CGFD_L(0, 0, 0); break;
     case 1: CGFD_L(0, 0, 1); break;
@@ -639,69 +630,69 @@ static hipError_t launch_march_t(const StageArgs<TS, TC> &a, const MarchSet &b, 
 
 // tile shapes the autotuner chooses from (TX x TY threads)
 #if !defined(CGFD_TYPE) || CGFD_TYPE == 0
-// 0..3: one thread per column; 4..6: split roles, two threads per column
-int interior_num_shapes() { return 7; }
+// 0..3: one thread per column; 4..6: split roles, two threads per column;
+// 7..8: two columns per thread (16-bit storage; other storage types run
+// shape 2 instead)
+int interior_num_shapes() { return 9; }
 
 void interior_shape(int id, int *tx, int *ty) {
-  static const int T[7][2] = {{64, 4}, {32, 8}, {64, 8}, {32, 16}, {64, 8}, {32, 8}, {64, 4}};
-  id = id < 0 ? 0 : id > 6 ? 6 : id;
+  static const int T[9][2] = {{64, 4}, {32, 8}, {64, 8}, {32, 16}, {64, 8}, {32, 8}, {64, 4}, {64, 8}, {64, 16}};
+  id = id < 0 ? 0 : id > 8 ? 8 : id;
   *tx = T[id][0];
   *ty = T[id][1];
 }
 #endif
 
-template <typename TS, typename TC>
-hipError_t launch_interior(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
-                           hipStream_t st) {
-  if (b.nbox < 1) return hipSuccess;
-  if (b.shape >= 4) {
+template <typename TS, typename TC, bool PML>
+static hipError_t launch_march_shape(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz,
+                                     int nshots, hipStream_t st) {
+  // split roles have no PML form (each role would carry its own PML state
+  // under a 128-register cap): PML sets run the one-role 64 x 8 tile instead
+  if (!PML && b.shape >= 4 && b.shape <= 6) {
     if (a.met_const) {
       switch (b.shape) {
-        case 5: return launch_split_t<TS, TC, 32, 8, 4, true, false>(a, b, dx, dy, dz, nshots, st);
-        case 6: return launch_split_t<TS, TC, 64, 4, 4, true, false>(a, b, dx, dy, dz, nshots, st);
-        default: return launch_split_t<TS, TC, 64, 8, 1, true, false>(a, b, dx, dy, dz, nshots, st);
+        case 5: return launch_split_t<TS, TC, 32, 8, 4, true>(a, b, dx, dy, dz, nshots, st);
+        case 6: return launch_split_t<TS, TC, 64, 4, 4, true>(a, b, dx, dy, dz, nshots, st);
+        default: return launch_split_t<TS, TC, 64, 8, 1, true>(a, b, dx, dy, dz, nshots, st);
       }
     }
     switch (b.shape) {
-      case 5: return launch_split_t<TS, TC, 32, 8, 4, false, false>(a, b, dx, dy, dz, nshots, st);
-      case 6: return launch_split_t<TS, TC, 64, 4, 4, false, false>(a, b, dx, dy, dz, nshots, st);
-      default: return launch_split_t<TS, TC, 64, 8, 1, false, false>(a, b, dx, dy, dz, nshots, st);
+      case 5: return launch_split_t<TS, TC, 32, 8, 4, false>(a, b, dx, dy, dz, nshots, st);
+      case 6: return launch_split_t<TS, TC, 64, 4, 4, false>(a, b, dx, dy, dz, nshots, st);
+      default: return launch_split_t<TS, TC, 64, 8, 1, false>(a, b, dx, dy, dz, nshots, st);
     }
   }
   if (a.met_const) {
     switch (b.shape) {
-      case 1: return launch_march_t<TS, TC, 32, 8, 2, true>(a, b, dx, dy, dz, nshots, st);
-      case 2: return launch_march_t<TS, TC, 64, 8, 1, true>(a, b, dx, dy, dz, nshots, st);
-      case 3: return launch_march_t<TS, TC, 32, 16, 1, true>(a, b, dx, dy, dz, nshots, st);
-      default: return launch_march_t<TS, TC, 64, 4, 2, true>(a, b, dx, dy, dz, nshots, st);
+      case 0: return launch_march_t<TS, TC, 64, 4, 2, true, PML>(a, b, dx, dy, dz, nshots, st);
+      case 1: return launch_march_t<TS, TC, 32, 8, 2, true, PML>(a, b, dx, dy, dz, nshots, st);
+      case 3: return launch_march_t<TS, TC, 32, 16, 1, true, PML>(a, b, dx, dy, dz, nshots, st);
+      default: return launch_march_t<TS, TC, 64, 8, 1, true, PML>(a, b, dx, dy, dz, nshots, st);
     }
   }
   switch (b.shape) {
-    case 1: return launch_march_t<TS, TC, 32, 8, 2, false>(a, b, dx, dy, dz, nshots, st);
-    case 2: return launch_march_t<TS, TC, 64, 8, 1, false>(a, b, dx, dy, dz, nshots, st);
-    case 3: return launch_march_t<TS, TC, 32, 16, 1, false>(a, b, dx, dy, dz, nshots, st);
-    default: return launch_march_t<TS, TC, 64, 4, 2, false>(a, b, dx, dy, dz, nshots, st);
+    case 0: return launch_march_t<TS, TC, 64, 4, 2, false, PML>(a, b, dx, dy, dz, nshots, st);
+    case 1: return launch_march_t<TS, TC, 32, 8, 2, false, PML>(a, b, dx, dy, dz, nshots, st);
+    case 3: return launch_march_t<TS, TC, 32, 16, 1, false, PML>(a, b, dx, dy, dz, nshots, st);
+    default: return launch_march_t<TS, TC, 64, 8, 1, false, PML>(a, b, dx, dy, dz, nshots, st);
   }
 }
 
-// PML slabs are thin (nlay+1 points) across one axis: a square 16 x 16 tile
-// keeps most lanes busy on x, y and z slabs alike.  Measured slower than the
-// per-point shell kernel on 10-layer slabs (a third of the lanes idle, and the
-// bottom slab too thin for the z queue to pay), hence opt-in.
 template <typename TS, typename TC>
-hipError_t launch_pml_march(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
-                            hipStream_t st) {
+hipError_t launch_interior(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
+                           hipStream_t st) {
   if (b.nbox < 1) return hipSuccess;
-  // split roles keep the PML state per role (about 150 registers each)
-  if (a.met_const) return launch_split_t<TS, TC, 16, 16, 2, true, true>(a, b, dx, dy, dz, nshots, st);
-  return launch_split_t<TS, TC, 16, 16, 2, false, true>(a, b, dx, dy, dz, nshots, st);
+  // the two-column sweep has no PML form either
+  if (b.pml) return launch_march_shape<TS, TC, true>(a, b, dx, dy, dz, nshots, st);
+  if (b.shape >= 7) {
+    if constexpr (sizeof(TS) == 2) return launch_interior_h2<TS>(a, b, dx, dy, dz, nshots, st);
+  }
+  return launch_march_shape<TS, TC, false>(a, b, dx, dy, dz, nshots, st);
 }
 
-#define CGFD_INST(TS, TC)                                                                                     \
-  template hipError_t launch_interior<TS, TC>(const StageArgs<TS, TC> &, const MarchSet &, int, int, int, int,   \
-                                              hipStream_t);                                                   \
-  template hipError_t launch_pml_march<TS, TC>(const StageArgs<TS, TC> &, const MarchSet &, int, int, int, int,  \
-                                               hipStream_t);
+#define CGFD_INST(TS, TC)                                                                                   \
+  template hipError_t launch_interior<TS, TC>(const StageArgs<TS, TC> &, const MarchSet &, int, int, int, int, \
+                                              hipStream_t);
 // CGFD_TYPE selects one storage type per object so the four compile in
 // parallel; without it every type is instantiated here.
 #if !defined(CGFD_TYPE) || CGFD_TYPE == 0

@@ -28,13 +28,14 @@ template <typename T> hipError_t launch_fill(T *p, long n, T v, hipStream_t st);
 
 // Boxes swept by the z-marching kernel ([i0,i1]x[j0,j1]x[k0,k1], physical,
 // inclusive).  Every point must lie at least 4 rows below a free surface.
-// The interior sweep takes one box outside every PML face box; the PML sweep
-// takes the face slabs and applies the CFS-PML terms.
+// With pml set the kernel applies the CFS-PML terms to points inside a face
+// box; without it no point of the set may lie in one.
 struct MarchSet {
   int nbox;
   int box[6][6];
   int zchunk;  // planes per workgroup along z
   int shape;   // tile shape id, see interior_shape()
+  int pml;     // the boxes reach into PML face boxes
 };
 
 // per-box tile counts, filled by the launcher for the chosen tile shape
@@ -50,8 +51,9 @@ template <typename TS, typename TC>
 hipError_t launch_interior(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
                            hipStream_t st);
 
-template <typename TS, typename TC>
-hipError_t launch_pml_march(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
-                            hipStream_t st);
+// two columns per lane, 16-bit storage only (tile shapes 7 and 8)
+template <typename TS>
+hipError_t launch_interior_h2(const StageArgs<TS, float> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
+                              hipStream_t st);
 
 }  // namespace cgfd

@@ -94,6 +94,96 @@ __device__ __forceinline__ void pml_face(const StageArgs<TS, TC> &a, int fi, int
   }
 }
 
+template <typename TS, typename TC>
+__device__ __forceinline__ bool pml_in_face(const StageArgs<TS, TC> &a, int fi, int i, int j, int k) {
+  const FaceArgs<TS, TC> &F = a.f[fi];
+  return F.on && i >= F.i1 && i <= F.i2 && j >= F.j1 && j <= F.j2 && k >= F.k1 && k <= F.k2;
+}
+
+// pml_face for a point off the free-surface row (no surface term), the form
+// the z-marching sweep uses: the face axis DIM is a compile-time constant,
+// every face argument is read once, the RK mode branches once around
+// straight-line component loops, and on z faces the profile index is uniform
+// across the wave, so the three profile values come through scalar loads.
+template <int DIM, typename TS, typename TC>
+__device__ __forceinline__ void pml_face_march(const StageArgs<TS, TC> &a, int fi, int shot, int i, int j, int k,
+                                               const TC *Dd, TC e1, TC e2, TC e3, TC lam, TC mu, TC lam2mu, TC slw,
+                                               TC *h) {
+  const FaceArgs<TS, TC> &F = a.f[fi];
+  const int i1 = F.i1, j1 = F.j1, k1 = F.k1, bi = F.bi, bj = F.bj;
+  const long fso = (long)shot * F.shot_stride;
+  const TS *pin = F.a_in + fso, *ppre = F.a_pre + fso;
+  TS *pout = F.a_out + fso, *pend = F.a_end + fso;
+  const unsigned fs = (unsigned)F.fsize;
+  const unsigned ia = (unsigned)(((k - k1) * bj + (j - j1)) * bi + (i - i1));
+  const int mode = a.mode;
+  auto qb = [&](int c) { return (c * fs + ia) * (unsigned)sizeof(TS); };
+  auto S = [&](int c) -> TC { return St<TS>::scaled ? (TC)a.sc[c] : TC(1); };
+  auto IS = [&](int c) -> TC { return St<TS>::scaled ? (TC)a.isc[c] : TC(1); };
+  TC R[kNC];
+  R[VX] = slw * (e1 * Dd[TXX] + e2 * Dd[TXY] + e3 * Dd[TXZ]);
+  R[VY] = slw * (e1 * Dd[TXY] + e2 * Dd[TYY] + e3 * Dd[TYZ]);
+  R[VZ] = slw * (e1 * Dd[TXZ] + e2 * Dd[TYZ] + e3 * Dd[TZZ]);
+  R[TXX] = lam2mu * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam * e3 * Dd[VZ];
+  R[TYY] = lam * e1 * Dd[VX] + lam2mu * e2 * Dd[VY] + lam * e3 * Dd[VZ];
+  R[TZZ] = lam * e1 * Dd[VX] + lam * e2 * Dd[VY] + lam2mu * e3 * Dd[VZ];
+  R[TXY] = mu * (e2 * Dd[VX] + e1 * Dd[VY]);
+  R[TXZ] = mu * (e3 * Dd[VX] + e1 * Dd[VZ]);
+  R[TYZ] = mu * (e3 * Dd[VY] + e2 * Dd[VZ]);
+  TC P[kNC], pre[kNC], en[kNC];
+#pragma unroll
+  for (int c = 0; c < kNC; c++) P[c] = (TC)St<TS>::ld(&at_b(pin, qb(c))) * S(c);
+  if (mode == MODE_S12) {
+#pragma unroll
+    for (int c = 0; c < kNC; c++) {
+      pre[c] = (TC)St<TS>::ld(&at_b(ppre, qb(c))) * S(c);
+      en[c] = (TC)St<TS>::ld(&at_b(pend, qb(c))) * S(c);
+    }
+  } else if (mode == MODE_S3) {
+#pragma unroll
+    for (int c = 0; c < kNC; c++) en[c] = (TC)St<TS>::ld(&at_b(pend, qb(c))) * S(c);
+  }
+  TC cD, cA, cB;
+  if constexpr (DIM == 2) {
+    typedef const __attribute__((address_space(4))) TC *cptr;
+    const int ic = k - k1;
+    cD = ((cptr)F.D)[ic];
+    cA = ((cptr)F.A)[ic];
+    cB = ((cptr)F.B)[ic];
+  } else {
+    const int ic = DIM == 0 ? i - i1 : j - j1;
+    cD = F.D[ic];
+    cA = F.A[ic];
+    cB = F.B[ic];
+  }
+  const TC Bm1 = cB - TC(1);
+  TC Ph[kNC];
+#pragma unroll
+  for (int c = 0; c < kNC; c++) {
+    h[c] += Bm1 * R[c] - cB * P[c];
+    Ph[c] = cD * R[c] - cA * P[c];
+  }
+  if (mode == MODE_S12) {
+#pragma unroll
+    for (int c = 0; c < kNC; c++) {
+      St<TS>::st(&at_b(pout, qb(c)), (pre[c] + a.ca * Ph[c]) * IS(c));
+      St<TS>::st(&at_b(pend, qb(c)), (en[c] + a.cb * Ph[c]) * IS(c));
+    }
+  } else if (mode == MODE_S0) {
+#pragma unroll
+    for (int c = 0; c < kNC; c++) {
+      St<TS>::st(&at_b(pout, qb(c)), (P[c] + a.ca * Ph[c]) * IS(c));
+      St<TS>::st(&at_b(pend, qb(c)), (P[c] + a.cb * Ph[c]) * IS(c));
+    }
+  } else if (mode == MODE_S3) {
+#pragma unroll
+    for (int c = 0; c < kNC; c++) St<TS>::st(&at_b(pend, qb(c)), (en[c] + a.cb * Ph[c]) * IS(c));
+  } else {
+#pragma unroll
+    for (int c = 0; c < kNC; c++) St<TS>::st(&at_b(pout, qb(c)), Ph[c] * IS(c));
+  }
+}
+
 // Every face box containing the point, in the reference's face order.
 template <typename TS, typename TC>
 __device__ __forceinline__ void pml_apply(const StageArgs<TS, TC> &a, int shot, int i, int j, int k, const TC *Dx,

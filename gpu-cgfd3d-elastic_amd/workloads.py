@@ -110,8 +110,24 @@ def cells(m):
 
 
 def interior_cells(m, free_top=True):
+    """Points of the box inside every PML face box (the sweep's set when the
+    per-point kernel takes the PML slabs)."""
     lay = PML["nlay"] + 1
     return (m["ni"] - 2 * lay) * (m["nj"] - 2 * lay) * (m["nk"] - lay - (4 if free_top else 0))
+
+
+def sweep_cells(m, free_top=True):
+    """Points of the default z-marching sweep: whole planes from the bottom to
+    the last plane below the 4 free-surface rows."""
+    return m["ni"] * m["nj"] * (m["nk"] - (4 if free_top else 0))
+
+
+def sweep_pml_pairs(m, free_top=True):
+    """(point, PML face) pairs inside the default sweep: each carries nine
+    memory variables (faces x1, x2, y1, y2 and z1 of nlay+1 points each)."""
+    lay = PML["nlay"] + 1
+    nz = m["nk"] - (4 if free_top else 0)
+    return 2 * lay * m["nj"] * nz + 2 * lay * m["ni"] * nz + lay * m["ni"] * m["nj"]
 
 
 def shot_requests(m, n, seed, nt_range=(500, 2000), f0_range=(1.0, 4.0), depth_m=(1000.0, 10000.0)):

@@ -70,9 +70,10 @@ typedef struct {
 
 /* use the plain per-point kernel everywhere (no z-marching interior kernel) */
 #define CGFD_FLAG_PLAIN 1
-/* sweep the PML face slabs with the z-marching kernel's PML variant instead of
- * the per-point kernel */
-#define CGFD_FLAG_PML_MARCH 2
+/* PML face slabs by the per-point kernel, the z-marching kernel only on the
+ * box inside them (by default the z-marching kernel sweeps whole planes,
+ * PML included, and the per-point kernel takes the 4 free-surface rows) */
+#define CGFD_FLAG_PML_POINT 2
 
 /* Point-source tables of one shot: the device mirror of src_t
  * (src/forward/src_t.h:14-53, filled by src_read_locate_file src_t.cu:143-790). */
@@ -183,12 +184,21 @@ int cgfd_fetch_subvolume(cgfd_session *s, int shot, int level, unsigned cmp_mask
  * with the face box indexed iz*(nj*ni)+iy*ni+ix (sv_curv_col_el_iso_gpu.cu:775). */
 int cgfd_fetch_pml_aux(cgfd_session *s, int shot, int face, int level, int dtype, void *out);
 
-/* Interior tile of the z-marching kernel: shape id (0..3: 64x4, 32x8, 64x8,
- * 32x16 threads) and z planes per workgroup.  By default the first
- * cgfd_run times every candidate once and keeps the fastest (cached per grid
- * and storage for the process); shape < 0 returns to that automatic choice. */
+/* Tile of the z-marching kernel: shape id (0..3: 64x4, 32x8, 64x8, 32x16
+ * threads, one per column; 4..6: 64x8, 32x8, 64x4 columns with two threads
+ * each (split roles); 7..8: 64x8, 64x16 columns with two columns per thread,
+ * 16-bit storage only) and z planes per workgroup.  By default the first
+ * cgfd_run times every candidate tile under each work split (below) and keeps
+ * the fastest pair (cached per grid, storage, shots and flags for the
+ * process); shape < 0 returns to that automatic choice.  A pinned tile runs
+ * with the whole-plane split unless CGFD_FLAG_PML_POINT is set. */
 int cgfd_set_tile(cgfd_session *s, int shape, int zchunk);
 int cgfd_get_tile(cgfd_session *s, int *shape, int *tx, int *ty, int *zchunk);
+
+/* Work split of a stage between the z-marching and the per-point kernel. */
+#define CGFD_SPLIT_SHELL 0  /* sweep inside the PML face boxes; per-point kernel on the slabs and 4 surface rows */
+#define CGFD_SPLIT_PLANES 1 /* sweep whole planes with the PML terms; per-point kernel on the 4 surface rows */
+int cgfd_get_split(cgfd_session *s, int *split);
 
 /* Enable or disable (default) HIP-event timing of every interior kernel
  * launch.  Timed runs are enqueued launch by launch, without graphs. */

@@ -37,7 +37,7 @@ def _aux_concat(sess, P, level):
     return np.concatenate(parts) if parts else np.zeros(0)
 
 
-@pytest.mark.parametrize("flags", [cgfd.FLAG_PLAIN, 0, cgfd.FLAG_PML_MARCH], ids=["plain", "zmarch", "pmlmarch"])
+@pytest.mark.parametrize("flags", [cgfd.FLAG_PLAIN, 0, cgfd.FLAG_PML_POINT], ids=["plain", "zmarch", "pmlpoint"])
 def test_onestage_rhs_all_pairs(flags):
     """Every (pair, stage) RHS incl. traction image, vlow rows, PML and source."""
     P = Problem(ni=40, nj=34, nk=30, curvilinear=True, pml=(4, 4, 4, 4, 4, 0), hetero=0.05,
@@ -67,7 +67,7 @@ def test_onestage_rhs_all_pairs(flags):
     s.close()
 
 
-@pytest.mark.parametrize("flags", [cgfd.FLAG_PLAIN, 0, cgfd.FLAG_PML_MARCH], ids=["plain", "zmarch", "pmlmarch"])
+@pytest.mark.parametrize("flags", [cgfd.FLAG_PLAIN, 0, cgfd.FLAG_PML_POINT], ids=["plain", "zmarch", "pmlpoint"])
 @pytest.mark.parametrize("curv", [False, True], ids=["cart", "curv"])
 def test_run_matches_oracle_fp32(flags, curv):
     P = Problem(ni=40, nj=36, nk=32, curvilinear=curv, pml=(5, 5, 5, 5, 5, 0), nt_total=40, seed=1,
@@ -135,15 +135,20 @@ def test_multishot_equals_single_shot():
     many.close()
 
 
+@pytest.mark.parametrize("flags", [0, cgfd.FLAG_PML_POINT], ids=["planes", "shell"])
 @pytest.mark.parametrize("shape,zchunk", [(0, 7), (1, 48), (2, 5), (3, 24), (4, 6), (5, 11), (6, 24)])
-def test_every_interior_tile_shape(shape, zchunk):
-    """Each z-marching tile shape and a ragged z chunk reproduce the oracle."""
+def test_every_interior_tile_shape(shape, zchunk, flags):
+    """Each z-marching tile shape and a ragged z chunk reproduce the oracle,
+    under both work splits (a pinned tile runs the whole-plane split unless
+    the per-point shell is asked for; there the split-role shapes 4-6 run as
+    64x8 one-role tiles with the PML terms)."""
     P = Problem(ni=75, nj=41, nk=37, curvilinear=True, pml=(4, 3, 5, 2, 4, 0), nt_total=12, seed=12, hetero=0.04)
-    s = _sess(P)
+    s = _sess(P, flags=flags)
     s.set_tile(shape, zchunk)
     s.run(0, P.nt_total, P.dt)
     t = s.get_tile()
     assert t["shape"] == shape and t["zchunk"] == zchunk
+    assert s.get_split() == (cgfd.SPLIT_SHELL if flags else cgfd.SPLIT_PLANES)
     so, wo, _ = P.oracle_run(32)
     assert rel_l2(s.read_seismo(0), so) <= TOL32
     _cmp_levels(s.fetch_level(0, cgfd.LEVEL_PRE), wo, TOL32, f"shape {shape}")
@@ -151,9 +156,21 @@ def test_every_interior_tile_shape(shape, zchunk):
 
 
 def test_autotune_picks_a_valid_tile():
-    P = Problem(ni=48, nj=40, nk=36, nt_total=4, seed=13)
+    P = Problem(ni=48, nj=40, nk=36, nt_total=4, seed=13, pml=(5, 5, 5, 5, 5, 0))
     s = _sess(P)
     s.run(0, 2, P.dt)
     t = s.get_tile()
-    assert 0 <= t["shape"] < 7 and t["zchunk"] >= 1
+    assert 0 <= t["shape"] < 9 and t["zchunk"] >= 1
+    assert s.get_split() in (cgfd.SPLIT_SHELL, cgfd.SPLIT_PLANES)
+    s.close()
+
+
+def test_autotuned_split_matches_oracle():
+    """Whatever split and tile the autotuner keeps, the run reproduces the oracle."""
+    P = Problem(ni=64, nj=50, nk=40, curvilinear=True, pml=(6, 6, 6, 6, 6, 0), nt_total=10, seed=14, hetero=0.03)
+    s = _sess(P)
+    s.run(0, P.nt_total, P.dt)
+    so, wo, _ = P.oracle_run(32)
+    assert rel_l2(s.read_seismo(0), so) <= TOL32
+    _cmp_levels(s.fetch_level(0, cgfd.LEVEL_PRE), wo, TOL32, f"autotuned split {s.get_split()}")
     s.close()
