@@ -145,7 +145,7 @@ __global__ __launch_bounds__(TXP *TY, MINB) void k_march_h2(StageArgs<TS, float>
   const bool need_end = mode == MODE_S12 || mode == MODE_S3, need_pre = mode == MODE_S12;
   // wavefield and coefficients of the next plane are fetched one plane ahead;
   // the RK levels, needed only in the epilogue, are fetched for the current
-  // plane before its barrier
+  // plane before that prefetch
   unsigned n_lead[kNC], n_h[kNC] = {0};
   float2 n_met[12];
   auto issue = [&](int kk) {
@@ -183,12 +183,15 @@ __global__ __launch_bounds__(TXP *TY, MINB) void k_march_h2(StageArgs<TS, float>
     }
 #pragma unroll
     for (int mm = 0; mm < 12; mm++) mt[mm] = n_met[mm];
-    issue(k < kend ? k + 1 : kend);
+    // this plane's RK levels go out before the next plane's prefetch: memory
+    // counters complete in order, so a load issued after the prefetch could
+    // only be waited for together with all of it
 #pragma unroll
     for (int c = 0; c < kNC; c++) {
       pre_v[c] = need_pre ? ldp(a.w_pre, c, o) : 0u;
       end_v[c] = need_end ? ldp(a.w_end, c, o) : 0u;
     }
+    issue(k < kend ? k + 1 : kend);
     const int buf = (k - kbeg) & 1;
     TC *lb = &lds[buf][0][0][0];
 #pragma unroll
