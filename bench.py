@@ -81,86 +81,108 @@ def measured_traffic(workload, storage, split):
     return None
 
 
-def serving_sample(n, nt_range, load, seed, nseq):
-    """Served stream on the 100x100x60 medium: n seeded shot requests (half
-    FP32, half FP16 storage) arriving as a Poisson stream at `load` times the
-    rate of sequential plain FP32 runs; returns shots/hour both ways and the
-    served latency percentiles."""
-    m = workloads.small()
-    reqs = workloads.shot_requests(m, n, seed, nt_range=nt_range)
-    rng = np.random.default_rng(seed + 1)
+STREAM_SIZES = ((64, 64, 64), (128, 128, 96), (256, 256, 128), (400, 400, 200), (512, 512, 256))
+
+
+def served_stream_sample(n, nt_range, load, seed, nseq_per_size, nrec=20):
+    """Served stream: n seeded shot requests on five resident topography
+    models (64^3, 128x128x96, 256x256x128, 400x400x200, 512x512x256; seeded
+    hills, 3 perturbed layers, 10-layer CFS-PML, free surface), each request
+    on a uniformly drawn model, FP16 storage with probability 1/2, random
+    source and steps in nt_range, arriving as a Poisson stream at `load`
+    times the rate of sequential plain runs.  Sequential plain run = the
+    request alone in an FP32 single-shot session of its model (graphs on),
+    timed on nseq_per_size requests per model and scaled by steps.  Returns
+    shots/hour both ways and the served latency percentiles."""
+    rng = np.random.default_rng(seed)
+    models, reqs = [], []
+    for q, (ni, nj, nk) in enumerate(STREAM_SIZES):
+        m = workloads.topo(seed=seed + 10 * q, nt=nt_range[1], ni=ni, nj=nj, nk=nk)
+        m["recv_indx"], m["recv_frac"] = m["recv_indx"][:nrec], m["recv_frac"][:nrec]
+        models.append(m)
+    pick = rng.integers(0, len(models), n)
     fp16 = rng.random(n) < 0.5
-    ni, nj, nk, dt = m["ni"], m["nj"], m["nk"], m["dt"]
-    arrs = workloads.model_arrays(m)
-    nrec = len(m["recv_indx"])
-    # sequential plain runs: one FP32 single-shot session, model uploaded once
-    sess = cgfd.Session(ni, nj, nk)
-    workloads.upload(sess, m, nt_total=8)
-    sess.run(0, 8, dt)  # autotune and first captures outside the clock
-    alone = {}
-    t0 = time.perf_counter()
-    steps_seq = 0
-    for q in range(nseq):
-        r = reqs[q]
-        sess.reset_shot(0)
-        sess.upload_source(0, r["src"])
-        sess.upload_receivers(0, m["recv_indx"], np.asarray(m["recv_frac"], np.float32), r["nt"])
-        sess.run(0, r["nt"], dt)
-        alone[q] = sess.read_seismo(0)
-        steps_seq += r["nt"]
-    t_seq_sample = time.perf_counter() - t0
-    sess.close()
-    steps_all = sum(r["nt"] for r in reqs)
-    t_seq = t_seq_sample * steps_all / steps_seq
+    per_model = [workloads.shot_requests(m, n, seed + 1000 + q, nt_range=nt_range) for q, m in enumerate(models)]
+    for r in range(n):
+        reqs.append(dict(model=int(pick[r]), fp16=bool(fp16[r]), **per_model[pick[r]][r]))
+    # sequential plain runs: per model, a few requests alone in an FP32 single-shot session
+    step_s, alone = {}, {}
+    for q, m in enumerate(models):
+        sess = cgfd.Session(m["ni"], m["nj"], m["nk"])
+        workloads.upload(sess, m, nt_total=8)
+        sess.run(0, 8, m["dt"])  # autotune and first captures outside the clock
+        mine = [r for r in range(n) if reqs[r]["model"] == q][:nseq_per_size]
+        t_used, steps = 0.0, 0
+        for r in mine:
+            sess.reset_shot(0)
+            sess.upload_source(0, reqs[r]["src"])
+            sess.upload_receivers(0, m["recv_indx"], np.asarray(m["recv_frac"], np.float32), reqs[r]["nt"])
+            t0 = time.perf_counter()
+            sess.run(0, reqs[r]["nt"], m["dt"])
+            sess.read_seismo(0)
+            t_used += time.perf_counter() - t0
+            steps += reqs[r]["nt"]
+            alone[r] = sess.read_seismo(0)
+        sess.close()
+        step_s[q] = t_used / steps if steps else 0.0
+    t_seq = sum(reqs[r]["nt"] * step_s[reqs[r]["model"]] for r in range(n))
     seq_rate = n / t_seq
-    # served: FP32 and FP16 models resident, continuous batching
-    ex = np.max([cgfd.storage_scales(cgfd.STORE_F16, ni, nj, nk, m["metric"], m["lam"], m["mu"], m["slw"],
-                                     r["src"], dt) for r, h in zip(reqs, fp16) if h] or [[0] * 9], axis=0)
-    srv = cgfd.Server(max_batch=32, chunk_steps=16)
-    nt_max = nt_range[1]
-    m32 = srv.add_model(ni, nj, nk, dt=dt, storage=cgfd.STORE_F32, max_nrecv=nrec, max_nt=nt_max, **arrs)
-    m16 = srv.add_model(ni, nj, nk, dt=dt, storage=cgfd.STORE_F16, scale_exp2=ex, max_nrecv=nrec, max_nt=nt_max,
-                        **arrs)
-    # warm both groups (autotune, graph capture) with one short request each
-    for mid in (m32, m16):
-        t = srv.submit(mid, 16, reqs[0]["src"], m["recv_indx"], m["recv_frac"])
+    # served: every model resident in FP32 and FP16 form, continuous batching
+    srv = cgfd.Server(max_batch=16, chunk_steps=16)
+    mid = {}
+    for q, m in enumerate(models):
+        arrs = workloads.model_arrays(m)
+        ex = np.max([cgfd.storage_scales(cgfd.STORE_F16, m["ni"], m["nj"], m["nk"], m["metric"], m["lam"], m["mu"],
+                                         m["slw"], reqs[r]["src"], m["dt"]) for r in range(n)
+                     if reqs[r]["model"] == q and reqs[r]["fp16"]] or [[0] * 9], axis=0)
+        for h in (False, True):
+            mid[(q, h)] = srv.add_model(m["ni"], m["nj"], m["nk"], dt=m["dt"],
+                                        storage=cgfd.STORE_F16 if h else cgfd.STORE_F32,
+                                        scale_exp2=ex if h else None, max_nrecv=nrec, max_nt=nt_range[1], **arrs)
+        del arrs
+    # warm every group (autotune, graph capture) with one short request each
+    for key in mid:
+        m = models[key[0]]
+        t = srv.submit(mid[key], 16, per_model[key[0]][0]["src"], m["recv_indx"], m["recv_frac"])
         srv.wait(t)
         srv.result(t)
     gaps = rng.exponential(1.0 / (load * seq_rate), n)
     tickets = []
     start = time.perf_counter()
     t_next = start
-    for q, r in enumerate(reqs):
-        t_next += gaps[q]
+    for r in range(n):
+        t_next += gaps[r]
         while time.perf_counter() < t_next:
             time.sleep(0.0005)
-        tickets.append(srv.submit(m16 if fp16[q] else m32, r["nt"], r["src"], m["recv_indx"], m["recv_frac"]))
+        m = models[reqs[r]["model"]]
+        tickets.append(srv.submit(mid[(reqs[r]["model"], reqs[r]["fp16"])], reqs[r]["nt"], reqs[r]["src"],
+                                  m["recv_indx"], m["recv_frac"]))
     lat, errs = [], []
-    for q, t in enumerate(tickets):
+    for r, t in enumerate(tickets):
         srv.wait(t)
         seis, _, t_done = srv.result(t)
         lat.append(t_done)
-        if q in alone and not fp16[q]:
-            d = np.linalg.norm(alone[q])
-            errs.append(float(np.linalg.norm(seis - alone[q]) / d) if d > 0 else 0.0)
+        if r in alone and not reqs[r]["fp16"]:
+            d = np.linalg.norm(alone[r])
+            errs.append(float(np.linalg.norm(seis - alone[r]) / d) if d > 0 else 0.0)
     end = time.perf_counter()
     stats = srv.stats()
     srv.close()
-    # throughput of the stream: completions over the span from the first
-    # submission to the last completion (arrivals outpace the server, so the
-    # span measures its capacity, tail included)
     served_rate = n / (end - start)
     lat = np.array(lat)
+    sizes = "/".join(f"{a}x{b}x{c}" for a, b, c in STREAM_SIZES)
     return dict(
-        workload=f"{n} shot requests on the 100x100x60 medium, steps U{list(nt_range)}, f0 U[1,4] Hz, depth "
-                 f"1-10 km clamped, {int(fp16.sum())} FP16-storage + {int((~fp16).sum())} FP32, Poisson arrivals "
-                 f"at {load}x the sequential rate",
+        workload=f"{n} shot requests on 5 resident topography models ({sizes}), model uniform, steps "
+                 f"U{list(nt_range)}, f0 U[1,4] Hz, {int(fp16.sum())} FP16-storage + {int((~fp16).sum())} FP32, "
+                 f"Poisson arrivals at {load}x the sequential rate",
         served_shots_per_hour=round(served_rate * 3600, 1),
         sequential_plain_shots_per_hour=round(seq_rate * 3600, 1),
         speedup=round(served_rate / seq_rate, 3),
         p50_latency_s=round(float(np.percentile(lat, 50)), 4),
         p99_latency_s=round(float(np.percentile(lat, 99)), 4),
-        sequential_sample=f"{nseq} shots timed alone ({steps_seq} steps), scaled to all {steps_all} steps",
+        sequential_sample=f"{nseq_per_size} requests per model timed alone in an FP32 single-shot session, "
+                          f"per-step time scaled to all {sum(r['nt'] for r in reqs)} steps",
+        sequential_ms_per_step={f"{a}x{b}x{c}": round(step_s[q] * 1e3, 4) for q, (a, b, c) in enumerate(STREAM_SIZES)},
         served_vs_alone_max_rel_l2=max(errs) if errs else None,
         groups=stats["groups"], rejected=stats["rejected"])
 
@@ -234,11 +256,11 @@ def main():
                     help="steps of the per-launch event timing pass (0: skip)")
     ap.add_argument("--reduced-steps", type=int, default=200,
                     help="timed steps of the FP16/BF16-storage runs of the same workload (0: skip)")
-    ap.add_argument("--serve-shots", type=int, default=128, help="served-stream sample size (0: skip)")
+    ap.add_argument("--serve-shots", type=int, default=40, help="served-stream sample size (0: skip)")
     ap.add_argument("--batch-shots", type=int, default=64, help="batched-shots sample size (0: skip)")
     ap.add_argument("--batch-steps", default="500,2000", help="step range of the batched shots")
-    ap.add_argument("--serve-steps", default="120,480", help="step range of the served requests")
-    ap.add_argument("--serve-load", type=float, default=8.0, help="arrival rate / sequential plain rate")
+    ap.add_argument("--serve-steps", default="40,160", help="step range of the served requests")
+    ap.add_argument("--serve-load", type=float, default=4.0, help="arrival rate / sequential plain rate")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
@@ -356,7 +378,7 @@ def main():
     if args.serve_shots > 0:
         lo, hi = (int(v) for v in args.serve_steps.split(","))
         log(f"serving sample: {args.serve_shots} shots")
-        serving = serving_sample(args.serve_shots, (lo, hi), args.serve_load, 7, min(8, args.serve_shots))
+        serving = served_stream_sample(args.serve_shots, (lo, hi), args.serve_load, 7, 2)
         log(f"served {serving['served_shots_per_hour']} shots/h vs sequential "
             f"{serving['sequential_plain_shots_per_hour']}; p50 {serving['p50_latency_s']} s p99 "
             f"{serving['p99_latency_s']} s")

@@ -79,7 +79,8 @@ class SnapDesc(C.Structure):
 
 
 class ServerDesc(C.Structure):
-    _fields_ = [("hbm_budget", C.c_longlong), ("max_batch", C.c_int), ("chunk_steps", C.c_int)]
+    _fields_ = [("hbm_budget", C.c_longlong), ("max_batch", C.c_int), ("chunk_steps", C.c_int),
+                ("group_share", C.c_double)]
 
 
 class ModelDesc(C.Structure):
@@ -536,10 +537,10 @@ class Server:
     """Continuous-batching shot server (cgfd_serve.h): models stay resident on
     the device, requests on one model share a batched session."""
 
-    def __init__(self, hbm_budget=0, max_batch=8, chunk_steps=16):
+    def __init__(self, hbm_budget=0, max_batch=8, chunk_steps=16, group_share=0.0):
         load()
         self.h = C.c_void_p()
-        d = ServerDesc(int(hbm_budget), int(max_batch), int(chunk_steps))
+        d = ServerDesc(int(hbm_budget), int(max_batch), int(chunk_steps), float(group_share))
         _check(_lib.cgfd_server_create(C.byref(d), C.byref(self.h)))
         self._meta = {}
 

@@ -170,7 +170,9 @@ Group *cgfd_server::make_group(int model, std::string &why) {
     groups.erase(std::find_if(groups.begin(), groups.end(), [&](auto &p) { return p.get() == victim; }));
   }
   int cap = std::max(1, d.max_batch);
-  while (cap > 1 && group_bytes(m, cap) > budget - used()) cap--;
+  const double share = d.group_share > 0 ? d.group_share : 0.25;
+  const long long room = std::min(budget - used(), (long long)(share * (double)budget));
+  while (cap > 1 && group_bytes(m, cap) > room) cap--;
   cgfd_desc dd{m.d.ni, m.d.nj, m.d.nk, m.d.storage, cap, 0};
   cgfd_session *s = nullptr;
   if (cgfd_create(&dd, &s) != CGFD_OK) {
@@ -322,12 +324,30 @@ void cgfd_server::run() {
       std::lock_guard<std::mutex> lk(mu);
       for (long long t : admitted) pending.erase(std::find(pending.begin(), pending.end(), t));
     }
-    // launch the next chunk of every idle group with shots on board: up to
-    // the first shot that finishes, at most chunk_steps while the group has
-    // free slots (new arrivals wait at most that long) and 8 x chunk_steps
-    // when it is full
-    for (auto &g : groups) {
-      if (g->inflight || g->active() == 0) continue;
+    // Launch the next chunk of idle groups with shots on board: up to the
+    // first shot that finishes, at most chunk_steps while the group has free
+    // slots (new arrivals wait at most that long) and 8 x chunk_steps when it
+    // is full.  A group whose shots fill the device alone (kBigCells active
+    // points or more) runs by itself: overlapping it with another sweep only
+    // thins the L2 reuse of both.  Small groups overlap each other.  Groups
+    // are visited least recently run first, and a big group that is waiting
+    // for the device holds back the groups behind it, so none starves.
+    constexpr double kBigCells = 16e6;
+    auto big = [&](const Group &g) { return (double)npts(models[g.model]->d) * g.active() >= kBigCells; };
+    bool any_run = false, big_run = false;
+    for (auto &g : groups)
+      if (g->inflight) {
+        any_run = true;
+        big_run |= big(*g);
+      }
+    std::vector<Group *> order;
+    for (auto &g : groups)
+      if (!g->inflight && g->active() > 0) order.push_back(g.get());
+    std::stable_sort(order.begin(), order.end(),
+                     [](const Group *a, const Group *b) { return a->last_used < b->last_used; });
+    for (Group *g : order) {
+      if (big_run) break;
+      if (big(*g) && any_run) break;  // waits for the device; later groups wait behind it
       const Model &m = *models[g->model];
       int left = 1 << 30;
       for (const Slot &sl : g->slot)
@@ -354,6 +374,8 @@ void cgfd_server::run() {
       g->clock += len;
       g->inflight = true;
       g->last_used = Clock::now();
+      any_run = true;
+      big_run |= big(*g);
     }
     {
       std::lock_guard<std::mutex> lk(mu);
@@ -380,6 +402,8 @@ int cgfd_server_create(const cgfd_server_desc *desc, cgfd_server **out) {
   if (!desc || !out) return sfail(CGFD_E_ARG, "null argument");
   *out = nullptr;
   if (desc->max_batch < 1 || desc->max_batch > 64) return sfail(CGFD_E_ARG, "max_batch %d not in 1..64", desc->max_batch);
+  if (!(desc->group_share >= 0 && desc->group_share <= 1))
+    return sfail(CGFD_E_ARG, "group_share %g not in [0, 1]", desc->group_share);
   if (desc->chunk_steps < 8 || desc->chunk_steps % 8)
     return sfail(CGFD_E_ARG, "chunk_steps %d must be a positive multiple of 8", desc->chunk_steps);
   int ndev = 0;

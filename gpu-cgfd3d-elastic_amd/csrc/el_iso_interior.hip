@@ -50,15 +50,17 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
   __shared__ TC lds[2][kNC][LH][LW];
 
   // XCD-aware remap: consecutive hardware workgroup ids land on different
-  // XCDs; give each XCD a contiguous range of tiles so halos hit its L2.
-  const int nwg = gridDim.x;
-  const int hw = blockIdx.x;
-  const int q8 = nwg / 8, r8 = nwg % 8;
-  const int xcd = hw % 8, m = hw / 8;
-  const int wg = xcd * q8 + (xcd < r8 ? xcd : r8) + m;
+  // XCDs (round robin).  Each XCD takes a contiguous eighth of every shot's
+  // tiles, shot after shot: halos of neighbouring tiles meet in its L2, and
+  // any subset of active shots keeps all eight XCDs busy.  The grid holds
+  // 8 x part x nshots workgroups; the few past a shot's last tile exit.
   const int per_shot = g.wg_beg[b.nbox];
-  const int shot = wg / per_shot;
-  const int rw = wg - shot * per_shot;
+  const int part = (per_shot + 7) / 8;
+  const int hw = blockIdx.x;
+  const int xcd = hw % 8, m = hw / 8;
+  const int shot = m / part;
+  const int rw = xcd * part + (m - shot * part);
+  if (rw >= per_shot) return;
   int bx = 0;
   while (bx < b.nbox - 1 && rw >= g.wg_beg[bx + 1]) bx++;
   const int lw = rw - g.wg_beg[bx];
@@ -531,14 +533,18 @@ template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int 
 __global__ __launch_bounds__(2 * TX *TY, MINW) void k_march_split(StageArgs<TS, TC> a, MarchSet b, MarchGrid g) {
   constexpr int NT = TX * TY;
   __shared__ TC lds[2][kNC][TY + 4][TX + 4];
-  const int nwg = gridDim.x;
-  const int hw = blockIdx.x;
-  const int q8 = nwg / 8, r8 = nwg % 8;
-  const int xcd = hw % 8, m = hw / 8;
-  const int wg = xcd * q8 + (xcd < r8 ? xcd : r8) + m;
+  // XCD-aware remap: consecutive hardware workgroup ids land on different
+  // XCDs (round robin).  Each XCD takes a contiguous eighth of every shot's
+  // tiles, shot after shot: halos of neighbouring tiles meet in its L2, and
+  // any subset of active shots keeps all eight XCDs busy.  The grid holds
+  // 8 x part x nshots workgroups; the few past a shot's last tile exit.
   const int per_shot = g.wg_beg[b.nbox];
-  const int shot = wg / per_shot;
-  const int rw = wg - shot * per_shot;
+  const int part = (per_shot + 7) / 8;
+  const int hw = blockIdx.x;
+  const int xcd = hw % 8, m = hw / 8;
+  const int shot = m / part;
+  const int rw = xcd * part + (m - shot * part);
+  if (rw >= per_shot) return;
   int bx = 0;
   while (bx < b.nbox - 1 && rw >= g.wg_beg[bx + 1]) bx++;
   const int lw = rw - g.wg_beg[bx];
@@ -575,7 +581,7 @@ static hipError_t launch_split_t(const StageArgs<TS, TC> &a, const MarchSet &b, 
     acc += g.ntx[q] * g.nty[q] * g.nzc[q];
   }
   g.wg_beg[b.nbox] = acc;
-  const long nwg = (long)acc * nshots;
+  const long nwg = 8L * ((acc + 7) / 8) * nshots;  // see the remap at the top of the kernel
   if (nwg == 0) return hipSuccess;
   dim3 grid((unsigned)nwg), block(2 * TX * TY);
   const int sel = dx * 4 + dy * 2 + dz;
@@ -608,7 +614,7 @@ static hipError_t launch_march_t(const StageArgs<TS, TC> &a, const MarchSet &b, 
     acc += g.ntx[q] * g.nty[q] * g.nzc[q];
   }
   g.wg_beg[b.nbox] = acc;
-  const long nwg = (long)acc * nshots;
+  const long nwg = 8L * ((acc + 7) / 8) * nshots;  // see the remap at the top of the kernel
   if (nwg == 0) return hipSuccess;
   dim3 grid((unsigned)nwg), block(TX * TY);
   const int sel = dx * 4 + dy * 2 + dz;

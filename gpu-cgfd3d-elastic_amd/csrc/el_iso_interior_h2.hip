@@ -56,14 +56,18 @@ __global__ __launch_bounds__(TXP *TY, MINB) void k_march_h2(StageArgs<TS, float>
   static_assert(NYP + NXH <= NT, "one halo item per thread and component");
   __shared__ TC lds[2][kNC][LH][LW];
 
-  const int nwg = gridDim.x;
-  const int hw = blockIdx.x;
-  const int q8 = nwg / 8, r8 = nwg % 8;
-  const int xcd = hw % 8, m = hw / 8;
-  const int wg = xcd * q8 + (xcd < r8 ? xcd : r8) + m;
+  // XCD-aware remap: consecutive hardware workgroup ids land on different
+  // XCDs (round robin).  Each XCD takes a contiguous eighth of every shot's
+  // tiles, shot after shot: halos of neighbouring tiles meet in its L2, and
+  // any subset of active shots keeps all eight XCDs busy.  The grid holds
+  // 8 x part x nshots workgroups; the few past a shot's last tile exit.
   const int per_shot = g.wg_beg[b.nbox];
-  const int shot = wg / per_shot;
-  const int rw = wg - shot * per_shot;
+  const int part = (per_shot + 7) / 8;
+  const int hw = blockIdx.x;
+  const int xcd = hw % 8, m = hw / 8;
+  const int shot = m / part;
+  const int rw = xcd * part + (m - shot * part);
+  if (rw >= per_shot) return;
   int bx = 0;
   while (bx < b.nbox - 1 && rw >= g.wg_beg[bx + 1]) bx++;
   const int lw = rw - g.wg_beg[bx];
@@ -328,7 +332,7 @@ static hipError_t launch_h2_t(const StageArgs<TS, float> &a, const MarchSet &b, 
     acc += g.ntx[q] * g.nty[q] * g.nzc[q];
   }
   g.wg_beg[b.nbox] = acc;
-  const long nwg = (long)acc * nshots;
+  const long nwg = 8L * ((acc + 7) / 8) * nshots;  // see the remap at the top of the kernel
   if (nwg == 0) return hipSuccess;
   dim3 grid((unsigned)nwg), block(TXP * TY);
   const int sel = dx * 4 + dy * 2 + dz;

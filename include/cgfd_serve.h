@@ -33,6 +33,8 @@ typedef struct {
   int chunk_steps;      /* a positive multiple of 8: the longest a group with free slots runs between
                            admission points; a full group runs up to its first finishing shot, at
                            most 8 x chunk_steps */
+  double group_share;   /* most of the budget one group may take (a group always gets one shot);
+                           0: 1/4, so the groups of several models stay resident together */
 } cgfd_server_desc;
 
 /* A model: grid, medium, boundaries and time step, arrays in the reference
