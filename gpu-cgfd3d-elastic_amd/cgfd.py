@@ -34,7 +34,7 @@ EXPORTS = [
     "cgfd_grid_cartesian", "cgfd_grid_topography", "cgfd_metric_calc", "cgfd_dt_estimate",
     "cgfd_media_layered", "cgfd_media_homogeneous", "cgfd_rho_to_slow", "cgfd_pml_len_dh",
     "cgfd_pml_profile", "cgfd_free_surface_matrices", "cgfd_wavelet", "cgfd_angle2moment",
-    "cgfd_source_point", "cgfd_receiver_index", "cgfd_storage_scales",
+    "cgfd_source_point", "cgfd_receiver_index", "cgfd_locate_points", "cgfd_storage_scales",
     "cgfd_server_create", "cgfd_server_destroy", "cgfd_server_add_model", "cgfd_server_submit",
     "cgfd_server_wait", "cgfd_server_result", "cgfd_server_get_stats", "cgfd_fetch_subvolume",
     "cgfd_sac_write", "cgfd_station_sac", "cgfd_snap_open", "cgfd_snap_append", "cgfd_snap_close",
@@ -178,6 +178,7 @@ def load(path=LIB_PATH):
         "cgfd_angle2moment": ([d, d, d, vp], i),
         "cgfd_source_point": ([i, i, i, d, d, d, i, d, d, d, i, d, d, d, vp, vp, vp, vp, vp, vp, vp], i),
         "cgfd_receiver_index": ([i, i, i, d, d, d, i, vp, vp], i),
+        "cgfd_locate_points": ([i, i, i, vp, vp, vp, i, vp, vp, vp, vp, vp], i),
         "cgfd_storage_scales": ([i, i, i, i, vp, vp, vp, vp, C.POINTER(SourceDesc), d, vp], i),
         "cgfd_fetch_subvolume": ([vp, i, i, C.c_uint, i, i, i, i, i, i, i, i, i, i, vp], i),
         "cgfd_sac_write": ([C.c_char_p, vp, i, C.c_float, C.c_float, vp, vp, C.c_char_p, C.c_char_p], i),
@@ -517,6 +518,34 @@ def receiver_index(ni, nj, nk, rx, ry, rz, depth_flag):
     fr = _z(3)
     rc = _lib.cgfd_receiver_index(ni, nj, nk, rx, ry, rz, depth_flag, _ptr(ix), _ptr(fr))
     return (ix, fr) if rc == 0 else (None, None)
+
+
+def locate_points(ni, nj, nk, x, y, z, pts, depth_flag=None):
+    """Points given by coordinates (cgfd_locate_points): pts [n][3] = x, y and
+    z or depth (where depth_flag is 1).  Returns (idx [n][3] physical 0-based
+    lower corner, frac [n][3] in [0,1), found [n] bool)."""
+    load()
+    P = np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(-1, 3))
+    n = P.shape[0]
+    idx = np.zeros((n, 3), dtype=np.int32)
+    fr = np.zeros((n, 3))
+    fd = np.zeros(n, dtype=np.int32)
+    xs = [_arr(a, np.float64) for a in (x, y, z)]
+    df = None if depth_flag is None else np.ascontiguousarray(np.broadcast_to(np.asarray(depth_flag, np.int32), (n,)))
+    _check(_lib.cgfd_locate_points(ni, nj, nk, *[_ptr(a) for a in xs], n, _ptr(P), None if df is None else _ptr(df),
+                                   _ptr(idx), _ptr(fr), _ptr(fd)))
+    return idx, fr, fd.astype(bool)
+
+
+def receiver_at(ni, nj, nk, idx, frac):
+    """The 8 trilinear corner offsets and fractions of a receiver at a located
+    point (the form cgfd_upload_receivers takes)."""
+    i, j, k = (int(v) for v in idx)
+    nx, ny = ni + 6, nj + 6
+    p = (k + 3) * nx * ny + (j + 3) * nx + (i + 3)
+    sy, sz = nx, nx * ny
+    ix = np.array([p, p + 1, p + sy, p + 1 + sy, p + sz, p + 1 + sz, p + sy + sz, p + 1 + sy + sz], dtype=np.int64)
+    return ix, np.asarray(frac, dtype=np.float64).copy()
 
 
 def storage_scales(storage, ni, nj, nk, metric, lam, mu, slw, src, dt):

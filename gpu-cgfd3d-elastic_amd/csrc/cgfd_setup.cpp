@@ -538,6 +538,300 @@ int cgfd_receiver_index(int ni, int nj, int nk, double rx, double ry, double rz,
   return CGFD_OK;
 }
 
+// ---- points given by coordinates ---------------------------------------------------
+// Restatement of the reference's coordinate location, in float as there
+// (its grid arrays are float): depth below the free surface -> z on the
+// surface quad holding (x, y) (gd_curv_depth_to_axis, gd_t.cu:1395-1486, with
+// fdlib_math_isPoint2InQuad / fdlib_math_rdinterp_2d, fdlib_math.cu:168-250);
+// nearest grid node by brute force, the first of the 8 cells around it that
+// holds the point (isPointInHexahedron_c, gd_t.cu:1767-1800) and a 101^3
+// trilinear sampling of that cell (gd_curv_coord_to_local_indx and
+// gd_curv_coord2index_sample, gd_t.cu:1241-1392, 1574-1650); negative
+// fractions move to the lower cell (io_recv_read_locate, io_funcs.cu:156-170).
+namespace {
+
+struct FGrid {
+  G3 g;
+  std::vector<float> x, y, z;
+  float xmin, xmax, ymin, ymax, zmin, zmax;
+  // 8x8x8-node blocks with coordinate bounds, for an exact pruned nearest-node search
+  int bx, by, bz;
+  std::vector<float> bb;  // per block: xmin, xmax, ymin, ymax, zmin, zmax
+  FGrid(int ni, int nj, int nk, const double *X, const double *Y, const double *Z) : g(ni, nj, nk) {
+    x.assign(X, X + g.n);
+    y.assign(Y, Y + g.n);
+    z.assign(Z, Z + g.n);
+    xmin = *std::min_element(x.begin(), x.end());
+    xmax = *std::max_element(x.begin(), x.end());
+    ymin = *std::min_element(y.begin(), y.end());
+    ymax = *std::max_element(y.begin(), y.end());
+    zmin = *std::min_element(z.begin(), z.end());
+    zmax = *std::max_element(z.begin(), z.end());
+    bx = (g.nx + 7) / 8;
+    by = (g.ny + 7) / 8;
+    bz = (g.nz + 7) / 8;
+    bb.assign((size_t)bx * by * bz * 6, 0.0f);
+    for (int kb = 0; kb < bz; kb++)
+      for (int jb = 0; jb < by; jb++)
+        for (int ib = 0; ib < bx; ib++) {
+          float m[6] = {1e30f, -1e30f, 1e30f, -1e30f, 1e30f, -1e30f};
+          for (int k = kb * 8; k < std::min(g.nz, kb * 8 + 8); k++)
+            for (int j = jb * 8; j < std::min(g.ny, jb * 8 + 8); j++)
+              for (int i = ib * 8; i < std::min(g.nx, ib * 8 + 8); i++) {
+                const long p = g.at(i, j, k);
+                m[0] = std::min(m[0], x[p]); m[1] = std::max(m[1], x[p]);
+                m[2] = std::min(m[2], y[p]); m[3] = std::max(m[3], y[p]);
+                m[4] = std::min(m[4], z[p]); m[5] = std::max(m[5], z[p]);
+              }
+          std::copy(m, m + 6, &bb[(((size_t)kb * by + jb) * bx + ib) * 6]);
+        }
+  }
+  float dist(float sx, float sy, float sz, long p) const {
+    return sqrtf((sx - x[p]) * (sx - x[p]) + (sy - y[p]) * (sy - y[p]) + (sz - z[p]) * (sz - z[p]));
+  }
+  // the first node (k, j, i order) at the least float distance: blocks in
+  // order of their distance bound, stopping once a bound exceeds the best
+  long nearest(float sx, float sy, float sz) const {
+    const size_t nb = (size_t)bx * by * bz;
+    std::vector<std::pair<double, size_t>> order(nb);
+    for (size_t b = 0; b < nb; b++) {
+      const float *m = &bb[b * 6];
+      auto gap = [](double v, double lo, double hi) { return v < lo ? lo - v : v > hi ? v - hi : 0.0; };
+      const double dx = gap(sx, m[0], m[1]), dy = gap(sy, m[2], m[3]), dz = gap(sz, m[4], m[5]);
+      order[b] = {std::sqrt(dx * dx + dy * dy + dz * dz), b};
+    }
+    std::sort(order.begin(), order.end());
+    float best = dist(sx, sy, sz, 0);
+    long bestp = 0;
+    for (const auto &ob : order) {
+      // float rounding of the node distances: compare with margin
+      if (ob.first > (double)best * (1.0 + 1e-5) + 1e-6) break;
+      const size_t b = ob.second;
+      const int ib = (int)(b % bx), jb = (int)((b / bx) % by), kb = (int)(b / ((size_t)bx * by));
+      for (int k = kb * 8; k < std::min(g.nz, kb * 8 + 8); k++)
+        for (int j = jb * 8; j < std::min(g.ny, jb * 8 + 8); j++)
+          for (int i = ib * 8; i < std::min(g.nx, ib * 8 + 8); i++) {
+            const long p = g.at(i, j, k);
+            const float d = dist(sx, sy, sz, p);
+            if (d < best || (d == best && p < bestp)) {
+              best = d;
+              bestp = p;
+            }
+          }
+    }
+    return bestp;
+  }
+};
+
+float dot3(const float *a, const float *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// the hexahedron test of the reference: the point lies on the inner side of
+// the six faces, each taken as the plane through three of its corners
+bool in_hexahedron(float px, float py, float pz, const float *vx, const float *vy, const float *vz) {
+  static const int F[6][3] = {{0, 4, 6}, {7, 5, 1}, {5, 4, 0}, {2, 6, 7}, {4, 5, 7}, {3, 1, 0}};
+  for (int f = 0; f < 6; f++) {
+    const int a = F[f][0], b = F[f][1], c = F[f][2];
+    const float p2f[3] = {vx[a] - px, vy[a] - py, vz[a] - pz};
+    const float A[3] = {vx[b] - vx[a], vy[b] - vy[a], vz[b] - vz[a]};
+    const float B[3] = {vx[c] - vx[a], vy[c] - vy[a], vz[c] - vz[a]};
+    const float nrm[3] = {A[1] * B[2] - A[2] * B[1], A[2] * B[0] - A[0] * B[2], A[0] * B[1] - A[1] * B[0]};
+    const float len = sqrtf(dot3(nrm, nrm));
+    const float nu[3] = {nrm[0] / len, nrm[1] / len, nrm[2] / len};
+    float sign = dot3(p2f, nu);
+    sign /= sqrtf(dot3(p2f, p2f));
+    if (sign < 0.0f) return false;
+  }
+  return true;
+}
+
+// corner c of the cell whose first node is (i, j, k): c = n1 + 2 n2 + 4 n3
+void cell_corners(const FGrid &G, int i, int j, int k, float *vx, float *vy, float *vz) {
+  for (int c = 0; c < 8; c++) {
+    const long p = G.g.at(i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2));
+    vx[c] = G.x[p];
+    vy[c] = G.y[p];
+    vz[c] = G.z[p];
+  }
+}
+
+// the 101^3 sample of the cell nearest to the point: (n1, n2, n3) / 100
+void cell_sample(float sx, float sy, float sz, const float *vx, const float *vy, const float *vz, int *s1, int *s2,
+                 int *s3) {
+  constexpr int N = 100;
+  float best = sqrtf((sx - vx[0]) * (sx - vx[0]) + (sy - vy[0]) * (sy - vy[0]) + (sz - vz[0]) * (sz - vz[0]));
+  int b1 = 0, b2 = 0, b3 = 0;
+  for (int n3 = 0; n3 <= N; n3++) {
+    float Lz[2];
+    Lz[1] = (float)n3 / (float)N;
+    Lz[0] = (float)(1.0 - Lz[1]);
+    for (int n2 = 0; n2 <= N; n2++) {
+      float Ly[2];
+      Ly[1] = (float)n2 / (float)N;
+      Ly[0] = (float)(1.0 - Ly[1]);
+      for (int n1 = 0; n1 <= N; n1++) {
+        float Lx[2];
+        Lx[1] = (float)n1 / (float)N;
+        Lx[0] = (float)(1.0 - Lx[1]);
+        float xp = 0, yp = 0, zp = 0;
+        for (int c = 0; c < 8; c++) {
+          const float w = Lx[c & 1] * Ly[(c >> 1) & 1] * Lz[c >> 2];
+          xp += w * vx[c];
+          yp += w * vy[c];
+          zp += w * vz[c];
+        }
+        const float d = sqrtf((sx - xp) * (sx - xp) + (sy - yp) * (sy - yp) + (sz - zp) * (sz - zp));
+        if (best > d) {
+          best = d;
+          b1 = n1;
+          b2 = n2;
+          b3 = n3;
+        }
+      }
+    }
+  }
+  *s1 = b1;
+  *s2 = b2;
+  *s3 = b3;
+}
+
+bool in_quad(float px, float py, const float *qx, const float *qy) {
+  // corners 0,1,3,2 walk the quad anticlockwise
+  const float vx[4] = {qx[0], qx[1], qx[3], qx[2]}, vy[4] = {qy[0], qy[1], qy[3], qy[2]};
+  for (int i = 0; i < 4; i++) {
+    const int n = (i + 1) % 4;
+    const float ex = vx[n] - vx[i], ey = vy[n] - vy[i];
+    float sign = ex * (py - vy[i]) - ey * (px - vx[i]);
+    sign /= sqrtf(ex * ex + ey * ey);
+    if (sign < -1e-6) return false;
+  }
+  return true;
+}
+
+// inverse-distance weights over the 4 quad corners; a point on a corner takes
+// that corner's value (the reference reads an unset weight when it is corner 0)
+float idw4(float px, float py, const float *qx, const float *qy, const float *qv) {
+  float w[4], tot = 0.0f;
+  int at = -1;
+  for (int c = 0; c < 4; c++) {
+    const float d = sqrtf((px - qx[c]) * (px - qx[c]) + (py - qy[c]) * (py - qy[c]));
+    if (d < 1e-9) {
+      at = c;
+      w[c] = 0.0f;
+    } else {
+      w[c] = (float)(1.0 / d);
+      tot += w[c];
+    }
+  }
+  if (at >= 0) {
+    for (int c = 0; c < 4; c++) w[c] = 0.0f;
+    w[at] = 1.0f;
+    tot = 1.0f;
+  }
+  float v = 0.0f;
+  for (int c = 0; c < 4; c++) {
+    w[c] = (float)((double)w[c] * (1.0 / (double)tot));
+    v += w[c] * qv[c];
+  }
+  return v;
+}
+
+// z of the free surface under (sx, sy): the first top-plane quad holding it,
+// visited in the reference's tile order (4 x 4 tiles of the physical columns)
+bool surface_z(const FGrid &G, float sx, float sy, float *zs) {
+  const G3 &g = G.g;
+  const int k = NG + g.nk - 1;
+  auto tiles = [](int n, int t) {
+    std::vector<std::pair<int, int>> r(4);
+    int start = NG;
+    for (int q = 0; q < 4; q++) {
+      int len = n / 4 + (q < n % 4 ? 1 : 0);
+      r[q] = {start, start + len - 1};
+      start += len;
+    }
+    (void)t;
+    return r;
+  };
+  const auto ti = tiles(g.ni, 0), tj = tiles(g.nj, 1);
+  for (int jt = 0; jt < 4; jt++)
+    for (int it = 0; it < 4; it++)
+      for (int j = tj[jt].first; j <= tj[jt].second; j++)
+        for (int i = ti[it].first; i <= ti[it].second; i++) {
+          // bounds of the 3-D cell (its upper plane is the ghost plane above)
+          float xlo = 1e30f, xhi = -1e30f, ylo = 1e30f, yhi = -1e30f;
+          for (int c = 0; c < 8; c++) {
+            const long p = g.at(i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2));
+            xlo = std::min(xlo, G.x[p]); xhi = std::max(xhi, G.x[p]);
+            ylo = std::min(ylo, G.y[p]); yhi = std::max(yhi, G.y[p]);
+          }
+          if (sx < xlo || sx > xhi || sy < ylo || sy > yhi) continue;
+          float qx[4], qy[4], qz[4];
+          for (int c = 0; c < 4; c++) {
+            const long p = g.at(i + (c & 1), j + (c >> 1), k);
+            qx[c] = G.x[p];
+            qy[c] = G.y[p];
+            qz[c] = G.z[p];
+          }
+          if (in_quad(sx, sy, qx, qy)) {
+            *zs = idw4(sx, sy, qx, qy, qz);
+            return true;
+          }
+        }
+  return false;
+}
+
+}  // namespace
+
+int cgfd_locate_points(int ni, int nj, int nk, const double *x, const double *y, const double *z, int n,
+                       const double *pts, const int *depth_flag, int *idx, double *frac, int *found) {
+  if (ni < 2 || nj < 2 || nk < 2 || !x || !y || !z || n < 0 || (n > 0 && (!pts || !idx || !frac || !found)))
+    return CGFD_E_ARG;
+  const FGrid G(ni, nj, nk, x, y, z);
+  par_for(n, [&](int q) {
+    float sx = (float)pts[3 * q], sy = (float)pts[3 * q + 1], sz = (float)pts[3 * q + 2];
+    found[q] = 0;
+    for (int d = 0; d < 3; d++) {
+      idx[3 * q + d] = -1000;
+      frac[3 * q + d] = 0.0;
+    }
+    // depth -> z; as in the reference, a point under no surface quad keeps
+    // its depth value as z (and is then normally outside the grid)
+    float zs;
+    if (depth_flag && depth_flag[q] && sx >= G.xmin && sx <= G.xmax && sy >= G.ymin && sy <= G.ymax &&
+        surface_z(G, sx, sy, &zs))
+      sz = zs - sz;
+    if (sx < G.xmin || sx > G.xmax || sy < G.ymin || sy > G.ymax || sz < G.zmin || sz > G.zmax) return;
+    const long p = G.nearest(sx, sy, sz);
+    const int mi = (int)(p % G.g.sy), mj = (int)((p / G.g.sy) % G.g.ny), mk = (int)(p / G.g.sz);
+    if (mi < NG || mi > NG + ni - 1 || mj < NG || mj > NG + nj - 1 || mk < NG || mk > NG + nk - 1) return;
+    found[q] = 1;
+    float inc[3] = {0.0f, 0.0f, 0.0f};
+    for (int c = 0; c < 8; c++) {
+      const int ci = mi - 1 + (c & 1), cj = mj - 1 + ((c >> 1) & 1), ck = mk - 1 + (c >> 2);
+      float vx[8], vy[8], vz[8];
+      cell_corners(G, ci, cj, ck, vx, vy, vz);
+      if (!in_hexahedron(sx, sy, sz, vx, vy, vz)) continue;
+      int s1, s2, s3;
+      cell_sample(sx, sy, sz, vx, vy, vz, &s1, &s2, &s3);
+      inc[0] = ((float)ci + (float)s1 / 100.0f) - (float)mi;
+      inc[1] = ((float)cj + (float)s2 / 100.0f) - (float)mj;
+      inc[2] = ((float)ck + (float)s3 / 100.0f) - (float)mk;
+      break;
+    }
+    const int m[3] = {mi - NG, mj - NG, mk - NG};
+    for (int d = 0; d < 3; d++) {
+      int id = m[d];
+      float f = inc[d];
+      if (f < 0.0f) {
+        f = (float)(1.0 + f);
+        id -= 1;
+      }
+      idx[3 * q + d] = id;
+      frac[3 * q + d] = f;
+    }
+  });
+  return CGFD_OK;
+}
+
 int cgfd_storage_scales(int storage, int ni, int nj, int nk, const double *metric, const double *lam,
                         const double *mu, const double *slw, const cgfd_source_desc *src, double dt, int *exp2) {
   if (!exp2 || ni < 1 || nj < 1 || nk < 1 || !(dt > 0)) return CGFD_E_ARG;

@@ -132,6 +132,20 @@ int cgfd_source_point(int ni, int nj, int nk, double sx, double sy, double sz, i
 
 /* ---- receivers ------------------------------------------------------------------- */
 
+/* Points given by physical coordinates (sources and stations located by
+ * coordinates: src_read_locate_file / io_recv_read_locate with
+ * gd_curv_depth_to_axis and gd_curv_coord_to_glob_indx, gd_t.cu:1190-1486,
+ * 1574-1650, 1767-1800; io_funcs.cu:100-170).  pts[n][3] = x, y and z, or
+ * depth below the free surface where depth_flag[q] is 1 (depth_flag may be
+ * NULL).  For each point found inside the physical grid: idx[q][3] = the
+ * physical 0-based index of the lower cell corner and frac[q][3] in [0,1)
+ * its fractional position (resolution 0.01, from the reference's 101^3
+ * sampling of the cell), found[q] = 1; otherwise found[q] = 0 and idx =
+ * -1000.  Computed in float on float copies of x, y, z, as the reference's
+ * grid arrays are float. */
+int cgfd_locate_points(int ni, int nj, int nk, const double *x, const double *y, const double *z, int n,
+                       const double *pts, const int *depth_flag, int *idx, double *frac, int *found);
+
 /* Receiver given by grid index as in a station file (io_funcs.cu:162-251):
  * x,y 1-based; z depth below the surface (depth_flag 1) or 1-based index.
  * Fills the 8 trilinear corner indices and the fractions; returns
