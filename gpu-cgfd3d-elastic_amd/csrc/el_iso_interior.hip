@@ -109,9 +109,9 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
   };
 
   // this thread's halo cells: compile-time count, positions fixed over the sweep
+  // LDS slot (x16) and component of each halo cell in one register; -1: none
   int h_lds[NPER];
   unsigned h_off[NPER];
-  int h_c[NPER];
 #pragma unroll
   for (int p = 0; p < NPER; p++) {
     const int e = tid + p * NT;
@@ -134,8 +134,7 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
     }
     hi = hi <= L.ni + 2 ? hi : L.ni + 2;
     hj = hj <= L.nj + 2 ? hj : L.nj + 2;
-    h_c[p] = c;
-    h_lds[p] = e < NHT ? (c * LH + lj) * LW + li : -1;
+    h_lds[p] = e < NHT ? ((c * LH + lj) * LW + li) * 16 + c : -1;
     h_off[p] = (unsigned)(hj + kNG) * upx + (unsigned)(hi + L.xa);
   }
   TC *ldsf = &lds[0][0][0][0];
@@ -152,6 +151,9 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
   // latency overlaps the previous plane's arithmetic.
   const int mode = a.mode;
   const bool need_end = mode == MODE_S12 || mode == MODE_S3, need_pre = mode == MODE_S12;
+  // The PML form fetches the RK levels for the current plane (not one plane
+  // ahead): the 18 registers it saves are what its face update needs.
+  constexpr bool LATE_RK = PML;
   TC n_lead[kNC], n_hv[NPER], n_met[12], n_pre[kNC], n_end[kNC];
 #pragma unroll
   for (int c = 0; c < kNC; c++) { n_pre[c] = TC(0); n_end[c] = TC(0); }
@@ -161,14 +163,14 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
 #pragma unroll
     for (int c = 0; c < kNC; c++) n_lead[c] = gld(c, (unsigned)(kk + zlo + 4 + kNG) * upz + col);
 #pragma unroll
-    for (int p = 0; p < NPER; p++) n_hv[p] = gld(h_c[p], pk + h_off[p]);
+    for (int p = 0; p < NPER; p++) n_hv[p] = gld(h_lds[p] < 0 ? kNC - 1 : (h_lds[p] & 15), pk + h_off[p]);
     const unsigned ob = o * (unsigned)sizeof(TC);
 #pragma unroll
     for (int m = 0; m < 9; m++) n_met[m] = CART ? a.cmet[m] : (at_b(a.met + (XIX + m) * cs, ob));
 #pragma unroll
     for (int m = 0; m < 3; m++) n_met[9 + m] = (at_b(a.med + m * cs, ob));
     const unsigned qb = o * (unsigned)sizeof(TS);
-    if (need_end) {
+    if (!LATE_RK && need_end) {
 #pragma unroll
       for (int c = 0; c < kNC; c++) {
         n_end[c] = (TC)St<TS>::ld(&at_b(a.w_end + so + c * cs, qb)) * sc(c);
@@ -195,6 +197,15 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
     for (int p = 0; p < NPER; p++) hv[p] = n_hv[p];
 #pragma unroll
     for (int m = 0; m < 12; m++) mt[m] = n_met[m];
+    if (LATE_RK && need_end) {
+      // this plane's RK levels, issued before the next plane's prefetch
+      // (memory counters complete in order)
+#pragma unroll
+      for (int c = 0; c < kNC; c++) {
+        end_v[c] = (TC)St<TS>::ld(&at_b(a.w_end + so + c * cs, qb)) * sc(c);
+        pre_v[c] = need_pre ? (TC)St<TS>::ld(&at_b(a.w_pre + so + c * cs, qb)) * sc(c) : TC(0);
+      }
+    }
     issue(k < kend ? k + 1 : kend);
     const TC xix = mt[0], xiy = mt[1], xiz = mt[2], etx = mt[3], ety = mt[4], etz = mt[5];
     const TC ztx = mt[6], zty = mt[7], ztz = mt[8], lam = mt[9], mu = mt[10], slw = mt[11];
@@ -204,7 +215,7 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
     for (int c = 0; c < kNC; c++) lds[buf][c][ly + HYL][lx + HXL] = q[c][zc0];
 #pragma unroll
     for (int p = 0; p < NPER; p++)
-      if (h_lds[p] >= 0) ldsf[buf * LBUF + h_lds[p]] = hv[p];
+      if (h_lds[p] >= 0) ldsf[buf * LBUF + (h_lds[p] >> 4)] = hv[p];
     __syncthreads();
 
     // each component's derivatives fold straight into the velocity gradient
