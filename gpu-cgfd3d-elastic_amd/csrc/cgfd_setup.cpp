@@ -1,5 +1,8 @@
 // cgfd_setup.cpp -- FP64 host preparation of the el_iso inputs (see cgfd_setup.h).
 #include <algorithm>
+#include <cctype>
+#include <cfloat>
+#include <cstdlib>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -11,6 +14,7 @@
 
 #include "../../include/cgfd_el_iso.h"
 #include "../../include/cgfd_setup.h"
+#include "cgfd_internal.h"
 
 namespace {
 
@@ -229,6 +233,211 @@ double cgfd_dt_estimate(int ni, int nj, int nk, const double *x, const double *y
     best[k] = b;
   });
   return *std::min_element(best.begin(), best.end());
+}
+
+// ---- media from a layer (md3lay) file ----------------------------------------------
+// Restatement of the reference's layered-model path for elastic_isotropic in
+// float, as there: the interface file reader (read_interface_file,
+// src/media/media_read_file.cpp:22-260) and the "loc" discretization
+// (media_layer2model_el_iso, parametrization_layer_el_iso_loc,
+// AssignLayerMediaPara2Point, CalPointValue_layer, media_layer2model.cpp:
+// 121-171, 401-533, 1313-1362; BilinearInterpolation, findLast/First-
+// GreaterEqualIndex, isEqual, media_utility.cpp:25-180).
+namespace {
+
+struct Md3Lay {
+  int ni = 0, nx = 0, ny = 0;
+  float minx = 0, miny = 0, dx = 0, dy = 0;
+  // [layer][ny][nx]: elevation and rho, vp, vs with their gradient and power
+  std::vector<float> elev, rho, rho_g, rho_p, vp, vp_g, vp_p, vs, vs_g, vs_p;
+};
+
+int md3lay_read(const char *path, Md3Lay &L, std::string &why) {
+  FILE *f = std::fopen(path, "r");
+  if (!f) {
+    why = std::string("cannot open ") + path;
+    return CGFD_E_ARG;
+  }
+  // lines starting with '#' or empty are comments; everything else is
+  // whitespace-separated tokens
+  std::string text;
+  char line[8192];
+  while (std::fgets(line, sizeof line, f)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    text += line;
+    text += ' ';
+  }
+  std::fclose(f);
+  std::vector<std::string> tok;
+  {
+    size_t i = 0;
+    while (i < text.size()) {
+      while (i < text.size() && std::isspace((unsigned char)text[i])) i++;
+      size_t j = i;
+      while (j < text.size() && !std::isspace((unsigned char)text[j])) j++;
+      if (j > i) tok.emplace_back(text.substr(i, j - i));
+      i = j;
+    }
+  }
+  size_t t = 0;
+  auto next = [&](std::string &out) {
+    if (t >= tok.size()) return false;
+    out = tok[t++];
+    return true;
+  };
+  std::string w;
+  if (!next(w)) { why = "no media_type"; return CGFD_E_ARG; }
+  if (w != "elastic_isotropic") {
+    why = "media_type " + w + " is not elastic_isotropic";
+    return CGFD_E_ARG;
+  }
+  auto num_i = [&](int &v) { std::string s; if (!next(s)) return false; v = std::atoi(s.c_str()); return true; };
+  auto num_f = [&](float &v) { std::string s; if (!next(s)) return false; v = std::strtof(s.c_str(), nullptr); return true; };
+  if (!num_i(L.ni) || L.ni < 1) { why = "bad number of interfaces"; return CGFD_E_ARG; }
+  if (!num_i(L.nx) || !num_i(L.ny) || !num_f(L.minx) || !num_f(L.miny) || !num_f(L.dx) || !num_f(L.dy)) {
+    why = "bad interface mesh line";
+    return CGFD_E_ARG;
+  }
+  if (L.nx < 2 || L.ny < 2) { why = "interface mesh needs NX, NY >= 2"; return CGFD_E_ARG; }
+  const size_t n = (size_t)L.ni * L.nx * L.ny;
+  std::vector<float> *cols[10] = {&L.elev, &L.rho, &L.rho_g, &L.rho_p, &L.vp, &L.vp_g, &L.vp_p, &L.vs, &L.vs_g, &L.vs_p};
+  for (auto *c : cols) c->resize(n);
+  for (size_t q = 0; q < n; q++)
+    for (auto *c : cols)
+      if (!num_f((*c)[q])) {
+        why = "insufficient data in the interface file";
+        return CGFD_E_ARG;
+      }
+  return CGFD_OK;
+}
+
+struct LayerEval {
+  const Md3Lay &L;
+  std::vector<float> xv, yv;
+  explicit LayerEval(const Md3Lay &l) : L(l), xv(l.nx), yv(l.ny) {
+    for (int i = 0; i < l.nx; i++) xv[i] = l.minx + l.dx * i;
+    for (int j = 0; j < l.ny; j++) yv[j] = l.miny + l.dy * j;
+  }
+  // bilinear value of one interface's field at (xq, yq); FLT_MAX left of the mesh
+  float bilinear(const float *v, float xq, float yq) const {
+    const int ix = (int)((xq - xv[0]) / (xv[1] - xv[0]));
+    const int iy = (int)((yq - yv[0]) / (yv[1] - yv[0]));
+    if (ix < 0 || iy < 0) return FLT_MAX;
+    const int idx = iy * L.nx + ix;
+    if (ix == L.nx - 1 || iy == L.ny - 1) return v[idx];
+    const float area = (xv[ix + 1] - xv[ix]) * (yv[iy + 1] - yv[iy]);
+    float vq = v[idx] * (xv[ix + 1] - xq) * (yv[iy + 1] - yq) + v[idx + 1] * (xq - xv[ix]) * (yv[iy + 1] - yq) +
+               v[idx + L.nx] * (xv[ix + 1] - xq) * (yq - yv[iy]) + v[idx + 1 + L.nx] * (xq - xv[ix]) * (yq - yv[iy]);
+    return vq / area;
+  }
+};
+
+// the deepest interface at or above z, -1 above the first (elevations listed
+// from the top; ties go to the later interface)
+int last_ge(float value, const std::vector<float> &x) {
+  if (value > x[0]) return -1;
+  float dist = FLT_MAX;
+  int idx = -1;
+  for (size_t i = 0; i < x.size(); i++) {
+    if (x[i] == -FLT_MAX) continue;
+    const float nd = x[i] - value;
+    if (nd < 0) break;
+    if (nd >= 0 && nd <= dist) { dist = nd; idx = (int)i; }
+  }
+  return idx;
+}
+
+int first_ge(float value, const std::vector<float> &x) {
+  float dist = FLT_MAX;
+  int idx = -1;
+  for (size_t i = 0; i < x.size(); i++) {
+    if (x[i] == -FLT_MAX) continue;
+    const float nd = x[i] - value;
+    if (nd >= 0 && nd <= dist) {
+      if (nd <= 1e-6) return (int)i;
+      dist = nd;
+      idx = (int)i;
+    }
+  }
+  return idx;
+}
+
+// The reference's isEqual(a, b) is `abs(a - b) < FLT_EPSILON` with no std::
+// qualification, which binds C's int abs: it holds whenever |a - b| < 1.
+bool ref_is_equal(float a, float b) { return std::abs((int)(a - b)) < FLT_EPSILON; }
+
+}  // namespace
+
+int cgfd_media_layer_file(int ni, int nj, int nk, const double *x, const double *y, const double *z, const char *path,
+                          const char *method, double *lam, double *mu, double *rho) {
+  if (ni < 1 || nj < 1 || nk < 1 || !x || !y || !z || !path || !method || !lam || !mu || !rho) return CGFD_E_ARG;
+  if (std::strcmp(method, "loc") != 0) {
+    cgfd_internal_set_error("equivalent-medium method not supported (only \"loc\")");
+    return CGFD_E_ARG;
+  }
+  Md3Lay L;
+  std::string why;
+  if (md3lay_read(path, L, why) != CGFD_OK) {
+    cgfd_internal_set_error(why.c_str());
+    return CGFD_E_ARG;
+  }
+  const G3 g(ni, nj, nk);
+  const LayerEval E(L);
+  const float maxx = L.minx + (L.nx - 1) * L.dx, maxy = L.miny + (L.ny - 1) * L.dy;
+  const size_t slice = (size_t)L.nx * L.ny;
+  std::vector<int> err(g.nz, 0);
+  par_for(g.nz, [&](int k) {
+    std::vector<float> elev(L.ni);
+    for (int j = 0; j < g.ny && !err[k]; j++)
+      for (int i = 0; i < g.nx; i++) {
+        const long p = g.at(i, j, k);
+        const float ax = (float)x[p], ay = (float)y[p], az = (float)z[p];
+        if (ax < L.minx || ax > maxx || ay < L.miny || ay > maxy) { err[k] = 1; break; }
+        for (int l = 0; l < L.ni; l++) elev[l] = E.bilinear(&L.elev[l * slice], ax, ay);
+        int mi = last_ge(az, elev);
+        auto field = [&](const std::vector<float> &v, int l) { return E.bilinear(&v[l * slice], ax, ay); };
+        float r, vp, vs;
+        if (mi == -1) {
+          // above the top interface: the values at the top of the first layer
+          mi = last_ge(elev[0], elev);
+          r = field(L.rho, mi);
+          vp = field(L.vp, mi);
+          vs = field(L.vs, mi);
+        } else {
+          auto at = [&](int l, float dz, float &ro, float &p_, float &s_) {
+            // pow in double, as the reference's unqualified pow on floats
+            p_ = (float)(field(L.vp, l) + std::pow((double)dz, (double)field(L.vp_p, l)) * field(L.vp_g, l));
+            s_ = (float)(field(L.vs, l) + std::pow((double)dz, (double)field(L.vs_p, l)) * field(L.vs_g, l));
+            ro = (float)(field(L.rho, l) + std::pow((double)dz, (double)field(L.rho_p, l)) * field(L.rho_g, l));
+          };
+          float dz = elev[mi] - az;
+          at(mi, dz, r, vp, vs);
+          if (vp <= vs * std::sqrt(2.0)) { err[k] = 2; break; }
+          if (ref_is_equal(dz, 0.0f)) {
+            // on an interface: the mean of the layers below and above it
+            const int up = first_ge(az, elev) - 1;
+            if (up >= 0) {
+              float r2, p2, s2;
+              at(up, elev[up] - az, r2, p2, s2);
+              r = (float)((r + r2) / 2.0);
+              vp = (float)((vp + p2) / 2.0);
+              vs = (float)((vs + s2) / 2.0);
+            }
+          }
+        }
+        const float m = vs * vs * r;
+        mu[p] = m;
+        lam[p] = (float)((double)(vp * vp * r) - 2.0 * (double)m);
+        rho[p] = r;
+      }
+  });
+  for (int k = 0; k < g.nz; k++)
+    if (err[k]) {
+      cgfd_internal_set_error(err[k] == 1 ? "grid point outside the interface mesh"
+                                          : "vp must be larger than sqrt(2) vs in the layer file");
+      return CGFD_E_ARG;
+    }
+  return CGFD_OK;
 }
 
 int cgfd_media_homogeneous(int ni, int nj, int nk, double vp, double vs, double rho, double *lam, double *mu,
