@@ -65,20 +65,25 @@ def cpu_baseline(m, steps):
                        f"unfused reference stage order, {sec:.1f} s loop")
 
 
-def measured_traffic(workload, storage, split):
+def measured_traffic(workload, storage, split, tile):
     """HBM bytes per sweep launch from the committed counter summary, if any.
 
     The summary holds FETCH_SIZE + WRITE_SIZE per dispatch of the z-marching
     kernel on this workload and work split, scaled by the same counters on a
-    copy kernel of known size."""
+    copy kernel of known size; a line keyed by the tile ("[32x8/16]") wins
+    over an unkeyed one."""
     name = "planes" if split == cgfd.SPLIT_PLANES else "shell"
     path = os.path.join(ROOT, "profiles", f"k_march_{workload}_{storage}_{name}_hbm_counters.txt")
     if not os.path.exists(path):
         return None
+    key = f"[{tile['tx']}x{tile['ty']}/{tile['zchunk']}]"
+    plain = None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch"):
+        if line.startswith("traffic_bytes_per_launch" + key):
             return float(line.split("=")[1])
-    return None
+        if line.startswith("traffic_bytes_per_launch ") or line.startswith("traffic_bytes_per_launch="):
+            plain = float(line.split("=")[1])
+    return plain
 
 
 STREAM_SIZES = ((64, 64, 64), (128, 128, 96), (256, 256, 128), (400, 400, 200), (512, 512, 256))
@@ -347,7 +352,7 @@ def main():
                     avg_launch_ms=round(int_avg_ms, 5),
                     tile=f"{tile['tx']}x{tile['ty']} threads, {tile['zchunk']} planes",
                     sweep_share_of_step=round(4 * int_avg_ms / step_ms_timed, 4))
-        traffic = None if args.plain else measured_traffic(m["name"], args.storage, split)
+        traffic = None if args.plain else measured_traffic(m["name"], args.storage, split, tile)
         if traffic:
             roof["traffic"] = traffic
 
