@@ -368,11 +368,66 @@ bool ref_is_equal(float a, float b) { return std::abs((int)(a - b)) < FLT_EPSILO
 
 }  // namespace
 
+namespace {
+
+// rho, vp, vs of the layered model at one point (AssignLayerMediaPara2Point +
+// CalPointValue_layer for elastic_isotropic); 1: outside the interface mesh,
+// 2: vp <= sqrt(2) vs
+int layer_point(const Md3Lay &L, const LayerEval &E, float ax, float ay, float az, std::vector<float> &elev, float &r,
+                float &vp, float &vs) {
+  const float maxx = L.minx + (L.nx - 1) * L.dx, maxy = L.miny + (L.ny - 1) * L.dy;
+  if (ax < L.minx || ax > maxx || ay < L.miny || ay > maxy) return 1;
+  const size_t slice = (size_t)L.nx * L.ny;
+  for (int l = 0; l < L.ni; l++) elev[l] = E.bilinear(&L.elev[l * slice], ax, ay);
+  int mi = last_ge(az, elev);
+  auto field = [&](const std::vector<float> &v, int l) { return E.bilinear(&v[l * slice], ax, ay); };
+  if (mi == -1) {
+    // above the top interface: the values at the top of the first layer
+    mi = last_ge(elev[0], elev);
+    r = field(L.rho, mi);
+    vp = field(L.vp, mi);
+    vs = field(L.vs, mi);
+    return 0;
+  }
+  auto at = [&](int l, float dz, float &ro, float &p_, float &s_) {
+    // pow in double, as the reference's unqualified pow on floats
+    p_ = (float)(field(L.vp, l) + std::pow((double)dz, (double)field(L.vp_p, l)) * field(L.vp_g, l));
+    s_ = (float)(field(L.vs, l) + std::pow((double)dz, (double)field(L.vs_p, l)) * field(L.vs_g, l));
+    ro = (float)(field(L.rho, l) + std::pow((double)dz, (double)field(L.rho_p, l)) * field(L.rho_g, l));
+  };
+  const float dz = elev[mi] - az;
+  at(mi, dz, r, vp, vs);
+  if (vp <= vs * std::sqrt(2.0)) return 2;
+  if (ref_is_equal(dz, 0.0f)) {
+    // on an interface: the mean of the layers below and above it
+    const int up = first_ge(az, elev) - 1;
+    if (up >= 0) {
+      float r2, p2, s2;
+      at(up, elev[up] - az, r2, p2, s2);
+      r = (float)((r + r2) / 2.0);
+      vp = (float)((vp + p2) / 2.0);
+      vs = (float)((vs + s2) / 2.0);
+    }
+  }
+  return 0;
+}
+
+struct P3 {
+  float x, y, z;
+};
+P3 operator+(P3 a, P3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+P3 operator-(P3 a, P3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+P3 operator*(P3 a, float p) { return {a.x * p, a.y * p, a.z * p}; }
+P3 operator/(P3 a, float p) { return {a.x / p, a.y / p, a.z / p}; }
+
+}  // namespace
+
 int cgfd_media_layer_file(int ni, int nj, int nk, const double *x, const double *y, const double *z, const char *path,
                           const char *method, double *lam, double *mu, double *rho) {
   if (ni < 1 || nj < 1 || nk < 1 || !x || !y || !z || !path || !method || !lam || !mu || !rho) return CGFD_E_ARG;
-  if (std::strcmp(method, "loc") != 0) {
-    cgfd_internal_set_error("equivalent-medium method not supported (only \"loc\")");
+  const bool loc = !std::strcmp(method, "loc"), har = !std::strcmp(method, "har"), ari = !std::strcmp(method, "ari");
+  if (!loc && !har && !ari) {
+    cgfd_internal_set_error("equivalent-medium method must be \"loc\", \"har\" or \"ari\"");
     return CGFD_E_ARG;
   }
   Md3Lay L;
@@ -383,60 +438,139 @@ int cgfd_media_layer_file(int ni, int nj, int nk, const double *x, const double 
   }
   const G3 g(ni, nj, nk);
   const LayerEval E(L);
-  const float maxx = L.minx + (L.nx - 1) * L.dx, maxy = L.miny + (L.ny - 1) * L.dy;
-  const size_t slice = (size_t)L.nx * L.ny;
   std::vector<int> err(g.nz, 0);
+  auto fail = [&]() {
+    for (int k = 0; k < g.nz; k++)
+      if (err[k]) {
+        cgfd_internal_set_error(err[k] == 1 ? "grid point outside the interface mesh"
+                                            : "vp must be larger than sqrt(2) vs in the layer file");
+        return true;
+      }
+    return false;
+  };
+  // "loc": every point's own layer (parametrization_layer_el_iso_loc)
   par_for(g.nz, [&](int k) {
     std::vector<float> elev(L.ni);
     for (int j = 0; j < g.ny && !err[k]; j++)
       for (int i = 0; i < g.nx; i++) {
         const long p = g.at(i, j, k);
-        const float ax = (float)x[p], ay = (float)y[p], az = (float)z[p];
-        if (ax < L.minx || ax > maxx || ay < L.miny || ay > maxy) { err[k] = 1; break; }
-        for (int l = 0; l < L.ni; l++) elev[l] = E.bilinear(&L.elev[l * slice], ax, ay);
-        int mi = last_ge(az, elev);
-        auto field = [&](const std::vector<float> &v, int l) { return E.bilinear(&v[l * slice], ax, ay); };
         float r, vp, vs;
-        if (mi == -1) {
-          // above the top interface: the values at the top of the first layer
-          mi = last_ge(elev[0], elev);
-          r = field(L.rho, mi);
-          vp = field(L.vp, mi);
-          vs = field(L.vs, mi);
-        } else {
-          auto at = [&](int l, float dz, float &ro, float &p_, float &s_) {
-            // pow in double, as the reference's unqualified pow on floats
-            p_ = (float)(field(L.vp, l) + std::pow((double)dz, (double)field(L.vp_p, l)) * field(L.vp_g, l));
-            s_ = (float)(field(L.vs, l) + std::pow((double)dz, (double)field(L.vs_p, l)) * field(L.vs_g, l));
-            ro = (float)(field(L.rho, l) + std::pow((double)dz, (double)field(L.rho_p, l)) * field(L.rho_g, l));
-          };
-          float dz = elev[mi] - az;
-          at(mi, dz, r, vp, vs);
-          if (vp <= vs * std::sqrt(2.0)) { err[k] = 2; break; }
-          if (ref_is_equal(dz, 0.0f)) {
-            // on an interface: the mean of the layers below and above it
-            const int up = first_ge(az, elev) - 1;
-            if (up >= 0) {
-              float r2, p2, s2;
-              at(up, elev[up] - az, r2, p2, s2);
-              r = (float)((r + r2) / 2.0);
-              vp = (float)((vp + p2) / 2.0);
-              vs = (float)((vs + s2) / 2.0);
-            }
-          }
-        }
+        if ((err[k] = layer_point(L, E, (float)x[p], (float)y[p], (float)z[p], elev, r, vp, vs))) break;
         const float m = vs * vs * r;
         mu[p] = m;
         lam[p] = (float)((double)(vp * vp * r) - 2.0 * (double)m);
         rho[p] = r;
       }
   });
-  for (int k = 0; k < g.nz; k++)
-    if (err[k]) {
-      cgfd_internal_set_error(err[k] == 1 ? "grid point outside the interface mesh"
-                                          : "vp must be larger than sqrt(2) vs in the layer file");
-      return CGFD_E_ARG;
-    }
+  if (fail()) return CGFD_E_ARG;
+  if (loc) return CGFD_OK;
+
+  // "har" / "ari" (parametrization_layer_el_iso_har / _ari,
+  // media_layer2model.cpp:1963-2174, with GenerateHalfGrid, MarkInterfaceNumber,
+  // GenerateHalfMesh, MeshSubdivide, NumOfValues, media_utility.cpp:404-600 and
+  // media_layer2model.cpp:1494-1580): points whose dual cell (corners at the
+  // 8 surrounding cell centres) meets more than one layer take the average of
+  // the 9^3 points that subdivide that cell: harmonic for kappa and mu and
+  // arithmetic for rho, or arithmetic for lam, mu and rho.
+  const long n = g.n;
+  std::vector<float> hx(n), hy(n), hz(n);
+  std::vector<int> mat(n);
+  par_for(g.nz, [&](int k) {
+    for (int j = 0; j < g.ny; j++)
+      for (int i = 0; i < g.nx; i++) {
+        const long p = g.at(i, j, k);
+        if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
+          const long q[8] = {p, p + 1, p + g.sy + 1, p + g.sy, p + g.sz, p + g.sz + 1, p + g.sz + g.sy + 1, p + g.sy + g.sz};
+          float sx = 0, sy = 0, sz = 0;
+          for (int c = 0; c < 8; c++) {
+            sx += (float)x[q[c]];
+            sy += (float)y[q[c]];
+            sz += (float)z[q[c]];
+          }
+          hx[p] = (float)(sx / 8.0);
+          hy[p] = (float)(sy / 8.0);
+          hz[p] = (float)(sz / 8.0);
+        } else {
+          hx[p] = (float)x[p];
+          hy[p] = (float)y[p];
+          hz[p] = (float)z[p];
+        }
+      }
+  });
+  par_for(g.nz, [&](int k) {
+    std::vector<float> elev(L.ni);
+    const size_t slice = (size_t)L.nx * L.ny;
+    for (int j = 0; j < g.ny; j++)
+      for (int i = 0; i < g.nx; i++) {
+        const long p = g.at(i, j, k);
+        for (int l = 0; l < L.ni; l++) elev[l] = E.bilinear(&L.elev[l * slice], hx[p], hy[p]);
+        int mi = last_ge(hz[p], elev);
+        if (mi == -1) mi = last_ge(elev[0], elev);
+        mat[p] = mi;
+      }
+  });
+  constexpr int kSub = 8;  // subdivisions per direction of a dual cell
+  par_for(g.ny, [&](int j) {
+    if (j < 1 || j > g.ny - 2) return;
+    std::vector<float> elev(L.ni);
+    std::vector<int> tab(L.ni);
+    for (int k = 1; k < g.nz - 1 && !err[j % g.nz]; k++)
+      for (int i = 1; i < g.nx - 1; i++) {
+        const long p = g.at(i, j, k);
+        const long q[8] = {p - 1 - g.sy - g.sz, p - g.sy - g.sz, p - g.sz, p - 1 - g.sz,
+                           p - 1 - g.sy,        p - g.sy,        p,        p - 1};
+        // distinct layers among the 8 corners (a corner above every interface: none)
+        std::fill(tab.begin(), tab.end(), 0);
+        int distinct = 0;
+        bool above = false;
+        for (int c = 0; c < 8; c++) {
+          const int v = mat[q[c]];
+          if (v == -1) { above = true; break; }
+          if (tab[v]++ == 0) distinct++;
+        }
+        if (above || distinct <= 1) continue;
+        P3 V[8];
+        for (int c = 0; c < 8; c++) V[c] = {hx[q[c]], hy[q[c]], hz[q[c]]};
+        float a_rho = 0, a_lam = 0, a_mu = 0, h_kappa = 0, h_mu = 0;
+        const float ng = (float)kSub;
+        int bad = 0;
+        for (int kk = 0; kk <= kSub && !bad; kk++)
+          for (int jj = 0; jj <= kSub && !bad; jj++)
+            for (int ii = 0; ii <= kSub; ii++) {
+              const P3 lx00 = V[0] + (V[1] - V[0]) / ng * (float)ii;
+              const P3 lx10 = V[3] + (V[2] - V[3]) / ng * (float)ii;
+              const P3 lx01 = V[4] + (V[5] - V[4]) / ng * (float)ii;
+              const P3 lx11 = V[7] + (V[6] - V[7]) / ng * (float)ii;
+              const P3 lxy0 = lx00 + (lx10 - lx00) / ng * (float)jj;
+              const P3 lxy1 = lx01 + (lx11 - lx01) / ng * (float)jj;
+              const P3 s = lxy0 + (lxy1 - lxy0) / ng * (float)kk;
+              float r, vp, vs;
+              if ((bad = layer_point(L, E, s.x, s.y, s.z, elev, r, vp, vs))) break;
+              const float m = vs * vs * r;
+              const float lm = (float)((double)(vp * vp * r) - 2.0 * (double)m);
+              a_rho += r;
+              if (har) {
+                h_kappa = (float)((double)h_kappa + 1.0 / ((double)lm + 2.0 / 3.0 * (double)m));
+                h_mu = (float)((double)h_mu + 1.0 / (double)m);
+              } else {
+                a_lam += lm;
+                a_mu += m;
+              }
+            }
+        if (bad) { err[j % g.nz] = bad; break; }
+        constexpr int nsg = (kSub + 1) * (kSub + 1) * (kSub + 1);
+        if (har) {
+          const float hm = (float)(nsg * 1.0 / (double)h_mu), hk = (float)(nsg * 1.0 / (double)h_kappa);
+          lam[p] = (float)((double)hk - 2.0 / 3.0 * (double)hm);
+          mu[p] = hm;
+        } else {
+          lam[p] = (float)((double)a_lam / (nsg * 1.0));
+          mu[p] = (float)((double)a_mu / (nsg * 1.0));
+        }
+        rho[p] = (float)((double)a_rho / (nsg * 1.0));
+      }
+  });
+  if (fail()) return CGFD_E_ARG;
   return CGFD_OK;
 }
 

@@ -81,12 +81,17 @@ int cgfd_media_layered(int ni, int nj, int nk, const double *x, const double *y,
  * after '#' comment lines, "elastic_isotropic", the number of interfaces NI,
  * "NX NY MINX MINY DX DY", then NI x NY x NX rows of
  * "elevation rho rho_grad rho_pow vp vp_grad vp_pow vs vs_grad vs_pow",
- * interfaces listed from the top.  Method "loc" (the only one here) takes each
- * grid point's own layer: v = v0 + dz^pow * grad with dz the depth below the
- * layer's top interface (media_layer2model.cpp:401-533, 1313-1362), in float
- * as the reference.  Reference quirk kept: a point within 1 m (in z) of an
- * interface counts as on it (its isEqual binds C's int abs) and takes the mean
- * of the two layers.  Fills lam, mu, rho at every point, ghosts included.
+ * interfaces listed from the top.  Method "loc" takes each grid point's own
+ * layer: v = v0 + dz^pow * grad with dz the depth below the layer's top
+ * interface (media_layer2model.cpp:401-533, 1313-1362); "har" and "ari" then
+ * replace, at points whose dual cell (corners at the 8 surrounding cell
+ * centres) meets more than one layer, the values by the average over the 9^3
+ * points subdividing that cell: harmonic for kappa and mu with arithmetic rho
+ * ("har"), or arithmetic for lam, mu and rho ("ari")
+ * (media_layer2model.cpp:1494-1580, 1963-2174).  All in float as the
+ * reference.  Reference quirk kept: a point within 1 m (in z) of an interface
+ * counts as on it (its isEqual binds C's int abs) and takes the mean of the
+ * two layers.  Fills lam, mu, rho at every point, ghosts included.
  * CGFD_E_ARG (message in cgfd_last_error) for an unreadable file, a point
  * outside the interface mesh or vp <= sqrt(2) vs. */
 int cgfd_media_layer_file(int ni, int nj, int nk, const double *x, const double *y, const double *z, const char *path,

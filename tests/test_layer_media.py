@@ -147,6 +147,58 @@ def test_errors(tmp_path):
         cgfd.media_layer_file(ni, nj, nk, x, y, z, small)
     good = write_md3lay(tmp_path / "good.md3lay", [(5000.0, 2670.0, 0, 1, 6000.0, 0, 1, 3464.0, 0, 1)])
     with pytest.raises(cgfd.CgfdError, match="method"):
-        cgfd.media_layer_file(ni, nj, nk, x, y, z, good, method="har")
+        cgfd.media_layer_file(ni, nj, nk, x, y, z, good, method="geo")
     with pytest.raises(cgfd.CgfdError, match="cannot open"):
         cgfd.media_layer_file(ni, nj, nk, x, y, z, tmp_path / "missing.md3lay")
+
+
+def _two_flat_layers(tmp_path, z_if):
+    layers = [(1000.0, 2500.0, 0.0, 1.0, 4000.0, 0.0, 1.0, 2000.0, 0.0, 1.0),
+              (z_if, 2900.0, 0.0, 1.0, 6000.0, 0.0, 1.0, 3400.0, 0.0, 1.0)]
+    return write_md3lay(tmp_path / "two.md3lay", layers)
+
+
+@pytest.mark.parametrize("method", ["ari", "har"])
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_equivalent_medium_flat_interface(tmp_path, impl, method):
+    """Cartesian grid, dz = 100 m, interface at z = -430 m: the node at z = -400
+    has its dual cell over [-450, -350] and 9 sample planes every 12.5 m, of
+    which 2 (-450, -437.5) lie below the interface: weights 7/9 and 2/9."""
+    ni, nj, nk = 6, 5, 10
+    x, y, z = _grid(ni, nj, nk, dz=100.0, ztop=0.0)   # z in [-900, 0]
+    path = _two_flat_layers(tmp_path, -430.0)
+    if impl == "oracle":
+        lam, mu, rho = olm.media_equivalent(ni, nj, nk, x, y, z, path, method)
+    else:
+        lam, mu, rho = cgfd.media_layer_file(ni, nj, nk, x, y, z, path, method=method)
+    k = 3 + 5  # z = -400
+    ra, rb = 2500.0, 2900.0
+    ma, mb = 2000.0 ** 2 * ra, 3400.0 ** 2 * rb
+    la, lb = 4000.0 ** 2 * ra - 2 * ma, 6000.0 ** 2 * rb - 2 * mb
+    wa, wb = 7 / 9, 2 / 9
+    # the 729 sub-point sums accumulate in float, as the reference's: 2e-5 relative
+    assert rho[k, 4, 4] == pytest.approx(wa * ra + wb * rb, rel=2e-5)
+    if method == "ari":
+        assert mu[k, 4, 4] == pytest.approx(wa * ma + wb * mb, rel=2e-5)
+        assert lam[k, 4, 4] == pytest.approx(wa * la + wb * lb, rel=5e-5)
+    else:
+        ka, kb = la + 2 * ma / 3, lb + 2 * mb / 3
+        hm = 1 / (wa / ma + wb / mb)
+        hk = 1 / (wa / ka + wb / kb)
+        assert mu[k, 4, 4] == pytest.approx(hm, rel=2e-5)
+        assert lam[k, 4, 4] == pytest.approx(hk - 2 * hm / 3, rel=5e-5)
+    # a node far from the interface keeps its own layer's values
+    assert rho[3 + 1, 4, 4] == pytest.approx(rb, rel=1e-6) and rho[3 + 8, 4, 4] == pytest.approx(ra, rel=1e-6)
+
+
+@pytest.mark.parametrize("method", ["ari", "har"])
+def test_equivalent_medium_product_equals_oracle(tmp_path, method):
+    ni, nj, nk = 8, 7, 8
+    x, y, z = cgfd.grid_topography(ni, nj, nk, 250.0, 1.5, 2, (200.0, 500.0), (800.0, 1500.0), 30.0, 500.0, 6)
+    path = _topo_file(tmp_path)
+    a = olm.media_equivalent(ni, nj, nk, x, y, z, path, method)
+    b = cgfd.media_layer_file(ni, nj, nk, x, y, z, path, method=method)
+    loc = cgfd.media_layer_file(ni, nj, nk, x, y, z, path)
+    assert (b[2] != loc[2]).sum() > 10  # the interfaces cross the grid
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
