@@ -34,7 +34,8 @@ EXPORTS = [
     "cgfd_grid_cartesian", "cgfd_grid_topography", "cgfd_metric_calc", "cgfd_dt_estimate",
     "cgfd_media_layered", "cgfd_media_homogeneous", "cgfd_rho_to_slow", "cgfd_pml_len_dh",
     "cgfd_pml_profile", "cgfd_free_surface_matrices", "cgfd_wavelet", "cgfd_angle2moment",
-    "cgfd_source_point", "cgfd_receiver_index", "cgfd_locate_points", "cgfd_media_layer_file", "cgfd_storage_scales",
+    "cgfd_source_point", "cgfd_receiver_index", "cgfd_locate_points", "cgfd_media_layer_file", "cgfd_media_grid_file",
+    "cgfd_storage_scales",
     "cgfd_server_create", "cgfd_server_destroy", "cgfd_server_add_model", "cgfd_server_submit",
     "cgfd_server_wait", "cgfd_server_result", "cgfd_server_get_stats", "cgfd_fetch_subvolume",
     "cgfd_sac_write", "cgfd_station_sac", "cgfd_snap_open", "cgfd_snap_append", "cgfd_snap_close",
@@ -180,6 +181,7 @@ def load(path=LIB_PATH):
         "cgfd_receiver_index": ([i, i, i, d, d, d, i, vp, vp], i),
         "cgfd_locate_points": ([i, i, i, vp, vp, vp, i, vp, vp, vp, vp, vp], i),
         "cgfd_media_layer_file": ([i, i, i, vp, vp, vp, C.c_char_p, C.c_char_p, vp, vp, vp], i),
+        "cgfd_media_grid_file": ([i, i, i, vp, vp, vp, C.c_char_p, C.c_char_p, vp, vp, vp], i),
         "cgfd_storage_scales": ([i, i, i, i, vp, vp, vp, vp, C.POINTER(SourceDesc), d, vp], i),
         "cgfd_fetch_subvolume": ([vp, i, i, C.c_uint, i, i, i, i, i, i, i, i, i, i, vp], i),
         "cgfd_sac_write": ([C.c_char_p, vp, i, C.c_float, C.c_float, vp, vp, C.c_char_p, C.c_char_p], i),
@@ -546,6 +548,17 @@ def media_layer_file(ni, nj, nk, x, y, z, path, method="loc"):
     xs = [_arr(a, np.float64) for a in (x, y, z)]
     _check(_lib.cgfd_media_layer_file(ni, nj, nk, *[_ptr(a) for a in xs], str(path).encode(), method.encode(),
                                       _ptr(lam), _ptr(mu), _ptr(rho)))
+    return lam, mu, rho
+
+
+def media_grid_file(ni, nj, nk, x, y, z, path, method="loc"):
+    """lam, mu, rho from a grid media file in the md3grd format (cgfd_media_grid_file)."""
+    load()
+    sh = (nk + 6, nj + 6, ni + 6)
+    lam, mu, rho = _z(sh), _z(sh), _z(sh)
+    xs = [_arr(a, np.float64) for a in (x, y, z)]
+    _check(_lib.cgfd_media_grid_file(ni, nj, nk, *[_ptr(a) for a in xs], str(path).encode(), method.encode(),
+                                     _ptr(lam), _ptr(mu), _ptr(rho)))
     return lam, mu, rho
 
 

@@ -252,14 +252,14 @@ struct Md3Lay {
   std::vector<float> elev, rho, rho_g, rho_p, vp, vp_g, vp_p, vs, vs_g, vs_p;
 };
 
-int md3lay_read(const char *path, Md3Lay &L, std::string &why) {
+// The whitespace-separated tokens of a media file; lines starting with '#'
+// and empty lines are comments.
+int read_tokens(const char *path, std::vector<std::string> &tok, std::string &why) {
   FILE *f = std::fopen(path, "r");
   if (!f) {
     why = std::string("cannot open ") + path;
     return CGFD_E_ARG;
   }
-  // lines starting with '#' or empty are comments; everything else is
-  // whitespace-separated tokens
   std::string text;
   char line[8192];
   while (std::fgets(line, sizeof line, f)) {
@@ -268,17 +268,20 @@ int md3lay_read(const char *path, Md3Lay &L, std::string &why) {
     text += ' ';
   }
   std::fclose(f);
-  std::vector<std::string> tok;
-  {
-    size_t i = 0;
-    while (i < text.size()) {
-      while (i < text.size() && std::isspace((unsigned char)text[i])) i++;
-      size_t j = i;
-      while (j < text.size() && !std::isspace((unsigned char)text[j])) j++;
-      if (j > i) tok.emplace_back(text.substr(i, j - i));
-      i = j;
-    }
+  size_t i = 0;
+  while (i < text.size()) {
+    while (i < text.size() && std::isspace((unsigned char)text[i])) i++;
+    size_t j = i;
+    while (j < text.size() && !std::isspace((unsigned char)text[j])) j++;
+    if (j > i) tok.emplace_back(text.substr(i, j - i));
+    i = j;
   }
+  return CGFD_OK;
+}
+
+int md3lay_read(const char *path, Md3Lay &L, std::string &why) {
+  std::vector<std::string> tok;
+  if (read_tokens(path, tok, why) != CGFD_OK) return CGFD_E_ARG;
   size_t t = 0;
   auto next = [&](std::string &out) {
     if (t >= tok.size()) return false;
@@ -311,23 +314,25 @@ int md3lay_read(const char *path, Md3Lay &L, std::string &why) {
   return CGFD_OK;
 }
 
+// Bilinear evaluation on an interface mesh of nx x ny nodes from (minx, miny)
+// with spacing (dx, dy) (BilinearInterpolation, media_utility.cpp).
 struct LayerEval {
-  const Md3Lay &L;
+  int nx, ny;
   std::vector<float> xv, yv;
-  explicit LayerEval(const Md3Lay &l) : L(l), xv(l.nx), yv(l.ny) {
-    for (int i = 0; i < l.nx; i++) xv[i] = l.minx + l.dx * i;
-    for (int j = 0; j < l.ny; j++) yv[j] = l.miny + l.dy * j;
+  LayerEval(int nx_, int ny_, float minx, float miny, float dx, float dy) : nx(nx_), ny(ny_), xv(nx_), yv(ny_) {
+    for (int i = 0; i < nx; i++) xv[i] = minx + dx * i;
+    for (int j = 0; j < ny; j++) yv[j] = miny + dy * j;
   }
   // bilinear value of one interface's field at (xq, yq); FLT_MAX left of the mesh
   float bilinear(const float *v, float xq, float yq) const {
     const int ix = (int)((xq - xv[0]) / (xv[1] - xv[0]));
     const int iy = (int)((yq - yv[0]) / (yv[1] - yv[0]));
     if (ix < 0 || iy < 0) return FLT_MAX;
-    const int idx = iy * L.nx + ix;
-    if (ix == L.nx - 1 || iy == L.ny - 1) return v[idx];
+    const int idx = iy * nx + ix;
+    if (ix == nx - 1 || iy == ny - 1) return v[idx];
     const float area = (xv[ix + 1] - xv[ix]) * (yv[iy + 1] - yv[iy]);
     float vq = v[idx] * (xv[ix + 1] - xq) * (yv[iy + 1] - yq) + v[idx + 1] * (xq - xv[ix]) * (yv[iy + 1] - yq) +
-               v[idx + L.nx] * (xv[ix + 1] - xq) * (yq - yv[iy]) + v[idx + 1 + L.nx] * (xq - xv[ix]) * (yq - yv[iy]);
+               v[idx + nx] * (xv[ix + 1] - xq) * (yq - yv[iy]) + v[idx + 1 + nx] * (xq - xv[ix]) * (yq - yv[iy]);
     return vq / area;
   }
 };
@@ -422,6 +427,261 @@ P3 operator/(P3 a, float p) { return {a.x / p, a.y / p, a.z / p}; }
 
 }  // namespace
 
+extern "C++" {
+namespace {
+
+// The equivalent-medium pass shared by layer and grid media files ("har" /
+// "ari"; parametrization_{layer,grid}_el_iso_{har,ari},
+// media_layer2model.cpp:1963-2174, media_grid2model.cpp:1819-2079, with
+// GenerateHalfGrid, GenerateHalfMesh, MeshSubdivide, NumOfValues,
+// media_utility.cpp:404-600): cell centres of the grid (the mean of each
+// cell's 8 nodes; the last plane of each axis keeps the node) are marked with
+// a material number; a node whose dual cell (corners at the 8 surrounding
+// centres) meets more than one material takes the average of the 9^3 points
+// that subdivide that cell: harmonic for kappa and mu with arithmetic rho, or
+// arithmetic for lam, mu and rho.  Grid media files (grid_rules) leave out
+// the points on a layer boundary and divide the harmonic rho sum in float.
+template <typename Mark, typename Point>
+int equivalent_medium(const G3 &g, const double *x, const double *y, const double *z, bool har, int nmat,
+                      bool grid_rules, std::vector<int> &err, Mark mark, Point point, int nelev, double *lam,
+                      double *mu, double *rho) {
+  const long n = g.n;
+  std::vector<float> hx(n), hy(n), hz(n);
+  std::vector<int> mat(n);
+  par_for(g.nz, [&](int k) {
+    for (int j = 0; j < g.ny; j++)
+      for (int i = 0; i < g.nx; i++) {
+        const long p = g.at(i, j, k);
+        if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
+          const long q[8] = {p, p + 1, p + g.sy + 1, p + g.sy, p + g.sz, p + g.sz + 1, p + g.sz + g.sy + 1, p + g.sy + g.sz};
+          float sx = 0, sy = 0, sz = 0;
+          for (int c = 0; c < 8; c++) {
+            sx += (float)x[q[c]];
+            sy += (float)y[q[c]];
+            sz += (float)z[q[c]];
+          }
+          hx[p] = (float)(sx / 8.0);
+          hy[p] = (float)(sy / 8.0);
+          hz[p] = (float)(sz / 8.0);
+        } else {
+          hx[p] = (float)x[p];
+          hy[p] = (float)y[p];
+          hz[p] = (float)z[p];
+        }
+      }
+  });
+  par_for(g.nz, [&](int k) {
+    std::vector<float> elev(nelev);
+    for (int j = 0; j < g.ny; j++)
+      for (int i = 0; i < g.nx; i++) {
+        const long p = g.at(i, j, k);
+        mat[p] = mark(hx[p], hy[p], hz[p], elev);
+      }
+  });
+  constexpr int kSub = 8;  // subdivisions per direction of a dual cell
+  std::vector<int> jerr(g.ny, 0);
+  par_for(g.ny, [&](int j) {
+    if (j < 1 || j > g.ny - 2) return;
+    std::vector<float> elev(nelev);
+    std::vector<int> tab(nmat);
+    for (int k = 1; k < g.nz - 1 && !jerr[j]; k++)
+      for (int i = 1; i < g.nx - 1; i++) {
+        const long p = g.at(i, j, k);
+        const long q[8] = {p - 1 - g.sy - g.sz, p - g.sy - g.sz, p - g.sz, p - 1 - g.sz,
+                           p - 1 - g.sy,        p - g.sy,        p,        p - 1};
+        // distinct materials among the 8 corners (a corner above every interface: none)
+        std::fill(tab.begin(), tab.end(), 0);
+        int distinct = 0;
+        bool above = false;
+        for (int c = 0; c < 8; c++) {
+          const int v = mat[q[c]];
+          if (v == -1) { above = true; break; }
+          if (tab[v]++ == 0) distinct++;
+        }
+        if (above || distinct <= 1) continue;
+        P3 V[8];
+        for (int c = 0; c < 8; c++) V[c] = {hx[q[c]], hy[q[c]], hz[q[c]]};
+        float a_rho = 0, a_lam = 0, a_mu = 0, h_kappa = 0, h_mu = 0;
+        const float ng = (float)kSub;
+        constexpr int nsg = (kSub + 1) * (kSub + 1) * (kSub + 1);
+        int used = nsg;
+        int bad = 0;
+        for (int kk = 0; kk <= kSub && !bad; kk++)
+          for (int jj = 0; jj <= kSub && !bad; jj++)
+            for (int ii = 0; ii <= kSub; ii++) {
+              const P3 lx00 = V[0] + (V[1] - V[0]) / ng * (float)ii;
+              const P3 lx10 = V[3] + (V[2] - V[3]) / ng * (float)ii;
+              const P3 lx01 = V[4] + (V[5] - V[4]) / ng * (float)ii;
+              const P3 lx11 = V[7] + (V[6] - V[7]) / ng * (float)ii;
+              const P3 lxy0 = lx00 + (lx10 - lx00) / ng * (float)jj;
+              const P3 lxy1 = lx01 + (lx11 - lx01) / ng * (float)jj;
+              const P3 s = lxy0 + (lxy1 - lxy0) / ng * (float)kk;
+              float r, vp, vs;
+              bool on;
+              if ((bad = point(s.x, s.y, s.z, elev, r, vp, vs, on))) break;
+              if (grid_rules && on) {
+                used--;
+                continue;
+              }
+              const float m = vs * vs * r;
+              const float lm = (float)((double)(vp * vp * r) - 2.0 * (double)m);
+              a_rho += r;
+              if (har) {
+                h_kappa = (float)((double)h_kappa + 1.0 / ((double)lm + 2.0 / 3.0 * (double)m));
+                h_mu = (float)((double)h_mu + 1.0 / (double)m);
+              } else {
+                a_lam += lm;
+                a_mu += m;
+              }
+            }
+        if (bad) { jerr[j] = bad; break; }
+        if (har) {
+          const float hm = (float)(used * 1.0 / (double)h_mu), hk = (float)(used * 1.0 / (double)h_kappa);
+          lam[p] = (float)((double)hk - 2.0 / 3.0 * (double)hm);
+          mu[p] = hm;
+          rho[p] = grid_rules ? a_rho / (float)used : (float)((double)a_rho / (used * 1.0));
+        } else {
+          lam[p] = (float)((double)a_lam / (used * 1.0));
+          mu[p] = (float)((double)a_mu / (used * 1.0));
+          rho[p] = (float)((double)a_rho / (used * 1.0));
+        }
+      }
+  });
+  for (int j = 0; j < g.ny; j++)
+    if (jerr[j]) {
+      cgfd_internal_set_error(jerr[j] == 1 ? "grid point outside the interface mesh"
+                                           : "vp must be larger than sqrt(2) vs in the media file");
+      return CGFD_E_ARG;
+    }
+  (void)err;
+  return CGFD_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
+namespace {
+
+// A grid media file in the reference's md3grd format, elastic_isotropic,
+// holding only the window of the mesh that covers the grid
+// (read_grid_file, media_read_file.cpp:670-925).
+struct Md3Grd {
+  int nl = 0, ni = 0;  // layers; surfaces (the sum of the per-layer counts)
+  int nx = 0, ny = 0;  // window
+  float minx = 0, miny = 0, dx = 0, dy = 0;
+  std::vector<int> top;       // surface index of each layer's top
+  std::vector<char> starts;   // [ni]: surface that is the top of a layer below the first
+  std::vector<float> elev, rho, vp, vs;  // [surface][ny][nx]
+};
+
+int md3grd_read(const char *path, float xmin, float xmax, float ymin, float ymax, Md3Grd &G, std::string &why) {
+  std::vector<std::string> tok;
+  if (read_tokens(path, tok, why) != CGFD_OK) return CGFD_E_ARG;
+  size_t t = 0;
+  auto next = [&](std::string &out) {
+    if (t >= tok.size()) return false;
+    out = tok[t++];
+    return true;
+  };
+  std::string w;
+  if (!next(w)) { why = "no media_type"; return CGFD_E_ARG; }
+  if (w != "elastic_isotropic") {
+    why = "media_type " + w + " is not elastic_isotropic";
+    return CGFD_E_ARG;
+  }
+  auto num_i = [&](int &v) { std::string s; if (!next(s)) return false; v = std::atoi(s.c_str()); return true; };
+  auto num_f = [&](float &v) { std::string s; if (!next(s)) return false; v = std::strtof(s.c_str(), nullptr); return true; };
+  if (!num_i(G.nl) || G.nl < 1) { why = "bad number of layers"; return CGFD_E_ARG; }
+  std::vector<int> ngz(G.nl);
+  for (int l = 0; l < G.nl; l++) {
+    if (!num_i(ngz[l]) || ngz[l] < 1) { why = "bad number of surfaces in a layer"; return CGFD_E_ARG; }
+    G.top.push_back(G.ni);
+    G.ni += ngz[l];
+  }
+  G.starts.assign(G.ni, 0);
+  for (int l = 1; l < G.nl; l++) G.starts[G.top[l]] = 1;
+  int NX, NY;
+  float MINX, MINY;
+  if (!num_i(NX) || !num_i(NY) || !num_f(MINX) || !num_f(MINY) || !num_f(G.dx) || !num_f(G.dy)) {
+    why = "bad media mesh line";
+    return CGFD_E_ARG;
+  }
+  if (NX < 2 || NY < 2) { why = "media mesh needs NX, NY >= 2"; return CGFD_E_ARG; }
+  // the window [ix0, ix1] x [iy0, iy1] of mesh nodes that covers the grid
+  const float fx0 = (xmin - MINX) / G.dx, fy0 = (ymin - MINY) / G.dy;
+  const float fx1 = std::ceil((xmax - MINX) / G.dx), fy1 = std::ceil((ymax - MINY) / G.dy);
+  if (!(fx0 >= 0) || !(fx1 <= NX - 1)) { why = "the media mesh does not cover the grid in x"; return CGFD_E_ARG; }
+  if (!(fy0 >= 0) || !(fy1 <= NY - 1)) { why = "the media mesh does not cover the grid in y"; return CGFD_E_ARG; }
+  const int ix0 = (int)fx0, iy0 = (int)fy0, ix1 = (int)fx1, iy1 = (int)fy1;
+  G.nx = ix1 - ix0 + 1;
+  G.ny = iy1 - iy0 + 1;
+  G.minx = (float)ix0 * G.dx + MINX;
+  G.miny = (float)iy0 * G.dy + MINY;
+  const size_t slice = (size_t)G.nx * G.ny, n = slice * G.ni;
+  std::vector<float> *cols[4] = {&G.elev, &G.rho, &G.vp, &G.vs};
+  for (auto *c : cols) c->resize(n);
+  for (int s = 0; s < G.ni; s++)
+    for (int iy = 0; iy < NY; iy++)
+      for (int ix = 0; ix < NX; ix++) {
+        const bool keep = ix >= ix0 && ix <= ix1 && iy >= iy0 && iy <= iy1;
+        const size_t q = s * slice + (size_t)(iy - iy0) * G.nx + (ix - ix0);
+        for (auto *c : cols) {
+          float v;
+          if (!num_f(v)) { why = "insufficient data in the media file"; return CGFD_E_ARG; }
+          if (keep) (*c)[q] = v;
+        }
+      }
+  // the last surface of a layer and the first of the next must coincide
+  // (checkGridData, media_read_file.cpp:1243-1266)
+  for (int l = 1; l < G.nl; l++)
+    for (size_t q = 0; q < slice; q++)
+      if (G.elev[(G.top[l] - 1) * slice + q] != G.elev[G.top[l] * slice + q]) {
+        why = "the last surface of a layer differs from the first of the next";
+        return CGFD_E_ARG;
+      }
+  return CGFD_OK;
+}
+
+// rho, vp, vs of the grid medium at one point, and whether the point lies on
+// a layer boundary (AssignGridMediaPara2Point + CalPointValue_grid for
+// elastic_isotropic, media_grid2model.cpp:424-560): linear in z between the
+// two surfaces around the point; above the first surface the values at the
+// top; below the last those of the last; within 1 m under the top surface of
+// a layer below the first, the mean of that surface and the one above it.
+// 1: outside the media mesh, 2: vp <= sqrt(2) vs.
+int grid_point(const Md3Grd &G, const LayerEval &E, float ax, float ay, float az, std::vector<float> &elev,
+               float &r, float &vp, float &vs, bool &on) {
+  const float maxx = G.minx + (G.nx - 1) * G.dx, maxy = G.miny + (G.ny - 1) * G.dy;
+  if (ax < G.minx || ax > maxx || ay < G.miny || ay > maxy) return 1;
+  const size_t slice = (size_t)G.nx * G.ny;
+  for (int s = 0; s < G.ni; s++) elev[s] = E.bilinear(&G.elev[s * slice], ax, ay);
+  int mi = last_ge(az, elev);
+  on = mi >= 0 && G.starts[mi] && ref_is_equal(az, elev[mi]);
+  auto field = [&](const std::vector<float> &v, int s) { return E.bilinear(&v[s * slice], ax, ay); };
+  if (mi == -1 || mi == G.ni - 1) {
+    if (mi == -1) mi = last_ge(elev[0], elev);
+    r = field(G.rho, mi);
+    vp = field(G.vp, mi);
+    vs = field(G.vs, mi);
+    return 0;
+  }
+  float dis_r = 0.5f;
+  if (std::fabs(elev[mi + 1] - elev[mi]) > 1e-6) dis_r = (az - elev[mi]) / (elev[mi + 1] - elev[mi]);
+  if (mi > 0 && ref_is_equal(elev[mi], az) && G.starts[mi]) {
+    mi--;
+    dis_r = 0.5f;
+  }
+  const float p0 = field(G.vp, mi), s0 = field(G.vs, mi), r0 = field(G.rho, mi);
+  const float p1 = field(G.vp, mi + 1), s1 = field(G.vs, mi + 1), r1 = field(G.rho, mi + 1);
+  vp = p0 + (p1 - p0) * dis_r;
+  vs = s0 + (s1 - s0) * dis_r;
+  r = r0 + (r1 - r0) * dis_r;
+  if (vp <= std::sqrt(2.0) * vs) return 2;
+  return 0;
+}
+
+}  // namespace
+
 int cgfd_media_layer_file(int ni, int nj, int nk, const double *x, const double *y, const double *z, const char *path,
                           const char *method, double *lam, double *mu, double *rho) {
   if (ni < 1 || nj < 1 || nk < 1 || !x || !y || !z || !path || !method || !lam || !mu || !rho) return CGFD_E_ARG;
@@ -437,7 +697,7 @@ int cgfd_media_layer_file(int ni, int nj, int nk, const double *x, const double 
     return CGFD_E_ARG;
   }
   const G3 g(ni, nj, nk);
-  const LayerEval E(L);
+  const LayerEval E(L.nx, L.ny, L.minx, L.miny, L.dx, L.dy);
   std::vector<int> err(g.nz, 0);
   auto fail = [&]() {
     for (int k = 0; k < g.nz; k++)
@@ -465,113 +725,86 @@ int cgfd_media_layer_file(int ni, int nj, int nk, const double *x, const double 
   if (fail()) return CGFD_E_ARG;
   if (loc) return CGFD_OK;
 
-  // "har" / "ari" (parametrization_layer_el_iso_har / _ari,
-  // media_layer2model.cpp:1963-2174, with GenerateHalfGrid, MarkInterfaceNumber,
-  // GenerateHalfMesh, MeshSubdivide, NumOfValues, media_utility.cpp:404-600 and
-  // media_layer2model.cpp:1494-1580): points whose dual cell (corners at the
-  // 8 surrounding cell centres) meets more than one layer take the average of
-  // the 9^3 points that subdivide that cell: harmonic for kappa and mu and
-  // arithmetic for rho, or arithmetic for lam, mu and rho.
-  const long n = g.n;
-  std::vector<float> hx(n), hy(n), hz(n);
-  std::vector<int> mat(n);
+  return equivalent_medium(g, x, y, z, har, L.ni, false, err,
+                           [&](float hx, float hy, float hz, std::vector<float> &elev) {
+                             const size_t slice = (size_t)L.nx * L.ny;
+                             for (int l = 0; l < L.ni; l++) elev[l] = E.bilinear(&L.elev[l * slice], hx, hy);
+                             int mi = last_ge(hz, elev);
+                             if (mi == -1) mi = last_ge(elev[0], elev);
+                             return mi;
+                           },
+                           [&](float px, float py, float pz, std::vector<float> &elev, float &r, float &vp, float &vs,
+                               bool &on) {
+                             on = false;
+                             return layer_point(L, E, px, py, pz, elev, r, vp, vs);
+                           },
+                           L.ni, lam, mu, rho);
+}
+
+int cgfd_media_grid_file(int ni, int nj, int nk, const double *x, const double *y, const double *z, const char *path,
+                         const char *method, double *lam, double *mu, double *rho) {
+  if (ni < 1 || nj < 1 || nk < 1 || !x || !y || !z || !path || !method || !lam || !mu || !rho) return CGFD_E_ARG;
+  const bool loc = !std::strcmp(method, "loc"), har = !std::strcmp(method, "har"), ari = !std::strcmp(method, "ari");
+  if (!loc && !har && !ari) {
+    cgfd_internal_set_error("equivalent-medium method must be \"loc\", \"har\" or \"ari\"");
+    return CGFD_E_ARG;
+  }
+  const G3 g(ni, nj, nk);
+  // the grid's extent in x and y, ghosts included, in float
+  float xmin = (float)x[0], xmax = xmin, ymin = (float)y[0], ymax = ymin;
+  for (long p = 0; p < g.n; p++) {
+    xmin = std::min(xmin, (float)x[p]);
+    xmax = std::max(xmax, (float)x[p]);
+    ymin = std::min(ymin, (float)y[p]);
+    ymax = std::max(ymax, (float)y[p]);
+  }
+  Md3Grd G;
+  std::string why;
+  if (md3grd_read(path, xmin, xmax, ymin, ymax, G, why) != CGFD_OK) {
+    cgfd_internal_set_error(why.c_str());
+    return CGFD_E_ARG;
+  }
+  const LayerEval E(G.nx, G.ny, G.minx, G.miny, G.dx, G.dy);
+  std::vector<int> err(g.nz, 0);
+  // "loc": every point's own value (parametrization_grid_el_iso_loc,
+  // media_grid2model.cpp:1067-1110)
   par_for(g.nz, [&](int k) {
-    for (int j = 0; j < g.ny; j++)
+    std::vector<float> elev(G.ni);
+    for (int j = 0; j < g.ny && !err[k]; j++)
       for (int i = 0; i < g.nx; i++) {
         const long p = g.at(i, j, k);
-        if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
-          const long q[8] = {p, p + 1, p + g.sy + 1, p + g.sy, p + g.sz, p + g.sz + 1, p + g.sz + g.sy + 1, p + g.sy + g.sz};
-          float sx = 0, sy = 0, sz = 0;
-          for (int c = 0; c < 8; c++) {
-            sx += (float)x[q[c]];
-            sy += (float)y[q[c]];
-            sz += (float)z[q[c]];
-          }
-          hx[p] = (float)(sx / 8.0);
-          hy[p] = (float)(sy / 8.0);
-          hz[p] = (float)(sz / 8.0);
-        } else {
-          hx[p] = (float)x[p];
-          hy[p] = (float)y[p];
-          hz[p] = (float)z[p];
-        }
+        float r, vp, vs;
+        bool on;
+        if ((err[k] = grid_point(G, E, (float)x[p], (float)y[p], (float)z[p], elev, r, vp, vs, on))) break;
+        const float m = vs * vs * r;
+        mu[p] = m;
+        lam[p] = (float)((double)(vp * vp * r) - 2.0 * (double)m);
+        rho[p] = r;
       }
   });
-  par_for(g.nz, [&](int k) {
-    std::vector<float> elev(L.ni);
-    const size_t slice = (size_t)L.nx * L.ny;
-    for (int j = 0; j < g.ny; j++)
-      for (int i = 0; i < g.nx; i++) {
-        const long p = g.at(i, j, k);
-        for (int l = 0; l < L.ni; l++) elev[l] = E.bilinear(&L.elev[l * slice], hx[p], hy[p]);
-        int mi = last_ge(hz[p], elev);
-        if (mi == -1) mi = last_ge(elev[0], elev);
-        mat[p] = mi;
-      }
-  });
-  constexpr int kSub = 8;  // subdivisions per direction of a dual cell
-  par_for(g.ny, [&](int j) {
-    if (j < 1 || j > g.ny - 2) return;
-    std::vector<float> elev(L.ni);
-    std::vector<int> tab(L.ni);
-    for (int k = 1; k < g.nz - 1 && !err[j % g.nz]; k++)
-      for (int i = 1; i < g.nx - 1; i++) {
-        const long p = g.at(i, j, k);
-        const long q[8] = {p - 1 - g.sy - g.sz, p - g.sy - g.sz, p - g.sz, p - 1 - g.sz,
-                           p - 1 - g.sy,        p - g.sy,        p,        p - 1};
-        // distinct layers among the 8 corners (a corner above every interface: none)
-        std::fill(tab.begin(), tab.end(), 0);
-        int distinct = 0;
-        bool above = false;
-        for (int c = 0; c < 8; c++) {
-          const int v = mat[q[c]];
-          if (v == -1) { above = true; break; }
-          if (tab[v]++ == 0) distinct++;
-        }
-        if (above || distinct <= 1) continue;
-        P3 V[8];
-        for (int c = 0; c < 8; c++) V[c] = {hx[q[c]], hy[q[c]], hz[q[c]]};
-        float a_rho = 0, a_lam = 0, a_mu = 0, h_kappa = 0, h_mu = 0;
-        const float ng = (float)kSub;
-        int bad = 0;
-        for (int kk = 0; kk <= kSub && !bad; kk++)
-          for (int jj = 0; jj <= kSub && !bad; jj++)
-            for (int ii = 0; ii <= kSub; ii++) {
-              const P3 lx00 = V[0] + (V[1] - V[0]) / ng * (float)ii;
-              const P3 lx10 = V[3] + (V[2] - V[3]) / ng * (float)ii;
-              const P3 lx01 = V[4] + (V[5] - V[4]) / ng * (float)ii;
-              const P3 lx11 = V[7] + (V[6] - V[7]) / ng * (float)ii;
-              const P3 lxy0 = lx00 + (lx10 - lx00) / ng * (float)jj;
-              const P3 lxy1 = lx01 + (lx11 - lx01) / ng * (float)jj;
-              const P3 s = lxy0 + (lxy1 - lxy0) / ng * (float)kk;
-              float r, vp, vs;
-              if ((bad = layer_point(L, E, s.x, s.y, s.z, elev, r, vp, vs))) break;
-              const float m = vs * vs * r;
-              const float lm = (float)((double)(vp * vp * r) - 2.0 * (double)m);
-              a_rho += r;
-              if (har) {
-                h_kappa = (float)((double)h_kappa + 1.0 / ((double)lm + 2.0 / 3.0 * (double)m));
-                h_mu = (float)((double)h_mu + 1.0 / (double)m);
-              } else {
-                a_lam += lm;
-                a_mu += m;
-              }
-            }
-        if (bad) { err[j % g.nz] = bad; break; }
-        constexpr int nsg = (kSub + 1) * (kSub + 1) * (kSub + 1);
-        if (har) {
-          const float hm = (float)(nsg * 1.0 / (double)h_mu), hk = (float)(nsg * 1.0 / (double)h_kappa);
-          lam[p] = (float)((double)hk - 2.0 / 3.0 * (double)hm);
-          mu[p] = hm;
-        } else {
-          lam[p] = (float)((double)a_lam / (nsg * 1.0));
-          mu[p] = (float)((double)a_mu / (nsg * 1.0));
-        }
-        rho[p] = (float)((double)a_rho / (nsg * 1.0));
-      }
-  });
-  if (fail()) return CGFD_E_ARG;
-  return CGFD_OK;
+  for (int k = 0; k < g.nz; k++)
+    if (err[k]) {
+      cgfd_internal_set_error(err[k] == 1 ? "grid point outside the media mesh"
+                                          : "vp must be larger than sqrt(2) vs in the media file");
+      return CGFD_E_ARG;
+    }
+  // one layer: nothing to average (the reference falls back to "loc")
+  if (loc || G.nl < 2) return CGFD_OK;
+
+  // layers are numbered by the top surface of each (LayerNumberAtPoint,
+  // media_grid2model.cpp:1269-1307)
+  const size_t slice = (size_t)G.nx * G.ny;
+  return equivalent_medium(g, x, y, z, har, G.nl, true, err,
+                           [&](float hx, float hy, float hz, std::vector<float> &elev) {
+                             elev.resize(G.nl);
+                             for (int l = 0; l < G.nl; l++) elev[l] = E.bilinear(&G.elev[G.top[l] * slice], hx, hy);
+                             int mi = last_ge(hz, elev);
+                             if (mi == -1) mi = last_ge(elev[0], elev);
+                             return mi;
+                           },
+                           [&](float px, float py, float pz, std::vector<float> &elev, float &r, float &vp, float &vs,
+                               bool &on) { return grid_point(G, E, px, py, pz, elev, r, vp, vs, on); },
+                           G.ni, lam, mu, rho);
 }
 
 int cgfd_media_homogeneous(int ni, int nj, int nk, double vp, double vs, double rho, double *lam, double *mu,

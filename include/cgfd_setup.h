@@ -97,6 +97,31 @@ int cgfd_media_layered(int ni, int nj, int nk, const double *x, const double *y,
 int cgfd_media_layer_file(int ni, int nj, int nk, const double *x, const double *y, const double *z, const char *path,
                           const char *method, double *lam, double *mu, double *rho);
 
+/* Medium from a grid media file in the reference's md3grd format, media_type
+ * elastic_isotropic (read_grid_file, src/media/media_read_file.cpp:670-925,
+ * 1243-1266): "elastic_isotropic", the number of layers NL, the number of
+ * surfaces in each layer, "NX NY MINX MINY DX DY", then (sum of the surface
+ * counts) x NY x NX rows of "elevation rho vp vs", surfaces listed from the
+ * top; the last surface of a layer must equal the first of the next.  Only
+ * the window of the mesh covering the grid's x, y extent is kept, as there.
+ * Method "loc": rho, vp, vs linear in z between the two surfaces around each
+ * point, the top values above the first surface and the last surface's below
+ * it (media_grid2model.cpp:424-560, 1067-1110); "har" / "ari" average over the
+ * 9^3 points subdividing each dual cell that meets more than one layer, as
+ * cgfd_media_layer_file, with layers numbered by their top surfaces and the
+ * sub-points on a layer boundary left out (media_grid2model.cpp:1269-1355,
+ * 1819-2079); with one layer they give "loc".  Reference quirks kept: a point
+ * within 1 m under the top surface of a lower layer takes the mean of that
+ * surface and the one above; a point on the window's last mesh row or column
+ * takes the value of the node at its truncated index, without interpolation
+ * along the other axis (BilinearInterpolation, media_utility.cpp:145-182).
+ * Unlike the reference, a mesh whose window would
+ * reach past its last node is rejected rather than read uninitialised.
+ * CGFD_E_ARG (message in cgfd_last_error) for an unreadable file, a mesh that
+ * does not cover the grid or vp <= sqrt(2) vs. */
+int cgfd_media_grid_file(int ni, int nj, int nk, const double *x, const double *y, const double *z, const char *path,
+                         const char *method, double *lam, double *mu, double *rho);
+
 /* Homogeneous medium at every point. */
 int cgfd_media_homogeneous(int ni, int nj, int nk, double vp, double vs, double rho, double *lam, double *mu,
                            double *rhoa);
