@@ -35,6 +35,7 @@ EXPORTS = [
     "cgfd_media_layered", "cgfd_media_homogeneous", "cgfd_rho_to_slow", "cgfd_pml_len_dh",
     "cgfd_pml_profile", "cgfd_free_surface_matrices", "cgfd_wavelet", "cgfd_angle2moment",
     "cgfd_source_point", "cgfd_receiver_index", "cgfd_locate_points", "cgfd_media_layer_file", "cgfd_media_grid_file",
+    "cgfd_media_bin_file",
     "cgfd_storage_scales",
     "cgfd_server_create", "cgfd_server_destroy", "cgfd_server_add_model", "cgfd_server_submit",
     "cgfd_server_wait", "cgfd_server_result", "cgfd_server_get_stats", "cgfd_fetch_subvolume",
@@ -182,6 +183,8 @@ def load(path=LIB_PATH):
         "cgfd_locate_points": ([i, i, i, vp, vp, vp, i, vp, vp, vp, vp, vp], i),
         "cgfd_media_layer_file": ([i, i, i, vp, vp, vp, C.c_char_p, C.c_char_p, vp, vp, vp], i),
         "cgfd_media_grid_file": ([i, i, i, vp, vp, vp, C.c_char_p, C.c_char_p, vp, vp, vp], i),
+        "cgfd_media_bin_file": ([i, i, i, vp, vp, vp, vp, vp, vp, vp, C.c_char_p, C.c_char_p, C.c_char_p,
+                                 vp, vp, vp], i),
         "cgfd_storage_scales": ([i, i, i, i, vp, vp, vp, vp, C.POINTER(SourceDesc), d, vp], i),
         "cgfd_fetch_subvolume": ([vp, i, i, C.c_uint, i, i, i, i, i, i, i, i, i, i, vp], i),
         "cgfd_sac_write": ([C.c_char_p, vp, i, C.c_float, C.c_float, vp, vp, C.c_char_p, C.c_char_p], i),
@@ -559,6 +562,24 @@ def media_grid_file(ni, nj, nk, x, y, z, path, method="loc"):
     xs = [_arr(a, np.float64) for a in (x, y, z)]
     _check(_lib.cgfd_media_grid_file(ni, nj, nk, *[_ptr(a) for a in xs], str(path).encode(), method.encode(),
                                      _ptr(lam), _ptr(mu), _ptr(rho)))
+    return lam, mu, rho
+
+
+def media_bin_file(ni, nj, nk, x, y, z, order, size, spacing, origin, rho_path, vp_path, vs_path):
+    """lam, mu, rho from three raw float32 volumes (cgfd_media_bin_file).  order
+    names the axis of each file dimension, fastest first: a string such as
+    "xyz" or "zxy", or three of 0, 1, 2 for x, y, z."""
+    load()
+    if isinstance(order, str):
+        order = ["xyz".index(c) for c in order]
+    sh = (nk + 6, nj + 6, ni + 6)
+    lam, mu, rho = _z(sh), _z(sh), _z(sh)
+    xs = [_arr(a, np.float64) for a in (x, y, z)]
+    o, n = np.asarray(order, np.int32), np.asarray(size, np.int32)
+    h, g = np.asarray(spacing, np.float64), np.asarray(origin, np.float64)
+    _check(_lib.cgfd_media_bin_file(ni, nj, nk, *[_ptr(a) for a in xs], _ptr(o), _ptr(n), _ptr(h), _ptr(g),
+                                    str(rho_path).encode(), str(vp_path).encode(), str(vs_path).encode(),
+                                    _ptr(lam), _ptr(mu), _ptr(rho)))
     return lam, mu, rho
 
 

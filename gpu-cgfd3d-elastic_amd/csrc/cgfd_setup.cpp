@@ -807,6 +807,150 @@ int cgfd_media_grid_file(int ni, int nj, int nk, const double *x, const double *
                            G.ni, lam, mu, rho);
 }
 
+int cgfd_media_bin_file(int ni, int nj, int nk, const double *x, const double *y, const double *z,
+                        const int order[3], const int size[3], const double spacing[3], const double origin[3],
+                        const char *rho_path, const char *vp_path, const char *vs_path, double *lam, double *mu,
+                        double *rho) {
+  if (ni < 1 || nj < 1 || nk < 1 || !x || !y || !z || !order || !size || !spacing || !origin || !rho_path ||
+      !vp_path || !vs_path || !lam || !mu || !rho)
+    return CGFD_E_ARG;
+  // which file dimension holds x, y and z (order[d] = 0, 1, 2 for x, y, z)
+  int dim[3] = {-1, -1, -1};
+  for (int d = 0; d < 3; d++) {
+    if (order[d] < 0 || order[d] > 2 || dim[order[d]] != -1) {
+      cgfd_internal_set_error("bin order must name x (0), y (1) and z (2) once each");
+      return CGFD_E_ARG;
+    }
+    dim[order[d]] = d;
+  }
+  for (int d = 0; d < 3; d++)
+    if (size[d] < 2 || !(spacing[d] != 0)) {
+      cgfd_internal_set_error("bin sizes must be at least 2 with nonzero spacing");
+      return CGFD_E_ARG;
+    }
+  const G3 g(ni, nj, nk);
+  float xmin = (float)x[0], xmax = xmin, ymin = (float)y[0], ymax = ymin, zmin = FLT_MAX, zmax = -FLT_MAX;
+  for (long p = 0; p < g.n; p++) {
+    xmin = std::min(xmin, (float)x[p]);
+    xmax = std::max(xmax, (float)x[p]);
+    ymin = std::min(ymin, (float)y[p]);
+    ymax = std::max(ymax, (float)y[p]);
+    zmin = std::min(zmin, (float)z[p]);
+    zmax = std::max(zmax, (float)z[p]);
+  }
+  const float o[3] = {(float)origin[dim[0]], (float)origin[dim[1]], (float)origin[dim[2]]};
+  const float h[3] = {(float)spacing[dim[0]], (float)spacing[dim[1]], (float)spacing[dim[2]]};
+  // the window of file nodes that covers the grid in x and y
+  // (media_bin2model_el_iso, media_bin2model.cpp:83-137)
+  int lo[3], hi[3];
+  const float amin[2] = {xmin, ymin}, amax[2] = {xmax, ymax};
+  for (int a = 0; a < 2; a++) {
+    const int n = size[dim[a]];
+    lo[a] = amin[a] < o[a] ? 0 : (int)((amin[a] - o[a]) / h[a]);
+    if (lo[a] >= n || amax[a] < o[a]) {
+      cgfd_internal_set_error(a == 0 ? "the bin model does not cover the grid in x"
+                                     : "the bin model does not cover the grid in y");
+      return CGFD_E_ARG;
+    }
+    hi[a] = (int)std::ceil((amax[a] - o[a]) / h[a]);
+    if (hi[a] >= n) hi[a] = n - 1;
+    if (hi[a] <= lo[a]) {
+      cgfd_internal_set_error("the bin model window covering the grid needs two nodes per axis");
+      return CGFD_E_ARG;
+    }
+  }
+  lo[2] = 0;
+  hi[2] = size[dim[2]] - 1;
+  const float zn = o[2] + h[2] * (size[dim[2]] - 1);
+  if (zmin >= std::max(zn, o[2]) || zmax <= std::min(o[2], zn)) {
+    cgfd_internal_set_error("the bin model does not cover the grid in z");
+    return CGFD_E_ARG;
+  }
+  const int bn[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] + 1};
+  std::vector<float> vec[3];
+  for (int a = 0; a < 3; a++) {
+    vec[a].resize(bn[a]);
+    for (int q = 0; q < bn[a]; q++) vec[a][q] = o[a] + (float)(size_t)(lo[a] + q) * h[a];
+  }
+  // read the three files: dimension 1 fastest; keep the window as [z][y][x]
+  // (read_bin_file, media_read_file.cpp:1274-1320)
+  const size_t line = bn[0], slice = (size_t)bn[0] * bn[1], nfile = (size_t)size[0] * size[1] * size[2];
+  std::vector<float> fld[3];
+  const char *paths[3] = {rho_path, vp_path, vs_path};
+  std::vector<float> buf(nfile);
+  int flo[3], fhi[3];
+  for (int a = 0; a < 3; a++) {
+    flo[dim[a]] = lo[a];
+    fhi[dim[a]] = hi[a];
+  }
+  for (int f = 0; f < 3; f++) {
+    FILE *fp = std::fopen(paths[f], "rb");
+    if (!fp) {
+      cgfd_internal_set_error((std::string("cannot open ") + paths[f]).c_str());
+      return CGFD_E_ARG;
+    }
+    const size_t got = std::fread(buf.data(), sizeof(float), nfile, fp);
+    std::fclose(fp);
+    if (got < nfile) {
+      cgfd_internal_set_error((std::string("insufficient data in ") + paths[f]).c_str());
+      return CGFD_E_ARG;
+    }
+    fld[f].resize(slice * bn[2]);
+    int i[3];
+    for (i[2] = flo[2]; i[2] <= fhi[2]; i[2]++)
+      for (i[1] = flo[1]; i[1] <= fhi[1]; i[1]++)
+        for (i[0] = flo[0]; i[0] <= fhi[0]; i[0]++)
+          fld[f][(i[dim[0]] - lo[0]) + (size_t)(i[dim[1]] - lo[1]) * line + (size_t)i[dim[2]] * slice] =
+              buf[i[0] + (size_t)size[0] * (i[1] + (size_t)size[1] * i[2])];
+  }
+  // trilinear interpolation with the reference's clamping and operation order
+  // (TrilinearInterpolation, media_utility.cpp:189-284)
+  const float d[3] = {vec[0][1] - vec[0][0], vec[1][1] - vec[1][0], vec[2][1] - vec[2][0]};
+  auto tri = [&](const std::vector<float> &v, const int *i0, const int *i2, const float *q) {
+    auto at = [&](int a, int b, int c) { return v[a + b * line + c * slice]; };
+    if (i0[0] == i2[0] && i0[1] == i2[1] && i0[2] == i2[2]) return at(i0[0], i0[1], i0[2]);
+    const float v000 = at(i0[0], i0[1], i0[2]), v002 = at(i0[0], i0[1], i2[2]);
+    const float v200 = at(i2[0], i0[1], i0[2]), v202 = at(i2[0], i0[1], i2[2]);
+    const float v020 = at(i0[0], i2[1], i0[2]), v022 = at(i0[0], i2[1], i2[2]);
+    const float v220 = at(i2[0], i2[1], i0[2]), v222 = at(i2[0], i2[1], i2[2]);
+    float v100 = v000, v102 = v002, v122 = v022, v120 = v020;
+    if (i0[0] != i2[0]) {
+      const float w = (q[0] - vec[0][i0[0]]) / (vec[0][i2[0]] - vec[0][i0[0]]);
+      v100 = v000 + w * (v200 - v000);
+      v102 = v002 + w * (v202 - v002);
+      v122 = v022 + w * (v222 - v022);
+      v120 = v020 + w * (v220 - v020);
+    }
+    float v110 = v100, v112 = v102;
+    if (i0[1] != i2[1]) {
+      const float w = (q[1] - vec[1][i0[1]]) / (vec[1][i2[1]] - vec[1][i0[1]]);
+      v110 = v100 + w * (v120 - v100);
+      v112 = v102 + w * (v122 - v102);
+    }
+    if (i0[2] == i2[2]) return v110;
+    return v110 + (q[2] - vec[2][i0[2]]) / (vec[2][i2[2]] - vec[2][i0[2]]) * (v112 - v110);
+  };
+  par_for(g.nz, [&](int k) {
+    for (int j = 0; j < g.ny; j++)
+      for (int i = 0; i < g.nx; i++) {
+        const long p = g.at(i, j, k);
+        const float q[3] = {(float)x[p], (float)y[p], (float)z[p]};
+        int i0[3], i2[3];
+        for (int a = 0; a < 3; a++) {
+          const float t = (q[a] - vec[a][0]) / d[a];
+          i0[a] = std::min(std::max((int)std::floor(t), 0), bn[a] - 1);
+          i2[a] = std::min(std::max((int)std::ceil(t), 0), bn[a] - 1);
+        }
+        const float r = tri(fld[0], i0, i2, q), vp = tri(fld[1], i0, i2, q), vs = tri(fld[2], i0, i2, q);
+        const float m = vs * vs * r;
+        mu[p] = m;
+        lam[p] = (float)((double)(vp * vp * r) - 2.0 * (double)m);
+        rho[p] = r;
+      }
+  });
+  return CGFD_OK;
+}
+
 int cgfd_media_homogeneous(int ni, int nj, int nk, double vp, double vs, double rho, double *lam, double *mu,
                            double *rhoa) {
   if (ni < 1 || nj < 1 || nk < 1 || !lam || !mu || !rhoa || vp <= 0 || vs < 0 || rho <= 0) return CGFD_E_ARG;

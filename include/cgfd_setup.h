@@ -122,6 +122,25 @@ int cgfd_media_layer_file(int ni, int nj, int nk, const double *x, const double 
 int cgfd_media_grid_file(int ni, int nj, int nk, const double *x, const double *y, const double *z, const char *path,
                          const char *method, double *lam, double *mu, double *rho);
 
+/* Medium from three raw binary volumes of float32 rho, vp and vs
+ * (media_bin2model_el_iso, src/media/media_bin2model.cpp:27-230, with
+ * read_bin_file, media_read_file.cpp:1274-1320, and TrilinearInterpolation,
+ * media_utility.cpp:189-284; the driver's "binfile" medium,
+ * main_curv_col_el_3d.cu:439-470).  File dimension d (d = 0 fastest) has
+ * size[d] nodes at origin[d] + n spacing[d] and holds axis order[d]
+ * (0 = x, 1 = y, 2 = z), as the parameter file's dim1..dim3.  Only the window
+ * of file nodes covering the grid's x, y extent is kept.  Each grid point
+ * takes the trilinear interpolation of the 8 nodes around it, in float with
+ * the reference's order; outside the file a coordinate clamps to the edge
+ * node.  lam = vp^2 rho - 2 mu, mu = vs^2 rho.  CGFD_E_ARG (message in
+ * cgfd_last_error) for a bad order, a model that does not meet the grid, a
+ * covering window of one node on an axis (the reference reads past its
+ * mesh there), or a missing or short file. */
+int cgfd_media_bin_file(int ni, int nj, int nk, const double *x, const double *y, const double *z,
+                        const int order[3], const int size[3], const double spacing[3], const double origin[3],
+                        const char *rho_path, const char *vp_path, const char *vs_path, double *lam, double *mu,
+                        double *rho);
+
 /* Homogeneous medium at every point. */
 int cgfd_media_homogeneous(int ni, int nj, int nk, double vp, double vs, double rho, double *lam, double *mu,
                            double *rhoa);
