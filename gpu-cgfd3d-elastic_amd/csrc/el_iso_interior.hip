@@ -36,7 +36,7 @@
 
 namespace cgfd {
 
-template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINB, bool CART, bool PML>
+template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINB, bool CART, bool PML, int MODE>
 __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, MarchSet b, MarchGrid g) {
   constexpr int NT = TX * TY;
   constexpr int HXL = (DX == 0) ? 3 : 1;
@@ -149,8 +149,12 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
 
   // Everything plane k needs from HBM/L2, fetched one plane ahead so its
   // latency overlaps the previous plane's arithmetic.
-  const int mode = a.mode;
-  const bool need_end = mode == MODE_S12 || mode == MODE_S3, need_pre = mode == MODE_S12;
+  // The RK mode is a template argument: every load a mode needs is then
+  // unconditional, so no 16-bit value is converted inside a branch right
+  // after its load (which makes the wave wait there for every load in
+  // flight, the next plane's prefetch included).
+  constexpr int mode = MODE;
+  constexpr bool need_end = mode == MODE_S12 || mode == MODE_S3, need_pre = mode == MODE_S12;
   // The PML form fetches the RK levels for the current plane (not one plane
   // ahead): the 18 registers it saves are what its face update needs.
   constexpr bool LATE_RK = PML;
@@ -170,7 +174,7 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
 #pragma unroll
     for (int m = 0; m < 3; m++) n_met[9 + m] = (at_b(a.med + m * cs, ob));
     const unsigned qb = o * (unsigned)sizeof(TS);
-    if (!LATE_RK && need_end) {
+    if constexpr (!LATE_RK && need_end) {
 #pragma unroll
       for (int c = 0; c < kNC; c++) {
         n_end[c] = (TC)St<TS>::ld(&at_b(a.w_end + so + c * cs, qb)) * sc(c);
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
     for (int p = 0; p < NPER; p++) hv[p] = n_hv[p];
 #pragma unroll
     for (int m = 0; m < 12; m++) mt[m] = n_met[m];
-    if (LATE_RK && need_end) {
+    if constexpr (LATE_RK && need_end) {
       // this plane's RK levels, issued before the next plane's prefetch
       // (memory counters complete in order)
 #pragma unroll
@@ -295,7 +299,7 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
           const TC e1 = DIM == 0 ? xix : DIM == 1 ? etx : ztx;
           const TC e2 = DIM == 0 ? xiy : DIM == 1 ? ety : zty;
           const TC e3 = DIM == 0 ? xiz : DIM == 1 ? etz : ztz;
-          pml_face_march<DIM, TS, TC>(a, fi, shot, i, j, k, Dd, e1, e2, e3, lam, mu, lam2mu, slw, h);
+          pml_face_march<DIM, mode, TS, TC>(a, fi, shot, i, j, k, Dd, e1, e2, e3, lam, mu, lam2mu, slw, h);
         };
 #pragma unroll 1
         for (unsigned fm = plane_faces; fm; fm &= fm - 1) {
@@ -311,14 +315,14 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
         TS *wout = a.w_out + so + c * cs;
         TS *wend = a.w_end + so + c * cs;
         const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
-        if (mode == MODE_S12) {
+        if constexpr (mode == MODE_S12) {
           St<TS>::st(&at_b(wout, qb), (pre_v[c] + a.ca * h[c]) * is);
           St<TS>::st(&at_b(wend, qb), (end_v[c] + a.cb * h[c]) * is);
-        } else if (mode == MODE_S0) {
+        } else if constexpr (mode == MODE_S0) {
           const TC v = q[c][zc0];
           St<TS>::st(&at_b(wout, qb), (v + a.ca * h[c]) * is);
           St<TS>::st(&at_b(wend, qb), (v + a.cb * h[c]) * is);
-        } else if (mode == MODE_S3) {
+        } else if constexpr (mode == MODE_S3) {
           St<TS>::st(&at_b(wend, qb), (end_v[c] + a.cb * h[c]) * is);
         } else {
           St<TS>::st(&at_b(wout, qb), h[c] * is);
@@ -612,6 +616,25 @@ static hipError_t launch_split_t(const StageArgs<TS, TC> &a, const MarchSet &b, 
   return hipGetLastError();
 }
 
+// one RK mode's k_march for the operator directions sel = 4 dx + 2 dy + dz
+template <typename TS, typename TC, int TX, int TY, int MINB, bool CART, bool PML, int MODE>
+static void launch_march_m(const StageArgs<TS, TC> &a, const MarchSet &b, const MarchGrid &g, dim3 grid, dim3 block,
+                           int sel, hipStream_t st) {
+#define CGFD_L(X, Y, Z) \
+  hipLaunchKernelGGL((k_march<X, Y, Z, TS, TC, TX, TY, MINB, CART, PML, MODE>), grid, block, 0, st, a, b, g)
+  switch (sel) {
+    case 0: CGFD_L(0, 0, 0); break;
+    case 1: CGFD_L(0, 0, 1); break;
+    case 2: CGFD_L(0, 1, 0); break;
+    case 3: CGFD_L(0, 1, 1); break;
+    case 4: CGFD_L(1, 0, 0); break;
+    case 5: CGFD_L(1, 0, 1); break;
+    case 6: CGFD_L(1, 1, 0); break;
+    default: CGFD_L(1, 1, 1); break;
+  }
+#undef CGFD_L
+}
+
 template <typename TS, typename TC, int TX, int TY, int MINB, bool CART, bool PML>
 static hipError_t launch_march_t(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz,
                                  int nshots, hipStream_t st) {
@@ -629,19 +652,12 @@ static hipError_t launch_march_t(const StageArgs<TS, TC> &a, const MarchSet &b, 
   if (nwg == 0) return hipSuccess;
   dim3 grid((unsigned)nwg), block(TX * TY);
   const int sel = dx * 4 + dy * 2 + dz;
-#define CGFD_L(X, Y, Z) \
-  hipLaunchKernelGGL((k_march<X, Y, Z, TS, TC, TX, TY, MINB, CART, PML>), grid, block, 0, st, a, b, g)
-  switch (sel) {
-    case 0: CGFD_L(0, 0, 0); break;
-    case 1: CGFD_L(0, 0, 1); break;
-    case 2: CGFD_L(0, 1, 0); break;
-    case 3: CGFD_L(0, 1, 1); break;
-    case 4: CGFD_L(1, 0, 0); break;
-    case 5: CGFD_L(1, 0, 1); break;
-    case 6: CGFD_L(1, 1, 0); break;
-    default: CGFD_L(1, 1, 1); break;
+  switch (a.mode) {
+    case MODE_S0: launch_march_m<TS, TC, TX, TY, MINB, CART, PML, MODE_S0>(a, b, g, grid, block, sel, st); break;
+    case MODE_S12: launch_march_m<TS, TC, TX, TY, MINB, CART, PML, MODE_S12>(a, b, g, grid, block, sel, st); break;
+    case MODE_S3: launch_march_m<TS, TC, TX, TY, MINB, CART, PML, MODE_S3>(a, b, g, grid, block, sel, st); break;
+    default: launch_march_m<TS, TC, TX, TY, MINB, CART, PML, MODE_RHS>(a, b, g, grid, block, sel, st); break;
   }
-#undef CGFD_L
   return hipGetLastError();
 }
 

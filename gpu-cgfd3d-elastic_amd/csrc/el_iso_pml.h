@@ -101,11 +101,12 @@ __device__ __forceinline__ bool pml_in_face(const StageArgs<TS, TC> &a, int fi, 
 }
 
 // pml_face for a point off the free-surface row (no surface term), the form
-// the z-marching sweep uses: the face axis DIM is a compile-time constant,
-// every face argument is read once, the RK mode branches once around
-// straight-line component loops, and on z faces the profile index is uniform
-// across the wave, so the three profile values come through scalar loads.
-template <int DIM, typename TS, typename TC>
+// the z-marching sweep uses: the face axis DIM and the RK mode MODE are
+// compile-time constants (straight-line component loops whose loads are
+// unconditional), every face argument is read once, and on z faces the
+// profile index is uniform across the wave, so the three profile values come
+// through scalar loads.
+template <int DIM, int MODE, typename TS, typename TC>
 __device__ __forceinline__ void pml_face_march(const StageArgs<TS, TC> &a, int fi, int shot, int i, int j, int k,
                                                const TC *Dd, TC e1, TC e2, TC e3, TC lam, TC mu, TC lam2mu, TC slw,
                                                TC *h) {
@@ -116,7 +117,6 @@ __device__ __forceinline__ void pml_face_march(const StageArgs<TS, TC> &a, int f
   TS *pout = F.a_out + fso, *pend = F.a_end + fso;
   const unsigned fs = (unsigned)F.fsize;
   const unsigned ia = (unsigned)(((k - k1) * bj + (j - j1)) * bi + (i - i1));
-  const int mode = a.mode;
   auto qb = [&](int c) { return (c * fs + ia) * (unsigned)sizeof(TS); };
   auto S = [&](int c) -> TC { return St<TS>::scaled ? (TC)a.sc[c] : TC(1); };
   auto IS = [&](int c) -> TC { return St<TS>::scaled ? (TC)a.isc[c] : TC(1); };
@@ -133,13 +133,13 @@ __device__ __forceinline__ void pml_face_march(const StageArgs<TS, TC> &a, int f
   TC P[kNC], pre[kNC], en[kNC];
 #pragma unroll
   for (int c = 0; c < kNC; c++) P[c] = (TC)St<TS>::ld(&at_b(pin, qb(c))) * S(c);
-  if (mode == MODE_S12) {
+  if constexpr (MODE == MODE_S12) {
 #pragma unroll
     for (int c = 0; c < kNC; c++) {
       pre[c] = (TC)St<TS>::ld(&at_b(ppre, qb(c))) * S(c);
       en[c] = (TC)St<TS>::ld(&at_b(pend, qb(c))) * S(c);
     }
-  } else if (mode == MODE_S3) {
+  } else if constexpr (MODE == MODE_S3) {
 #pragma unroll
     for (int c = 0; c < kNC; c++) en[c] = (TC)St<TS>::ld(&at_b(pend, qb(c))) * S(c);
   }
@@ -163,19 +163,19 @@ __device__ __forceinline__ void pml_face_march(const StageArgs<TS, TC> &a, int f
     h[c] += Bm1 * R[c] - cB * P[c];
     Ph[c] = cD * R[c] - cA * P[c];
   }
-  if (mode == MODE_S12) {
+  if constexpr (MODE == MODE_S12) {
 #pragma unroll
     for (int c = 0; c < kNC; c++) {
       St<TS>::st(&at_b(pout, qb(c)), (pre[c] + a.ca * Ph[c]) * IS(c));
       St<TS>::st(&at_b(pend, qb(c)), (en[c] + a.cb * Ph[c]) * IS(c));
     }
-  } else if (mode == MODE_S0) {
+  } else if constexpr (MODE == MODE_S0) {
 #pragma unroll
     for (int c = 0; c < kNC; c++) {
       St<TS>::st(&at_b(pout, qb(c)), (P[c] + a.ca * Ph[c]) * IS(c));
       St<TS>::st(&at_b(pend, qb(c)), (P[c] + a.cb * Ph[c]) * IS(c));
     }
-  } else if (mode == MODE_S3) {
+  } else if constexpr (MODE == MODE_S3) {
 #pragma unroll
     for (int c = 0; c < kNC; c++) St<TS>::st(&at_b(pend, qb(c)), (en[c] + a.cb * Ph[c]) * IS(c));
   } else {
