@@ -15,8 +15,9 @@
 namespace cgfd {
 
 // One face fi (its box must contain the point): Dd are the raw derivatives
-// along the face's axis and e1..e3 the matching metric row.
-template <typename TS, typename TC>
+// along the face's axis and e1..e3 the matching metric row; MODE is the RK
+// mode (compile-time: each mode's loads are unconditional).
+template <int MODE, typename TS, typename TC>
 __device__ __forceinline__ void pml_face(const StageArgs<TS, TC> &a, int fi, int shot, int i, int j, int k,
                                          const TC *Dd, TC e1, TC e2, TC e3, TC ztx, TC zty, TC ztz, TC lam, TC mu,
                                          TC lam2mu, TC slw, TC *h) {
@@ -47,8 +48,8 @@ __device__ __forceinline__ void pml_face(const StageArgs<TS, TC> &a, int fi, int
   for (int c = 0; c < kNC; c++) {
     const unsigned qb = (c * fs + ia) * (unsigned)sizeof(TS);
     P[c] = (TC)St<TS>::ld(&at_b(F.a_in + fso, qb)) * S(c);
-    if (a.mode == MODE_S12) pre_a[c] = (TC)St<TS>::ld(&at_b(F.a_pre + fso, qb)) * S(c);
-    if (a.mode == MODE_S12 || a.mode == MODE_S3) end_a[c] = (TC)St<TS>::ld(&at_b(F.a_end + fso, qb)) * S(c);
+    if constexpr (MODE == MODE_S12) pre_a[c] = (TC)St<TS>::ld(&at_b(F.a_pre + fso, qb)) * S(c);
+    if constexpr (MODE == MODE_S12 || MODE == MODE_S3) end_a[c] = (TC)St<TS>::ld(&at_b(F.a_end + fso, qb)) * S(c);
   }
 #pragma unroll
   for (int c = 0; c < kNC; c++) {
@@ -80,12 +81,12 @@ __device__ __forceinline__ void pml_face(const StageArgs<TS, TC> &a, int fi, int
   for (int c = 0; c < kNC; c++) {
     const unsigned qb = (c * fs + ia) * (unsigned)sizeof(TS);
     const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
-    if (a.mode == MODE_RHS) {
+    if constexpr (MODE == MODE_RHS) {
       St<TS>::st(&at_b(F.a_out + fso, qb), Ph[c] * is);
-    } else if (a.mode == MODE_S0) {
+    } else if constexpr (MODE == MODE_S0) {
       St<TS>::st(&at_b(F.a_out + fso, qb), (P[c] + a.ca * Ph[c]) * is);
       St<TS>::st(&at_b(F.a_end + fso, qb), (P[c] + a.cb * Ph[c]) * is);
-    } else if (a.mode == MODE_S12) {
+    } else if constexpr (MODE == MODE_S12) {
       St<TS>::st(&at_b(F.a_out + fso, qb), (pre_a[c] + a.ca * Ph[c]) * is);
       St<TS>::st(&at_b(F.a_end + fso, qb), (end_a[c] + a.cb * Ph[c]) * is);
     } else {
@@ -185,7 +186,7 @@ __device__ __forceinline__ void pml_face_march(const StageArgs<TS, TC> &a, int f
 }
 
 // Every face box containing the point, in the reference's face order.
-template <typename TS, typename TC>
+template <int MODE, typename TS, typename TC>
 __device__ __forceinline__ void pml_apply(const StageArgs<TS, TC> &a, int shot, int i, int j, int k, const TC *Dx,
                                           const TC *Dy, const TC *Dz, TC xix, TC xiy, TC xiz, TC etx, TC ety,
                                           TC etz, TC ztx, TC zty, TC ztz, TC lam, TC mu, TC lam2mu, TC slw,
@@ -200,7 +201,7 @@ __device__ __forceinline__ void pml_apply(const StageArgs<TS, TC> &a, int shot, 
     const TC e1 = (dim == 0) ? xix : (dim == 1) ? etx : ztx;
     const TC e2 = (dim == 0) ? xiy : (dim == 1) ? ety : zty;
     const TC e3 = (dim == 0) ? xiz : (dim == 1) ? etz : ztz;
-    pml_face<TS, TC>(a, fi, shot, i, j, k, Dd, e1, e2, e3, ztx, zty, ztz, lam, mu, lam2mu, slw, h);
+    pml_face<MODE, TS, TC>(a, fi, shot, i, j, k, Dd, e1, e2, e3, ztx, zty, ztz, lam, mu, lam2mu, slw, h);
   }
 }
 

@@ -29,8 +29,10 @@ __device__ __forceinline__ TC drp5(F f, long s) {
          Drp<TC, DIR>::c(4) * f((lo + 4) * s);
 }
 
-// RK update of the wavefield at one point from its right-hand side h
-template <typename TS, typename TC>
+// RK update of the wavefield at one point from its right-hand side h (MODE:
+// the RK mode, a compile-time constant, so each mode's loads are
+// unconditional and issue together)
+template <int MODE, typename TS, typename TC>
 __device__ __forceinline__ void rk_store(const StageArgs<TS, TC> &a, long so, int o, const TC *h) {
   const long cs = a.L.cs;
   const unsigned qb = (unsigned)o * (unsigned)sizeof(TS);
@@ -41,13 +43,13 @@ __device__ __forceinline__ void rk_store(const StageArgs<TS, TC> &a, long so, in
     const TS *win = a.w_in + so + c * cs;
     const TC sc = St<TS>::scaled ? (TC)a.sc[c] : TC(1);
     const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
-    if (a.mode == MODE_RHS) {
+    if constexpr (MODE == MODE_RHS) {
       St<TS>::st(&at_b(wout, qb), h[c] * is);
-    } else if (a.mode == MODE_S0) {
+    } else if constexpr (MODE == MODE_S0) {
       const TC v = (TC)St<TS>::ld(&at_b(win, qb)) * sc;
       St<TS>::st(&at_b(wout, qb), (v + a.ca * h[c]) * is);
       St<TS>::st(&at_b(wend, qb), (v + a.cb * h[c]) * is);
-    } else if (a.mode == MODE_S12) {
+    } else if constexpr (MODE == MODE_S12) {
       const TC pre = (TC)St<TS>::ld(&at_b(wpre, qb)) * sc;
       const TC en = (TC)St<TS>::ld(&at_b(wend, qb)) * sc;
       St<TS>::st(&at_b(wout, qb), (pre + a.ca * h[c]) * is);
@@ -74,7 +76,7 @@ __device__ __forceinline__ bool box_point(const StageArgs<TS, TC> &a, long gid, 
   return true;
 }
 
-template <int DX, int DY, int DZ, typename TS, typename TC>
+template <int DX, int DY, int DZ, int MODE, typename TS, typename TC>
 __device__ __forceinline__ void point_update(const StageArgs<TS, TC> &a, int shot, int i, int j, int k) {
   const Layout &L = a.L;
   // element offsets inside one component stay below 2^31: 32-bit offsets from
@@ -195,19 +197,20 @@ __device__ __forceinline__ void point_update(const StageArgs<TS, TC> &a, int sho
   h[TYZ] = mu * (xiz * Dx[VY] + xiy * Dx[VZ] + etz * Dy[VY] + ety * Dy[VZ] + ztz * DzVy + zty * DzVz);
 
   // ---- CFS-PML faces, in the reference's face order ----
-  pml_apply<TS, TC>(a, shot, i, j, k, Dx, Dy, Dz, xix, xiy, xiz, etx, ety, etz, ztx, zty, ztz, lam, mu, lam2mu, slw, h);
+  pml_apply<MODE, TS, TC>(a, shot, i, j, k, Dx, Dy, Dz, xix, xiy, xiz, etx, ety, etz, ztx, zty, ztz, lam, mu, lam2mu,
+                          slw, h);
 
-  rk_store<TS, TC>(a, so, o, h);
+  rk_store<MODE, TS, TC>(a, so, o, h);
 }
 
-template <int DX, int DY, int DZ, typename TS, typename TC>
+template <int DX, int DY, int DZ, int MODE, typename TS, typename TC>
 __global__ __launch_bounds__(256) void k_point(StageArgs<TS, TC> a) {
   const int shot = blockIdx.y;
   if (a.shot_active && !a.shot_active[shot]) return;
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   if (!box_point(a, gid, i, j, k)) return;
-  point_update<DX, DY, DZ, TS, TC>(a, shot, i, j, k);
+  point_update<DX, DY, DZ, MODE, TS, TC>(a, shot, i, j, k);
 }
 
 // ---- source injection (sv_curv_col_el_iso_rhs_src_gpu, sv_curv_col_el_gpu.cu:227-301) ----
@@ -376,21 +379,35 @@ hipError_t launch_gather(const TS *w, const Layout &L, int i1, int j1, int k1, i
   return hipGetLastError();
 }
 
+// one RK mode's k_point for the operator directions sel = 4 dx + 2 dy + dz
+template <int MODE, typename TS, typename TC>
+static void launch_point_m(const StageArgs<TS, TC> &a, dim3 grid, int sel, hipStream_t st) {
+  const dim3 block(256);
+#define CGFD_L(X, Y, Z) hipLaunchKernelGGL((k_point<X, Y, Z, MODE, TS, TC>), grid, block, 0, st, a)
+  switch (sel) {
+    case 0: CGFD_L(0, 0, 0); break;
+    case 1: CGFD_L(0, 0, 1); break;
+    case 2: CGFD_L(0, 1, 0); break;
+    case 3: CGFD_L(0, 1, 1); break;
+    case 4: CGFD_L(1, 0, 0); break;
+    case 5: CGFD_L(1, 0, 1); break;
+    case 6: CGFD_L(1, 1, 0); break;
+    default: CGFD_L(1, 1, 1); break;
+  }
+#undef CGFD_L
+}
+
 template <typename TS, typename TC>
 hipError_t launch_point(const StageArgs<TS, TC> &a, int dx, int dy, int dz, int nshots, hipStream_t st) {
   const long n = a.nbox ? a.box_end[a.nbox - 1] : 0;
   if (n == 0) return hipSuccess;
   dim3 grid((unsigned)((n + 255) / 256), (unsigned)nshots), block(256);
   const int sel = dx * 4 + dy * 2 + dz;
-  switch (sel) {
-    case 0: hipLaunchKernelGGL((k_point<0, 0, 0, TS, TC>), grid, block, 0, st, a); break;
-    case 1: hipLaunchKernelGGL((k_point<0, 0, 1, TS, TC>), grid, block, 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_point<0, 1, 0, TS, TC>), grid, block, 0, st, a); break;
-    case 3: hipLaunchKernelGGL((k_point<0, 1, 1, TS, TC>), grid, block, 0, st, a); break;
-    case 4: hipLaunchKernelGGL((k_point<1, 0, 0, TS, TC>), grid, block, 0, st, a); break;
-    case 5: hipLaunchKernelGGL((k_point<1, 0, 1, TS, TC>), grid, block, 0, st, a); break;
-    case 6: hipLaunchKernelGGL((k_point<1, 1, 0, TS, TC>), grid, block, 0, st, a); break;
-    default: hipLaunchKernelGGL((k_point<1, 1, 1, TS, TC>), grid, block, 0, st, a); break;
+  switch (a.mode) {
+    case MODE_S0: launch_point_m<MODE_S0>(a, grid, sel, st); break;
+    case MODE_S12: launch_point_m<MODE_S12>(a, grid, sel, st); break;
+    case MODE_S3: launch_point_m<MODE_S3>(a, grid, sel, st); break;
+    default: launch_point_m<MODE_RHS>(a, grid, sel, st); break;
   }
   return hipGetLastError();
 }
