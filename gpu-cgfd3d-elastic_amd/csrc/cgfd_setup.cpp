@@ -443,7 +443,7 @@ namespace {
 // the points on a layer boundary and divide the harmonic rho sum in float.
 template <typename Mark, typename Point>
 int equivalent_medium(const G3 &g, const double *x, const double *y, const double *z, bool har, int nmat,
-                      bool grid_rules, std::vector<int> &err, Mark mark, Point point, int nelev, double *lam,
+                      bool grid_rules, Mark mark, Point point, int nelev, double *lam,
                       double *mu, double *rho) {
   const long n = g.n;
   std::vector<float> hx(n), hy(n), hz(n);
@@ -553,7 +553,6 @@ int equivalent_medium(const G3 &g, const double *x, const double *y, const doubl
                                            : "vp must be larger than sqrt(2) vs in the media file");
       return CGFD_E_ARG;
     }
-  (void)err;
   return CGFD_OK;
 }
 
@@ -725,7 +724,7 @@ int cgfd_media_layer_file(int ni, int nj, int nk, const double *x, const double 
   if (fail()) return CGFD_E_ARG;
   if (loc) return CGFD_OK;
 
-  return equivalent_medium(g, x, y, z, har, L.ni, false, err,
+  return equivalent_medium(g, x, y, z, har, L.ni, false,
                            [&](float hx, float hy, float hz, std::vector<float> &elev) {
                              const size_t slice = (size_t)L.nx * L.ny;
                              for (int l = 0; l < L.ni; l++) elev[l] = E.bilinear(&L.elev[l * slice], hx, hy);
@@ -794,7 +793,7 @@ int cgfd_media_grid_file(int ni, int nj, int nk, const double *x, const double *
   // layers are numbered by the top surface of each (LayerNumberAtPoint,
   // media_grid2model.cpp:1269-1307)
   const size_t slice = (size_t)G.nx * G.ny;
-  return equivalent_medium(g, x, y, z, har, G.nl, true, err,
+  return equivalent_medium(g, x, y, z, har, G.nl, true,
                            [&](float hx, float hy, float hz, std::vector<float> &elev) {
                              elev.resize(G.nl);
                              for (int l = 0; l < G.nl; l++) elev[l] = E.bilinear(&G.elev[G.top[l] * slice], hx, hy);
