@@ -733,7 +733,16 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
     s->split = split;
     const Shell sh = make_shell(s);
     const int nz = sh.inner.box[0][5] - sh.inner.box[0][4] + 1;
-    const int zcands[4] = {16, 24, 32, 48};
+    // z chunks near 16..48 planes that split the swept planes evenly: every
+    // chunk re-reads 4 planes below it, so a short last chunk costs almost a
+    // full one (196 planes: 17, 20, 25, 28, 33, 49 instead of 16 .. 48,
+    // whose last chunk has 4 planes)
+    const int ztarget[6] = {16, 20, 24, 28, 32, 48};
+    int zcands[6];
+    for (int zi = 0; zi < 6; zi++) {
+      const int n = std::max(1, (nz + ztarget[zi] / 2) / ztarget[zi]);
+      zcands[zi] = (nz + n - 1) / n;
+    }
     float best = 1e30f;
     int ts = 0, tz = 24;
     for (int shape = 0; shape < interior_num_shapes(); shape++) {
@@ -741,9 +750,9 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
       // 64 x 8); two-column tiles exist for 16-bit storage only
       if (sh.inner.pml && shape >= 4) continue;
       if (shape >= 7 && sizeof(TS) != 2) continue;
-      for (int zi = 0; zi < 4; zi++) {
-        int zc = std::min(zcands[zi], nz);
-        if (zi > 0 && zc == std::min(zcands[zi - 1], nz)) continue;
+      for (int zi = 0; zi < 6; zi++) {
+        const int zc = zcands[zi];
+        if (zi > 0 && zc == zcands[zi - 1]) continue;
         MarchSet ib = sh.inner;
         ib.shape = shape;
         ib.zchunk = zc;
