@@ -18,6 +18,7 @@ DT_F32, DT_F64 = 0, 1
 LEVEL_PRE, LEVEL_TMPA, LEVEL_TMPB, LEVEL_END, LEVEL_RHS = 0, 1, 2, 3, 4
 FLAG_PLAIN = 1
 FLAG_PML_POINT = 2
+FLAG_FULL_METRIC = 4
 SPLIT_SHELL = 0
 SPLIT_PLANES = 1
 

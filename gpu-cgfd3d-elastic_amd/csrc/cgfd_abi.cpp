@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -262,6 +263,10 @@ void drop_graphs(cgfd_session *s) {
 
 // kernel arguments changed: captured periods no longer apply
 void invalidate(cgfd_session *s) { s->epoch++; }
+
+// the kernels address one component (or one level of a face's memory
+// variables) with 32-bit byte offsets from a uniform base
+constexpr unsigned long long kMaxComponentBytes = 1ull << 32;
 
 Layout make_layout(int ni, int nj, int nk, size_t es) {
   Layout L;
@@ -665,31 +670,43 @@ int launch_stage(cgfd_session *s, const Shell &sh, StageArgs<TS, TC> &a, int dx,
 // tuning between steps leaves the state intact.  Results are cached per
 // (grid, storage, shots, work split) for the life of the process.
 struct TileKey {
-  int ni, nj, nk, storage, nshots, flags, met_const;
+  int ni, nj, nk, storage, nshots, flags, met_const, free_top;
+  std::array<int, 6> nlay;  // PML layers per face (0: none)
   bool operator<(const TileKey &o) const {
-    return std::tie(ni, nj, nk, storage, nshots, flags, met_const) <
-           std::tie(o.ni, o.nj, o.nk, o.storage, o.nshots, o.flags, o.met_const);
+    return std::tie(ni, nj, nk, storage, nshots, flags, met_const, free_top, nlay) <
+           std::tie(o.ni, o.nj, o.nk, o.storage, o.nshots, o.flags, o.met_const, o.free_top, o.nlay);
   }
 };
 struct TileChoice {
   int shape, zchunk, split;
 };
+// shared by every session of the process; sessions may tune on several
+// threads at once (the shot server's worker and callers of their own)
 std::map<TileKey, TileChoice> g_tile_cache;
+std::mutex g_tile_mu;
 
 template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
-  const TileKey key{s->d.ni, s->d.nj, s->d.nk, s->d.storage, s->d.nshots, s->d.flags, s->met_const ? 1 : 0};
-  auto hit = g_tile_cache.find(key);
-  if (hit != g_tile_cache.end()) {
-    s->tile_shape = hit->second.shape;
-    s->tile_zchunk = hit->second.zchunk;
-    s->split = hit->second.split;
-    return CGFD_OK;
+  TileKey key{s->d.ni, s->d.nj, s->d.nk, s->d.storage, s->d.nshots, s->d.flags, s->met_const ? 1 : 0, s->free_top, {}};
+  for (int f = 0; f < 6; f++) key.nlay[f] = s->face[f].on ? s->face[f].nlay : 0;
+  {
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    auto hit = g_tile_cache.find(key);
+    if (hit != g_tile_cache.end()) {
+      s->tile_shape = hit->second.shape;
+      s->tile_zchunk = hit->second.zchunk;
+      s->split = hit->second.split;
+      return CGFD_OK;
+    }
   }
+  auto remember = [&](TileChoice c) {
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    g_tile_cache[key] = c;
+  };
   s->tile_shape = 0;
   s->split = CGFD_SPLIT_PLANES;
   const Shell planes = make_shell(s);
   if (!planes.inner.nbox) {
-    g_tile_cache[key] = {s->tile_shape, s->tile_zchunk, s->split};
+    remember({s->tile_shape, s->tile_zchunk, s->split});
     return CGFD_OK;
   }
   const int PRE = s->slot[0], TA = s->slot[1], TB = s->slot[2], END = s->slot[3];
@@ -737,9 +754,10 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
     // chunk re-reads 4 planes below it, so a short last chunk costs almost a
     // full one (196 planes: 17, 20, 25, 28, 33, 49 instead of 16 .. 48,
     // whose last chunk has 4 planes)
-    const int ztarget[6] = {16, 20, 24, 28, 32, 48};
-    int zcands[6];
-    for (int zi = 0; zi < 6; zi++) {
+    constexpr int kNZ = 8;
+    const int ztarget[kNZ] = {16, 20, 24, 28, 32, 48, 64, 96};
+    int zcands[kNZ];
+    for (int zi = 0; zi < kNZ; zi++) {
       const int n = std::max(1, (nz + ztarget[zi] / 2) / ztarget[zi]);
       zcands[zi] = (nz + n - 1) / n;
     }
@@ -750,7 +768,7 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
       // 64 x 8); two-column tiles exist for 16-bit storage only
       if (sh.inner.pml && shape >= 4) continue;
       if (shape >= 7 && sizeof(TS) != 2) continue;
-      for (int zi = 0; zi < 6; zi++) {
+      for (int zi = 0; zi < kNZ; zi++) {
         const int zc = zcands[zi];
         if (zi > 0 && zc == zcands[zi - 1]) continue;
         MarchSet ib = sh.inner;
@@ -781,7 +799,7 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
   s->tile_shape = bs;
   s->tile_zchunk = bz;
   s->split = bsplit;
-  g_tile_cache[key] = {bs, bz, bsplit};
+  remember({bs, bz, bsplit});
   invalidate(s);
   return CGFD_OK;
 }
@@ -1001,16 +1019,21 @@ long long cgfd_footprint_bytes(const cgfd_desc *d, int nrecv, int nt_total) {
   if (!d || d->ni <= 0 || d->nj <= 0 || d->nk <= 0) return -1;
   size_t es = d->storage == CGFD_STORE_F64 ? 8 : d->storage == CGFD_STORE_F32 ? 4 : 2;
   size_t ms = d->storage == CGFD_STORE_F64 ? 8 : 4;
+  // model arrays share the wavefield's layout (padded for the storage type)
   Layout L = make_layout(d->ni, d->nj, d->nk, es);
-  Layout Lm = make_layout(d->ni, d->nj, d->nk, ms);
   long long ns = d->nshots > 0 ? d->nshots : 1;
   long long wav = ns * 4LL * kNC * L.cs * (long long)es;
-  long long model = (long long)(NMET + 3) * Lm.cs * (long long)ms + 18LL * d->ni * d->nj * (long long)ms;
+  long long model = (long long)(NMET + 3) * L.cs * (long long)ms + 18LL * d->ni * d->nj * (long long)ms;
+  // peak-motion maps (when enabled), the sub-volume gather buffer (at most
+  // one component) and the source tables (7^3 points, 9 tables of 4 stage
+  // values per step, per shot)
+  long long extra = ns * 18LL * d->ni * d->nj * (long long)ms + L.cs * (long long)ms +
+                    ns * (343LL * 24 + 9LL * 4 * (nt_total > 0 ? nt_total : 0) * (long long)ms);
   // PML memory: 5 faces of 11 layers at most is bounded by 5 levels over the shell
   long long shell = 2LL * 11 * d->nj * d->nk + 2LL * 11 * d->ni * d->nk + 2LL * 11 * d->ni * d->nj;
   long long aux = ns * 5LL * kNC * shell * (long long)es;
   long long rec = ns * (long long)nrecv * kNC * (long long)(nt_total > 0 ? nt_total : 0) * (long long)ms;
-  return wav + model + aux + rec;
+  return wav + model + aux + rec + extra;
 }
 
 int cgfd_create(const cgfd_desc *d, cgfd_session **out) {
@@ -1019,6 +1042,15 @@ int cgfd_create(const cgfd_desc *d, cgfd_session **out) {
   if (d->ni < 1 || d->nj < 1 || d->nk < 1) return fail(CGFD_E_ARG, "grid %d x %d x %d invalid", d->ni, d->nj, d->nk);
   if (d->storage < 0 || d->storage > 3) return fail(CGFD_E_ARG, "storage %d invalid", d->storage);
   if (d->nshots < 1) return fail(CGFD_E_ARG, "nshots %d invalid", d->nshots);
+  {
+    // kernels address inside one component with 32-bit byte offsets
+    const size_t es = d->storage == CGFD_STORE_F64 ? 8 : d->storage == CGFD_STORE_F32 ? 4 : 2;
+    const size_t ms = d->storage == CGFD_STORE_F64 ? 8 : 4;
+    const Layout L = make_layout(d->ni, d->nj, d->nk, es);
+    if ((unsigned long long)L.cs * std::max(es, ms) >= kMaxComponentBytes)
+      return fail(CGFD_E_ARG, "grid %d x %d x %d: one component needs %llu bytes, over the 4 GiB kernels address",
+                  d->ni, d->nj, d->nk, (unsigned long long)L.cs * std::max(es, ms));
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
     (void)hipGetLastError();
@@ -1085,13 +1117,23 @@ int cgfd_upload_metric(cgfd_session *s, int dtype, const void *metric) {
   int r = put_model(s, s->met, v);
   if (r) return r;
   s->have_met = true;
-  // a Cartesian grid gives one value per metric component everywhere (ghosts too)
+  // A Cartesian grid gives one value per metric component everywhere (ghosts
+  // too), up to the rounding of the metric computation: its off-diagonal
+  // terms come out as noise of ~1e-14 of the diagonal instead of 0.  The
+  // grid counts as constant when every component stays within 1e-10 of the
+  // largest metric magnitude of its first value; the sweep then takes the
+  // first point's nine values as arguments (a change below 1e-10 relative
+  // in any derivative, far under FP32 rounding).  CGFD_FLAG_FULL_METRIC
+  // keeps the per-point arrays.
   const size_t n = (size_t)(s->d.ni + 6) * (s->d.nj + 6) * (s->d.nk + 6);
-  bool cst = true;
+  double scale = 0.0;
+  for (int c = 0; c < 9; c++) scale = std::max(scale, std::fabs(rd(dtype, metric, (size_t)(c + 1) * n)));
+  const double tol = 1e-10 * scale;
+  bool cst = !(s->d.flags & CGFD_FLAG_FULL_METRIC) && scale > 0.0;
   for (int c = 0; c < 9 && cst; c++) {
     const double v0 = rd(dtype, metric, (size_t)(c + 1) * n);
     for (size_t q = 1; q < n; q++)
-      if (rd(dtype, metric, (size_t)(c + 1) * n + q) != v0) { cst = false; break; }
+      if (std::fabs(rd(dtype, metric, (size_t)(c + 1) * n + q) - v0) > tol) { cst = false; break; }
     s->cmet[c] = v0;
   }
   s->met_const = cst;
@@ -1149,6 +1191,9 @@ int cgfd_upload_pml(cgfd_session *s, int face, int nlay, int dtype, const void *
   if (nlay + 1 > axis_n) return fail(CGFD_E_ARG, "face %d: %d layers exceed the grid", face, nlay);
   if (!A || !B || !D) return fail(CGFD_E_ARG, "face %d: null profile", face);
   face_box(s, face, nlay, F);
+  if ((unsigned long long)kNC * F.fsize * s->es >= kMaxComponentBytes)
+    return fail(CGFD_E_ARG, "face %d: %llu bytes of memory variables per level, over the 4 GiB kernels address", face,
+                (unsigned long long)kNC * F.fsize * s->es);
   F.nlay = nlay;
   std::vector<double> v(3 * (size_t)(nlay + 1));
   for (int n = 0; n <= nlay; n++) {

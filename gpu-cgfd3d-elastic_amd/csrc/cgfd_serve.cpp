@@ -114,6 +114,7 @@ struct cgfd_server {
   bool admit_one(Request &r);
   Group *make_group(int model, std::string &why);
   void retire(Group &g);
+  void fail_group(Group &g, const std::string &why);
 };
 
 namespace {
@@ -203,6 +204,12 @@ Group *cgfd_server::make_group(int model, std::string &why) {
 
 // put one queued request into a free slot of its model's group
 bool cgfd_server::admit_one(Request &r) {
+  {
+    // the request may have been rejected by cgfd_server_destroy since the
+    // worker copied the queue; a stopping server admits nothing
+    std::lock_guard<std::mutex> lk(mu);
+    if (stop || r.state != QUEUED) return false;
+  }
   Group *g = nullptr;
   for (auto &p : groups)
     if (p->model == r.model && !p->inflight && p->active() < p->cap) { g = p.get(); break; }
@@ -236,11 +243,15 @@ bool cgfd_server::admit_one(Request &r) {
   if (!rc) rc = cgfd_upload_receivers(s, q, r.nrecv, r.ridx.data(), r.rdtype, r.rfrac.data(), r.nt);
   if (!rc) rc = cgfd_set_shot(s, q, 1, g->clock);
   std::lock_guard<std::mutex> lk(mu);
-  if (rc) {
-    r.state = REJECTED;
-    r.error = cgfd_last_error();
-    r.t_done = Clock::now();
-    st.rejected++;
+  if (rc || r.state != QUEUED) {
+    if (!rc) rc = CGFD_E_STATE;  // rejected meanwhile: free the slot again
+    cgfd_set_shot(s, q, 0, 0);
+    if (r.state == QUEUED) {
+      r.state = REJECTED;
+      r.error = cgfd_last_error();
+      r.t_done = Clock::now();
+      st.rejected++;
+    }
     cv_done.notify_all();
     return true;
   }
@@ -249,6 +260,25 @@ bool cgfd_server::admit_one(Request &r) {
   r.t_admit = Clock::now();
   st.admitted++;
   return true;
+}
+
+// end every shot of the group with an error and free its slots
+void cgfd_server::fail_group(Group &g, const std::string &why) {
+  std::lock_guard<std::mutex> lk(mu);
+  for (int q = 0; q < g.cap; q++) {
+    Slot &sl = g.slot[q];
+    if (!sl.busy) continue;
+    auto it = reqs.find(sl.ticket);
+    if (it != reqs.end()) {
+      it->second->state = REJECTED;
+      it->second->error = why;
+      it->second->t_done = Clock::now();
+      st.rejected++;
+    }
+    sl = Slot{};
+  }
+  g.inflight = false;
+  cv_done.notify_all();
 }
 
 // collect every shot of the group that has run its steps
@@ -292,12 +322,22 @@ void cgfd_server::run() {
   for (;;) {
     // retire finished chunks
     bool any_inflight = false;
-    for (auto &g : groups) {
-      if (g->inflight && cgfd_poll(g->s) == 0) {
-        g->inflight = false;
-        retire(*g);
+    for (size_t gi = 0; gi < groups.size();) {
+      Group &g = *groups[gi];
+      int pr = g.inflight ? cgfd_poll(g.s) : 1;
+      if (g.inflight && pr == 0) {
+        g.inflight = false;
+        retire(g);
+      } else if (g.inflight && pr < 0) {
+        // the group's stream failed: its shots end with the error and the
+        // group is dropped, so the worker never waits on it again
+        fail_group(g, cgfd_last_error());
+        cgfd_destroy(g.s);
+        groups.erase(groups.begin() + (long)gi);
+        continue;
       }
-      any_inflight |= g->inflight;
+      any_inflight |= g.inflight;
+      gi++;
     }
     // admit queued requests, oldest first, wherever a slot and memory allow
     std::vector<long long> todo;
@@ -309,7 +349,7 @@ void cgfd_server::run() {
         for (auto &g : groups) idle &= g->active() == 0;
         if (idle) cv_work.wait_for(lk, std::chrono::milliseconds(50));
       }
-      todo.assign(pending.begin(), pending.end());
+      if (!stop) todo.assign(pending.begin(), pending.end());
     }
     std::vector<long long> admitted;
     for (long long t : todo) {
@@ -322,7 +362,10 @@ void cgfd_server::run() {
     }
     {
       std::lock_guard<std::mutex> lk(mu);
-      for (long long t : admitted) pending.erase(std::find(pending.begin(), pending.end(), t));
+      for (long long t : admitted) {
+        auto it = std::find(pending.begin(), pending.end(), t);
+        if (it != pending.end()) pending.erase(it);
+      }
     }
     // Launch the next chunk of idle groups with shots on board: up to the
     // first shot that finishes, at most chunk_steps while the group has free
@@ -355,20 +398,7 @@ void cgfd_server::run() {
       int len = (std::max(left, 1) + 7) / 8 * 8;
       len = std::min(len, g->active() < g->cap ? d.chunk_steps : 8 * d.chunk_steps);
       if (cgfd_run_async(g->s, g->clock, len, m.d.dt) != CGFD_OK) {
-        // fail every shot of the group
-        std::lock_guard<std::mutex> lk(mu);
-        for (Slot &sl : g->slot) {
-          if (!sl.busy) continue;
-          auto it = reqs.find(sl.ticket);
-          if (it != reqs.end()) {
-            it->second->state = REJECTED;
-            it->second->error = cgfd_last_error();
-            it->second->t_done = Clock::now();
-            st.rejected++;
-          }
-          sl = Slot{};
-        }
-        cv_done.notify_all();
+        fail_group(*g, cgfd_last_error());
         continue;
       }
       g->clock += len;
@@ -434,8 +464,11 @@ void cgfd_server_destroy(cgfd_server *srv) {
     // queued requests will not run
     for (long long t : srv->pending) {
       Request &r = *srv->reqs[t];
+      if (r.state != QUEUED) continue;
       r.state = REJECTED;
       r.error = "server shut down";
+      r.t_done = Clock::now();
+      srv->st.rejected++;
     }
     srv->pending.clear();
   }

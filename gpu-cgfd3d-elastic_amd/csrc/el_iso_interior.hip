@@ -44,9 +44,17 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
   constexpr int LW = TX + 4, LH = TY + 4;
   constexpr int zlo = (DZ == 0) ? -3 : -1;
   constexpr int zc0 = -zlo;                       // queue slot of the centre plane
-  constexpr int NXH = 4 * TY, NYH = 4 * TX, NH = NXH + NYH;  // halo cells per component
-  constexpr int NHT = kNC * NH;                   // halo cells per plane
-  constexpr int NPER = (NHT + NT - 1) / NT;       // halo loads per thread
+  // Halo cells per component: 4 rows of TX (y halo) and 4 columns of TY (x
+  // halo).  A thread owns one y-halo cell and one x-halo cell, the same for
+  // every component it loads: NT / YH components per pass of the y halo
+  // (wave-uniform component, so each load has a scalar base) and NT / XH per
+  // pass of the x halo.
+  constexpr int YH = 4 * TX, XH = 4 * TY;
+  static_assert(NT % YH == 0 && NT % XH == 0, "tile must hold whole halo passes");
+  constexpr int YPC = NT / YH, XPC = NT / XH;           // components per pass
+  constexpr int NPY = (kNC + YPC - 1) / YPC;            // y-halo passes
+  constexpr int NPX = (kNC + XPC - 1) / XPC;            // x-halo passes
+  constexpr int NPER = NPY + NPX;                       // halo loads per thread
   __shared__ TC lds[2][kNC][LH][LW];
 
   // XCD-aware remap: consecutive hardware workgroup ids land on different
@@ -108,35 +116,33 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
     return (TC)St<TS>::ld(&at_b(wq(c), off * (unsigned)sizeof(TS))) * sc(c);
   };
 
-  // this thread's halo cells: compile-time count, positions fixed over the sweep
-  // LDS slot (x16) and component of each halo cell in one register; -1: none
-  int h_lds[NPER];
-  unsigned h_off[NPER];
-#pragma unroll
-  for (int p = 0; p < NPER; p++) {
-    const int e = tid + p * NT;
-    const int ee = e < NHT ? e : NHT - 1;
-    const int c = ee / NH, r = ee % NH;
-    int hi, hj, li, lj;
-    if (r < NXH) {
-      const int h = r % 4, row = r / 4;
-      li = (h < HXL) ? h : TX + h;
-      hi = i0 - HXL + li;
-      lj = row + HYL;
-      hj = j0 + row;
-    } else {
-      const int rr = r - NXH;
-      const int h = rr / TX, cc = rr % TX;
-      lj = (h < HYL) ? h : TY + h;
-      hj = j0 - HYL + lj;
-      li = cc + HXL;
-      hi = i0 + cc;
-    }
+  // this thread's halo cells (positions fixed over the sweep)
+  const int ycb = tid / YH, yr = tid % YH;   // y halo: first component, cell
+  const int xcb = tid / XH, xr = tid % XH;   // x halo: first component, cell
+  unsigned y_off, x_off;
+  int y_lds, x_lds;
+  {
+    const int h = yr / TX, cc = yr % TX;
+    const int lj = (h < HYL) ? h : TY + h;
+    int hj = j0 - HYL + lj;
+    int hi = i0 + cc;
     hi = hi <= L.ni + 2 ? hi : L.ni + 2;
     hj = hj <= L.nj + 2 ? hj : L.nj + 2;
-    h_lds[p] = e < NHT ? ((c * LH + lj) * LW + li) * 16 + c : -1;
-    h_off[p] = (unsigned)(hj + kNG) * upx + (unsigned)(hi + L.xa);
+    y_lds = lj * LW + cc + HXL;
+    y_off = (unsigned)(hj + kNG) * upx + (unsigned)(hi + L.xa);
   }
+  {
+    const int h = xr % 4, row = xr / 4;
+    const int li = (h < HXL) ? h : TX + h;
+    int hi = i0 - HXL + li;
+    int hj = j0 + row;
+    hi = hi <= L.ni + 2 ? hi : L.ni + 2;
+    hj = hj <= L.nj + 2 ? hj : L.nj + 2;
+    x_lds = (row + HYL) * LW + li;
+    x_off = (unsigned)(hj + kNG) * upx + (unsigned)(hi + L.xa);
+  }
+  // component of halo load p (y passes first), clamped; valid when < kNC
+  auto hcomp = [&](int p) -> int { return p < NPY ? p * YPC + ycb : (p - NPY) * XPC + xcb; };
   TC *ldsf = &lds[0][0][0][0];
   constexpr int LBUF = kNC * LH * LW;  // one LDS plane buffer
 
@@ -179,7 +185,10 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
 #pragma unroll
     for (int c = 0; c < kNC; c++) n_lead[c] = gld(c, (unsigned)(kk + zlo + 4 + kNG) * upz + col);
 #pragma unroll
-    for (int p = 0; p < NPER; p++) n_hv[p] = gld(h_lds[p] < 0 ? kNC - 1 : (h_lds[p] & 15), pk + h_off[p]);
+    for (int p = 0; p < NPER; p++) {
+      const int c = hcomp(p);
+      n_hv[p] = gld(c < kNC ? c : kNC - 1, pk + (p < NPY ? y_off : x_off));
+    }
     const unsigned ob = o * (unsigned)sizeof(TC);
 #pragma unroll
     for (int m = 0; m < 9; m++) n_met[m] = CART ? a.cmet[m] : (at_b(a.met + (XIX + m) * cs, ob));
@@ -222,7 +231,7 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
         pre_v[c] = need_pre ? (TC)ld_rk(&at_b(a.w_pre + so + c * cs, qb)) * sc(c) : TC(0);
       }
     }
-    issue(k < kend ? k + 1 : kend);
+    if (k < kend) issue(k + 1);
     const TC xix = mt[0], xiy = mt[1], xiz = mt[2], etx = mt[3], ety = mt[4], etz = mt[5];
     const TC ztx = mt[6], zty = mt[7], ztz = mt[8], lam = mt[9], mu = mt[10], slw = mt[11];
     // ---- stage the centre plane and its halo in LDS (double buffered) ----
@@ -230,8 +239,10 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
 #pragma unroll
     for (int c = 0; c < kNC; c++) lds[buf][c][ly + HYL][lx + HXL] = q[c][zc0];
 #pragma unroll
-    for (int p = 0; p < NPER; p++)
-      if (h_lds[p] >= 0) ldsf[buf * LBUF + (h_lds[p] >> 4)] = hv[p];
+    for (int p = 0; p < NPER; p++) {
+      const int c = hcomp(p);
+      if (c < kNC) ldsf[buf * LBUF + c * (LH * LW) + (p < NPY ? y_lds : x_lds)] = hv[p];
+    }
     __syncthreads();
 
     // each component's derivatives fold straight into the velocity gradient
@@ -465,7 +476,7 @@ __device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf
     }
 #pragma unroll
     for (int m = 0; m < 9 + NMD; m++) mt[m] = n_met[m];
-    issue(k < kend ? k + 1 : kend);
+    if (k < kend) issue(k + 1);
 #pragma unroll
     for (int c = 0; c < NOUT; c++) {
       pre_v[c] = need_pre ? (TC)St<TS>::ld(&at_b(a.w_pre + so + (OOFF + c) * cs, qb)) * sc(OOFF + c) : TC(0);

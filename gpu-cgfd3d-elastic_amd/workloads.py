@@ -63,6 +63,8 @@ def small(seed=1, nt=500, ni=100, nj=100, nk=60):
     m["slw"] = cgfd.rho_to_slow(rho)
     m["mvx"], m["mvy"] = cgfd.free_surface_matrices(ni, nj, nk, metric, lam, mu)
     m["pml"] = _pml_faces(m, ni, nj, nk, x, y, z, 6000.0)
+    m["src_params"] = dict(sx=ni / 2, sy=nj / 2, depth=30.0, f0=2.0, t0=0.6, m6=[1e16, 1e16, 1e16, 0, 0, 0],
+                           pml_vel=6000.0)
     m["src"] = _source(ni, nj, nk, ni / 2, nj / 2, 30.0, m["dt"], 2.0, 0.6, [1e16, 1e16, 1e16, 0, 0, 0],
                        m["nt_total"])
     m["recv_indx"], m["recv_frac"] = _surface_receivers(ni, nj, nk, 20, seed, PML["nlay"] + 1)
@@ -86,6 +88,8 @@ def topo(seed=2, nt=1000, ni=400, nj=400, nk=200):
     strike, dip, rake = rng.uniform(0, 360), rng.uniform(20, 80), rng.uniform(-180, 180)
     m["mech"] = (strike, dip, rake)
     m6 = cgfd.angle2moment(strike, dip, rake) * 1e16
+    m["src_params"] = dict(sx=ni / 2, sy=nj / 2, depth=30.0, f0=1.5, t0=0.8, m6=list(m6),
+                           pml_vel=float(vp.max() * 1.05))
     m["src"] = _source(ni, nj, nk, ni / 2, nj / 2, 30.0, m["dt"], 1.5, 0.8, m6, m["nt_total"])
     m["recv_indx"], m["recv_frac"] = _surface_receivers(ni, nj, nk, 400, seed, PML["nlay"] + 1)
     return m

@@ -74,6 +74,10 @@ typedef struct {
  * box inside them (by default the z-marching kernel sweeps whole planes,
  * PML included, and the per-point kernel takes the 4 free-surface rows) */
 #define CGFD_FLAG_PML_POINT 2
+/* keep the per-point metric arrays even when the grid is Cartesian (by
+ * default a metric whose components are constant to 1e-10 of their largest
+ * magnitude runs the constant-metric sweep with the first point's values) */
+#define CGFD_FLAG_FULL_METRIC 4
 
 /* Point-source tables of one shot: the device mirror of src_t
  * (src/forward/src_t.h:14-53, filled by src_read_locate_file src_t.cu:143-790). */

@@ -60,3 +60,28 @@ def test_server_arguments_checked_before_device_use():
         cgfd.Server(max_batch=0)
     with pytest.raises(cgfd.CgfdError, match="chunk_steps"):
         cgfd.Server(chunk_steps=12)
+
+
+def test_grid_over_32bit_component_offsets_is_refused_before_device_use():
+    """Kernels address a component with 32-bit byte offsets: FP64 at 800^3 or FP32 at 1024^3 must be refused."""
+    lib = cgfd.load()
+    for d in (cgfd.Desc(800, 800, 800, cgfd.STORE_F64, 1, 0), cgfd.Desc(1024, 1024, 1024, cgfd.STORE_F32, 1, 0)):
+        out = C.c_void_p()
+        assert lib.cgfd_create(C.byref(d), C.byref(out)) == -1          # CGFD_E_ARG, not CGFD_E_NODEV
+        assert b"4 GiB" in lib.cgfd_last_error()
+        assert not out.value
+    # the C3 precision grid stays inside the limit in every storage type
+    for st in (cgfd.STORE_F32, cgfd.STORE_F16, cgfd.STORE_BF16, cgfd.STORE_F64):
+        ok = cgfd.Desc(800, 800, 400, st, 1, 0)
+        out = C.c_void_p()
+        assert lib.cgfd_create(C.byref(ok), C.byref(out)) != -1 or b"4 GiB" not in lib.cgfd_last_error()
+
+
+def test_footprint_counts_the_storage_padded_model():
+    """Model arrays share the wavefield layout: 16-bit storage pads rows to 64 elements, not 32."""
+    lib = cgfd.load()
+    d = cgfd.Desc(400, 400, 200, cgfd.STORE_F16, 1, 0)
+    b = lib.cgfd_footprint_bytes(C.byref(d), 0, 0)
+    px = 512   # 64 (i=0 at 128 bytes) + 400 + 3 -> next multiple of 64 elements
+    cs = px * 406 * 206
+    assert b >= 4 * 9 * cs * 2 + 13 * cs * 4
