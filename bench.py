@@ -1,22 +1,31 @@
 #!/usr/bin/env python3
 """el_iso RK4 time-stepping throughput on one MI355X.
 
-Workload (default): the 400x400x200 curvilinear topography model (3 layers
-with correlated perturbations, double couple, 400 receivers) stepped in FP32
-through the C ABI of gpu-cgfd3d-elastic_amd/libcgfd_el_iso.so.  A "step" is
-one full RK4 step (4 stages) over the whole grid with receivers recorded;
-W warmup steps run first, then exactly K steps are timed.  Inputs are
-resident in HBM before the timed region.
+Headline workload: the 800x800x400 explosion model -- the 100x100x60
+configuration's physics (Cartesian 100 m grid, homogeneous Vp 6000 / Vs
+3464 / rho 2670, flat free surface, 10-layer CFS-PML on the other five
+faces, explosive moment tensor Mxx=Myy=Mzz=1e16 N m 3 km deep, Ricker 2 Hz,
+dt 0.005 s, 20 surface receivers) at the size of the FP32-vs-FP16-store
+comparison -- stepped in FP32 through the C ABI of
+gpu-cgfd3d-elastic_amd/libcgfd_el_iso.so.  A "step" is one full RK4 step (4
+stages) over the whole grid with receivers recorded; W warmup steps run
+first, then exactly K steps are timed.  Inputs are resident in HBM before
+the timed region.
 
-Prints one JSON line with the throughput in Gcell-RK-stages/s
-(= ni*nj*nk*4*K / t), the HBM roofline of the dominant (interior) kernel
-measured with HIP events around its launches, and a CPU baseline: the
-oracle (a C++/OpenMP restatement of the reference loop) timed on a few
-steps of the same grid on the host cores.
+The JSON line also carries: the HBM roofline of the dominant kernel (the
+z-marching sweep, average launch time from HIP events on its stream, 192
+algorithmic bytes per swept point and stage) beside the streaming read/copy
+ceilings measured in the same run; FP16 / BF16 storage and the FP64 path on
+the same model with their relative L2 errors against FP64; the 400x400x200
+topography model (curvilinear metrics, all 12 coefficient streams); 64
+batched shots on the small medium; a served stream over five resident
+models; and the CPU baseline (the oracle, a C++/OpenMP restatement of the
+reference loop) on a bounded sample of the headline model.
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -44,7 +53,23 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    model, n = "", 0
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = model or line.split(":", 1)[1].strip()
+                n += 1
+    except OSError:
+        pass
+    return model or platform.processor(), n
+
+
 def cpu_baseline(m, steps):
+    """The oracle (C++/OpenMP restatement of the reference's unfused per-point
+    loop, FP32) on `steps` full RK4 steps of workload m."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import ctypes as C
 
@@ -59,10 +84,13 @@ def cpu_baseline(m, steps):
         raise RuntimeError("oracle run failed")
     sec = L.orc_last_loop_seconds()
     cells = workloads.cells(m)
+    model, ncpu = host_cpu()
     return dict(value=round(cells * 4 * steps / sec / 1e9, 5), unit="Gcell-RK-stages/s",
                 cores=int(L.orc_max_threads()), kind="port",
-                sample=f"{steps} full RK4 steps of the same {m['ni']}x{m['nj']}x{m['nk']} grid, FP32, "
-                       f"unfused reference stage order, {sec:.1f} s loop")
+                sample=f"{steps} full RK4 steps of the same {m['ni']}x{m['nj']}x{m['nk']} model, FP32, unfused "
+                       f"reference stage order (RHS then axpys), {sec:.1f} s loop",
+                host=f"{model}, {ncpu} logical CPUs visible; OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} "
+                     f"OMP_PLACES={os.environ.get('OMP_PLACES')}")
 
 
 def measured_traffic(workload, storage, split, tile):
@@ -243,143 +271,225 @@ def batch_sample(n, nt_range, seed, nseq, storage):
                 p99_latency_s=round(float(np.percentile(lat, 99)), 3))
 
 
+STORE_NAMES = {"f32": "FP32", "f16": "FP16 storage, FP32 arithmetic", "bf16": "BF16 storage, FP32 arithmetic",
+               "f64": "FP64 (the GPU oracle of the same scheme)"}
+
+
+def scales_for(m, st, src=None):
+    return cgfd.storage_scales(st, m["ni"], m["nj"], m["nk"], m["metric"], m["lam"], m["mu"], m["slw"],
+                               m["src"] if src is None else src, m["dt"])
+
+
+def timed_session(m, storage, warmup, steps, launch_steps, flags=0, tile=""):
+    """Upload m into a single-shot session, run warmup steps, time `steps`
+    steps (8-step periods replayed as captured graphs), then a short pass with
+    HIP events around every sweep launch.  Returns a dict of the measurements."""
+    st = STORE[storage]
+    nt_total = warmup + steps + launch_steps
+    sess = cgfd.Session(m["ni"], m["nj"], m["nk"], storage=st, flags=flags)
+    workloads.upload(sess, m, nt_total=nt_total)
+    if st == cgfd.STORE_F16:
+        sess.set_storage_scale(scales_for(m, st))
+    if tile:
+        sh, zc = (int(v) for v in tile.split(","))
+        sess.set_tile(sh, zc)
+    sess.run(0, warmup, m["dt"])
+    t1 = time.perf_counter()
+    sess.run(warmup, steps, m["dt"])
+    t2 = time.perf_counter()
+    out = dict(wall=t2 - t1, device_ms=sess.last_run_timing()[0], int_avg_ms=0.0, n_int=0, step_ms_timed=0.0)
+    if launch_steps > 0:
+        sess.set_timing(True)
+        sess.run(warmup + steps, launch_steps, m["dt"])
+        run_ms, out["int_avg_ms"], out["n_int"] = sess.last_run_timing()
+        out["step_ms_timed"] = run_ms / launch_steps
+    out["finite"] = bool(np.isfinite(sess.read_seismo(0)).all())
+    out["tile"] = sess.get_tile()
+    out["split"] = sess.get_split()
+    out["const_metric"] = sess.const_metric()
+    sess.close()
+    return out
+
+
+def roofline(m, storage, r, stream):
+    """Dominant kernel: the z-marching sweep.  achieved = 192 B (FP32 unit of
+    the fused RK4 dataflow, interior point and stage) x swept points / the
+    sweep's average launch time; the PML memory traffic and the path's own
+    needs are reported beside it."""
+    if r["n_int"] == 0 or r["int_avg_ms"] <= 0:
+        return None
+    split = r["split"]
+    if split == cgfd.SPLIT_SHELL:
+        cells, pairs = workloads.interior_cells(m), 0
+    else:
+        cells, pairs = workloads.sweep_cells(m), workloads.sweep_pml_pairs(m)
+    unit = ALGO_BYTES[storage]
+    sec = r["int_avg_ms"] * 1e-3
+    ach = unit * cells / sec / 1e9
+    # with the constant-metric sweep 9 of the 12 coefficient words per point and stage are kernel arguments
+    es = {"f32": 4, "f16": 4, "bf16": 4, "f64": 8}[storage]
+    path_unit = unit - (9 * es if r["const_metric"] else 0)
+    tile = r["tile"]
+    roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
+                traffic=None,
+                kernel="k_march (z-marching sweep" + (", inside the PML slabs)" if split == cgfd.SPLIT_SHELL
+                                                      else ", whole planes with the PML slabs)"),
+                bytes_per_point_stage=unit, swept_points=cells, avg_launch_ms=round(r["int_avg_ms"], 5),
+                algorithmic_bytes_per_launch=unit * cells,
+                tile=f"{tile['tx']}x{tile['ty']} threads, {tile['zchunk']} planes",
+                sweep_share_of_step=round(4 * r["int_avg_ms"] / r["step_ms_timed"], 4) if r["step_ms_timed"] else None,
+                const_metric=r["const_metric"],
+                path_bytes_per_point_stage=path_unit,
+                path_GBps=round(path_unit * cells / sec / 1e9, 1),
+                pml_face_pairs=pairs,
+                with_pml_GBps=round((unit * cells + AUX_BYTES[storage] * pairs) / sec / 1e9, 1))
+    if stream:
+        roof["stream_read_GBps"] = round(stream[0], 1)
+        roof["stream_copy_GBps"] = round(stream[1], 1)
+        roof["path_frac_of_stream_copy"] = round(roof["path_GBps"] / stream[1], 4)
+    traffic = measured_traffic(m["name"], storage, split, tile)
+    if traffic:
+        roof["traffic"] = traffic
+        roof["traffic_GBps"] = round(traffic / sec / 1e9, 1)
+    return roof
+
+
+def precision_leg(m, steps):
+    """FP64, FP32, FP16 and BF16 storage from the same start for `steps`
+    steps: throughput of each (steps 8.. timed) and the relative L2 error of
+    the seismograms (9 components, all receivers) and of a stride-8 sample of
+    the final wavefield against the FP64 run."""
+    out, ref = {}, None
+    cells = workloads.cells(m)
+    ni, nj, nk = m["ni"], m["nj"], m["nk"]
+    cnt = ((ni - 4) // 8 + 1, (nj - 4) // 8 + 1, (nk - 4) // 8 + 1)
+    for name in ("f64", "f32", "f16", "bf16"):
+        st = STORE[name]
+        s = cgfd.Session(ni, nj, nk, storage=st)
+        workloads.upload(s, m, nt_total=steps)
+        ex = None
+        if st == cgfd.STORE_F16:
+            ex = scales_for(m, st)
+            s.set_storage_scale(ex)
+        s.run(0, 8, m["dt"])   # autotune and first graph captures outside the clock
+        ta = time.perf_counter()
+        s.run(8, steps - 8, m["dt"])
+        tb = time.perf_counter()
+        seis = s.read_seismo(0)
+        wave = s.fetch_subvolume(0, cgfd.LEVEL_PRE, list(range(9)), (2, 2, 2), cnt, (8, 8, 8), dtype=np.float64)
+        s.close()
+        v = cells * 4 * (steps - 8) / (tb - ta) / 1e9
+        rec = dict(value=round(v, 4), unit="Gcell-RK-stages/s", ms_per_step=round((tb - ta) * 1e3 / (steps - 8), 4),
+                   steps=steps, scales_exp2=ex if ex is not None else [0] * 9,
+                   seismograms_finite=bool(np.isfinite(seis).all()))
+        if name == "f64":
+            ref = (seis, wave)
+        else:
+            rec["rel_l2_seismograms_vs_f64"] = float(f"{np.linalg.norm(seis - ref[0]) / np.linalg.norm(ref[0]):.4g}")
+            rec["rel_l2_wavefield_vs_f64"] = float(f"{np.linalg.norm(wave - ref[1]) / np.linalg.norm(ref[1]):.4g}")
+        out[name] = rec
+        log(f"{name}: {v:.3f} Gcell-RK-stages/s over {steps - 8} steps" +
+            (f", rel L2 vs FP64 seismograms {rec['rel_l2_seismograms_vs_f64']:.3e} wavefield "
+             f"{rec['rel_l2_wavefield_vs_f64']:.3e}" if name != "f64" else ""))
+    for name in ("f16", "bf16", "f64"):
+        out[name]["speedup_vs_f32"] = round(out[name]["value"] / out["f32"]["value"], 3)
+    return out
+
+
+def topo_leg(steps, warmup, stream):
+    """The 400x400x200 topography model (curvilinear metric, all coefficient
+    streams): FP32 with its sweep roofline, FP16 and BF16 storage."""
+    m = workloads.topo(nt=warmup + steps + 8)
+    cells = workloads.cells(m)
+    out = {}
+    for name in ("f32", "f16", "bf16"):
+        r = timed_session(m, name, warmup, steps, 8 if name == "f32" else 0)
+        v = cells * 4 * steps / r["wall"] / 1e9
+        out[name] = dict(value=round(v, 4), unit="Gcell-RK-stages/s", ms_per_step=round(r["wall"] * 1e3 / steps, 4),
+                         steps=steps, seismograms_finite=r["finite"],
+                         split="planes" if r["split"] == cgfd.SPLIT_PLANES else "shell")
+        if name == "f32":
+            out[name]["roofline"] = roofline(m, name, r, stream)
+        log(f"topography {name}: {v:.3f} Gcell-RK-stages/s")
+    for name in ("f16", "bf16"):
+        out[name]["speedup_vs_f32"] = round(out[name]["value"] / out["f32"]["value"], 3)
+    out["workload"] = ("topo: 400x400x200 el_iso RK4, curvilinear topography, 3 perturbed layers, double couple, "
+                       "400 receivers, free surface + 10-layer CFS-PML, dt 0.005")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=16)
-    ap.add_argument("--workload", default="topo", choices=["topo", "small"])
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--workload", default="explosion800", choices=["explosion800", "topo", "small"])
     ap.add_argument("--storage", default="f32", choices=sorted(STORE))
-    ap.add_argument("--cpu-steps", type=int, default=8)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline sample (0: skip)")
     ap.add_argument("--plain", action="store_true", help="per-point kernel everywhere")
     ap.add_argument("--pml-point", action="store_true",
                     help="PML slabs by the per-point kernel (the sweep keeps to the box inside them)")
+    ap.add_argument("--full-metric", action="store_true", help="keep the per-point metric streams")
     ap.add_argument("--tile", default="", metavar="SHAPE,ZCHUNK",
                     help="pin the interior tile instead of autotuning (profiling runs)")
     ap.add_argument("--timed-launch-steps", type=int, default=8,
                     help="steps of the per-launch event timing pass (0: skip)")
-    ap.add_argument("--reduced-steps", type=int, default=200,
-                    help="timed steps of the FP16/BF16-storage runs of the same workload (0: skip)")
+    ap.add_argument("--error-steps", type=int, default=320,
+                    help="steps of the FP64/FP32/FP16/BF16 runs compared against FP64 (0: skip)")
+    ap.add_argument("--topo-steps", type=int, default=48, help="timed steps of the topography legs (0: skip)")
+    ap.add_argument("--stream-mib", type=int, default=2048, help="buffer size of the streaming ceilings (0: skip)")
     ap.add_argument("--serve-shots", type=int, default=40, help="served-stream sample size (0: skip)")
     ap.add_argument("--batch-shots", type=int, default=64, help="batched-shots sample size (0: skip)")
     ap.add_argument("--batch-steps", default="500,2000", help="step range of the batched shots")
     ap.add_argument("--serve-steps", default="40,160", help="step range of the served requests")
     ap.add_argument("--serve-load", type=float, default=4.0, help="arrival rate / sequential plain rate")
-    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
-                    help="only run a device-to-device copy of MIB MiB five times (counter calibration)")
+    ap.add_argument("--only-headline", action="store_true", help="skip every leg after the headline (profiling)")
     args = ap.parse_args()
-    if args.copy_calib:
-        import torch  # plumbing only: a plain copy of known size for HBM counter scaling
-
-        n = args.copy_calib * 1024 * 1024 // 4
-        x = torch.ones(n, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        y.copy_(x)
-        e0.record()
-        for _ in range(5):
-            y.copy_(x)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / 5
-        print(json.dumps({"copy_calib_bytes_each_way": n * 4, "copies": 6, "ms_per_copy": round(ms, 4),
-                          "copy_GBps_read_plus_write": round(2 * n * 4 / ms / 1e6, 1)}), flush=True)
-        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(the decomposition is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.only_headline:
+        args.error_steps = args.topo_steps = args.stream_mib = args.serve_shots = args.batch_shots = 0
+        args.cpu_steps = 0
 
     t0 = time.time()
-    m = workloads.topo() if args.workload == "topo" else workloads.small()
+    nt = args.warmup + args.steps + args.timed_launch_steps
+    if args.workload == "explosion800":
+        m = workloads.small(ni=800, nj=800, nk=400, nt=max(nt, args.error_steps))
+    elif args.workload == "topo":
+        m = workloads.topo(nt=nt)
+    else:
+        m = workloads.small(nt=max(nt, args.error_steps))
     log(f"workload {m['name']} {m['ni']}x{m['nj']}x{m['nk']} prepared in {time.time() - t0:.1f}s; "
         f"dt {m['dt']} (dtmax {m['dtmax']:.5f})")
     if m["dt"] > m["dtmax"]:
         raise SystemExit(f"dt {m['dt']} exceeds the stability limit {m['dtmax']}")
 
-    nt_total = args.warmup + args.steps
-    flags = (cgfd.FLAG_PLAIN if args.plain else 0) | (cgfd.FLAG_PML_POINT if args.pml_point else 0)
-    sess = cgfd.Session(m["ni"], m["nj"], m["nk"], storage=STORE[args.storage], flags=flags)
-    nt_total += args.timed_launch_steps
-    workloads.upload(sess, m, nt_total=nt_total)
-    if args.storage == "f16":
-        # FP16 needs power-of-two scales to keep stresses in range (BF16 has FP32's exponent range)
-        sess.set_storage_scale(cgfd.storage_scales(cgfd.STORE_F16, m["ni"], m["nj"], m["nk"], m["metric"], m["lam"],
-                                                   m["mu"], m["slw"], m["src"], m["dt"]))
-    if args.tile:
-        sh, zc = (int(v) for v in args.tile.split(","))
-        sess.set_tile(sh, zc)
-    log("uploaded; warmup")
-    sess.run(0, args.warmup, m["dt"])
-    log(f"timing {args.steps} steps")
-    # the timed region: K steps, 8-step periods replayed as captured graphs
-    t1 = time.perf_counter()
-    sess.run(args.warmup, args.steps, m["dt"])
-    t2 = time.perf_counter()
-    run_ms = sess.last_run_timing()[0]
-    # then a short pass with HIP events around every interior launch (on the
-    # session stream the kernels run on) for the kernel's average duration
-    int_avg_ms, n_int = 0.0, 0
-    if args.timed_launch_steps > 0:
-        sess.set_timing(True)
-        sess.run(args.warmup + args.steps, args.timed_launch_steps, m["dt"])
-        _, int_avg_ms, n_int = sess.last_run_timing()
-        step_ms_timed = sess.last_run_timing()[0] / args.timed_launch_steps
-    seis = sess.read_seismo(0)
-    finite = bool(np.isfinite(seis).all())
-    tile = sess.get_tile()
-    split = sess.get_split()
-    sess.close()
-    wall = t2 - t1
+    flags = (cgfd.FLAG_PLAIN if args.plain else 0) | (cgfd.FLAG_PML_POINT if args.pml_point else 0) | \
+        (cgfd.FLAG_FULL_METRIC if args.full_metric else 0)
+    log(f"timing {args.steps} steps ({args.warmup} warmup)")
+    r = timed_session(m, args.storage, args.warmup, args.steps, args.timed_launch_steps, flags, args.tile)
     cells = workloads.cells(m)
-    value = cells * 4 * args.steps / wall / 1e9
-    log(f"{value:.3f} Gcell-RK-stages/s wall, device {run_ms:.1f} ms, interior avg {int_avg_ms:.4f} ms x {n_int}")
+    value = cells * 4 * args.steps / r["wall"] / 1e9
+    log(f"{value:.3f} Gcell-RK-stages/s wall, device {r['device_ms']:.1f} ms, sweep avg {r['int_avg_ms']:.4f} ms "
+        f"x {r['n_int']}, tile {r['tile']}, constant metric {r['const_metric']}")
 
-    roof = None
-    if n_int > 0 and int_avg_ms > 0:
-        if split == cgfd.SPLIT_SHELL:
-            icells, pairs = workloads.interior_cells(m), 0
-        else:
-            icells, pairs = workloads.sweep_cells(m), workloads.sweep_pml_pairs(m)
-        algo = ALGO_BYTES[args.storage] * icells + AUX_BYTES[args.storage] * pairs
-        ach = algo / (int_avg_ms * 1e-3) / 1e9
-        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_march (z-marching sweep" + (", inside the PML slabs)" if split == cgfd.SPLIT_SHELL
-                                                          else ", whole planes with the PML slabs)"),
-                    sweep_cells=icells, pml_face_pairs=pairs, algorithmic_bytes_per_launch=algo,
-                    avg_launch_ms=round(int_avg_ms, 5),
-                    tile=f"{tile['tx']}x{tile['ty']} threads, {tile['zchunk']} planes",
-                    sweep_share_of_step=round(4 * int_avg_ms / step_ms_timed, 4))
-        traffic = None if args.plain else measured_traffic(m["name"], args.storage, split, tile)
-        if traffic:
-            roof["traffic"] = traffic
+    stream = None
+    if args.stream_mib > 0:
+        stream = cgfd.hbm_stream(args.stream_mib << 20, 10)
+        log(f"streaming ceilings: read {stream[0]:.0f} GB/s, copy {stream[1]:.0f} GB/s (read + write)")
+    roof = roofline(m, args.storage, r, stream) if not args.plain else None
 
-    reduced = None
-    if args.reduced_steps > 0 and args.storage == "f32":
-        reduced = {}
-        ex = cgfd.storage_scales(cgfd.STORE_F16, m["ni"], m["nj"], m["nk"], m["metric"], m["lam"], m["mu"], m["slw"],
-                                 m["src"], m["dt"])
-        # FP16 / BF16 storage, and the FP64 path (the GPU oracle of the same scheme) on fewer steps
-        for name, st in (("f16", cgfd.STORE_F16), ("bf16", cgfd.STORE_BF16), ("f64", cgfd.STORE_F64)):
-            nst = args.reduced_steps if st != cgfd.STORE_F64 else max(8, args.reduced_steps // 4)
-            rs = cgfd.Session(m["ni"], m["nj"], m["nk"], storage=st)
-            workloads.upload(rs, m, nt_total=args.warmup + nst)
-            if st == cgfd.STORE_F16:
-                rs.set_storage_scale(ex)
-            rs.run(0, args.warmup, m["dt"])
-            ta = time.perf_counter()
-            rs.run(args.warmup, nst, m["dt"])
-            tb = time.perf_counter()
-            fin = bool(np.isfinite(rs.read_seismo(0)).all())
-            rs.close()
-            v = cells * 4 * nst / (tb - ta) / 1e9
-            reduced[name] = dict(value=round(v, 4), unit="Gcell-RK-stages/s",
-                                 ms_per_step=round((tb - ta) * 1e3 / nst, 4),
-                                 speedup_vs_f32=round(v / value, 3), steps=nst,
-                                 scales_exp2=ex if st == cgfd.STORE_F16 else [0] * 9, seismograms_finite=fin)
-            log(f"{name} storage: {v:.3f} Gcell-RK-stages/s ({v / value:.2f}x FP32)")
+    precision = None
+    if args.error_steps > 0 and args.storage == "f32":
+        log(f"precision leg: FP64 / FP32 / FP16 / BF16, {args.error_steps} steps each")
+        precision = precision_leg(m, args.error_steps)
+
+    topo = None
+    if args.topo_steps > 0 and args.workload != "topo":
+        log(f"topography legs: {args.topo_steps} steps")
+        topo = topo_leg(args.topo_steps, args.warmup, stream)
 
     serving = None
     if args.serve_shots > 0:
@@ -401,24 +511,29 @@ def main():
                 f"{batched[st]['sequential_plain_shots_per_hour']} ({batched[st]['speedup']}x)")
 
     cpu = None
-    if not args.no_cpu:
+    if args.cpu_steps > 0:
         log(f"cpu baseline: {args.cpu_steps} steps")
         cpu = cpu_baseline(m, args.cpu_steps)
 
+    desc = {"explosion800": "explosion model at 800x800x400: Cartesian 100 m, homogeneous Vp 6000 / Vs 3464 / "
+                            "rho 2670, explosive source 3 km deep, Ricker 2 Hz, 20 surface receivers",
+            "small": "explosion model at 100x100x60: Cartesian 100 m, homogeneous, explosive source 3 km deep, "
+                     "Ricker 2 Hz, 20 surface receivers",
+            "topo": "curvilinear topography, 3 perturbed layers, double couple, 400 receivers"}[args.workload]
     line = {
         "metric": METRIC, "value": round(value, 4), "unit": "Gcell-RK-stages/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r["wall"] * 1e3 / args.steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.storage,
         "data": "synthetic",
-        "config": {"workload": f"{m['name']}: {m['ni']}x{m['nj']}x{m['nk']} el_iso RK4, "
-                               f"{'curvilinear topography, 3 perturbed layers, double couple, 400 receivers' if m['name'] == 'topo' else 'Cartesian homogeneous, explosion, 20 receivers'}, "
-                               f"free surface + 10-layer CFS-PML, dt {m['dt']}",
+        "config": {"workload": f"{m['name']}: {m['ni']}x{m['nj']}x{m['nk']} el_iso RK4, {desc}, free surface + "
+                               f"10-layer CFS-PML, dt {m['dt']}",
                    "cells": cells, "storage": args.storage,
-                   "kernel_path": "plain" if args.plain else ("zmarch planes" if split == cgfd.SPLIT_PLANES
-                                                                 else "zmarch + point shell"),
-                   "seismograms_finite": finite, "device_ms": round(run_ms, 2)},
+                   "kernel_path": "plain" if args.plain else ("zmarch planes" if r["split"] == cgfd.SPLIT_PLANES
+                                                              else "zmarch + point shell"),
+                   "seismograms_finite": r["finite"], "device_ms": round(r["device_ms"], 2)},
         "roofline": roof,
-        "reduced_storage": reduced,
+        "reduced_storage": precision,
+        "topography": topo,
         "serving": serving,
         "batched_shots": batched,
         "cpu_baseline": cpu,

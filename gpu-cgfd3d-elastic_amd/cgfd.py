@@ -30,7 +30,7 @@ EXPORTS = [
     "cgfd_upload_receivers", "cgfd_set_level", "cgfd_reset_shot", "cgfd_run",
     "cgfd_el_iso_onestage", "cgfd_read_seismo", "cgfd_fetch_level",
     "cgfd_fetch_pml_aux", "cgfd_set_timing", "cgfd_last_run_timing",
-    "cgfd_set_tile", "cgfd_get_tile", "cgfd_get_split", "cgfd_set_graph", "cgfd_set_shot",
+    "cgfd_set_tile", "cgfd_get_tile", "cgfd_get_split", "cgfd_get_const_metric", "cgfd_set_graph", "cgfd_set_shot",
     "cgfd_run_async", "cgfd_poll", "cgfd_wait", "cgfd_reserve_receivers",
     "cgfd_grid_cartesian", "cgfd_grid_topography", "cgfd_metric_calc", "cgfd_dt_estimate",
     "cgfd_media_layered", "cgfd_media_homogeneous", "cgfd_rho_to_slow", "cgfd_pml_len_dh",
@@ -41,7 +41,7 @@ EXPORTS = [
     "cgfd_server_create", "cgfd_server_destroy", "cgfd_server_add_model", "cgfd_server_submit",
     "cgfd_server_wait", "cgfd_server_result", "cgfd_server_get_stats", "cgfd_fetch_subvolume",
     "cgfd_sac_write", "cgfd_station_sac", "cgfd_snap_open", "cgfd_snap_append", "cgfd_snap_close",
-    "cgfd_set_peak_motion", "cgfd_read_peak_motion", "cgfd_peak_motion_write",
+    "cgfd_set_peak_motion", "cgfd_read_peak_motion", "cgfd_peak_motion_write", "cgfd_hbm_stream",
 ]
 
 
@@ -163,6 +163,7 @@ def load(path=LIB_PATH):
         "cgfd_set_tile": ([vp, i, i], i),
         "cgfd_get_tile": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)], i),
         "cgfd_get_split": ([vp, C.POINTER(i)], i),
+        "cgfd_get_const_metric": ([vp, C.POINTER(i)], i),
         "cgfd_last_run_timing": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)], i),
     }
     d = C.c_double
@@ -204,6 +205,7 @@ def load(path=LIB_PATH):
         "cgfd_server_wait": ([vp, ll, d], i),
         "cgfd_server_result": ([vp, ll, i, vp, C.POINTER(d), C.POINTER(d)], i),
         "cgfd_server_get_stats": ([vp, C.POINTER(ServerStats)], i),
+        "cgfd_hbm_stream": ([ll, i, C.POINTER(d), C.POINTER(d)], i),
     })
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -345,6 +347,12 @@ class Session:
         v = C.c_int()
         _check(_lib.cgfd_get_split(self.h, C.byref(v)))
         return v.value
+
+    def const_metric(self):
+        """True when the sweep runs with the metric as nine constants (Cartesian grid)."""
+        v = C.c_int()
+        _check(_lib.cgfd_get_const_metric(self.h, C.byref(v)))
+        return bool(v.value)
 
     def get_tile(self):
         a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
@@ -696,6 +704,14 @@ class Server:
         st = ServerStats()
         _check(_lib.cgfd_server_get_stats(self.h, C.byref(st)))
         return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+
+def hbm_stream(nbytes=2 << 30, reps=10):
+    """(read GB/s, copy GB/s counting read + write bytes) of the streaming ceiling kernels."""
+    load()
+    r, c = C.c_double(), C.c_double()
+    _check(_lib.cgfd_hbm_stream(int(nbytes), int(reps), C.byref(r), C.byref(c)))
+    return r.value, c.value
 
 
 def footprint_bytes(ni, nj, nk, storage=STORE_F32, nshots=1, nrecv=0, nt_total=0):

@@ -156,7 +156,7 @@ struct cgfd_session {
   double cmet[9] = {0};
   int free_top = 0;
   FaceHost face[6];
-  int slot[4] = {0, 1, 2, 3};  // physical level slot of PRE, TMPA, TMPB, END
+  int slot[4] = {0, 1, 2, 3};  // physical level slot of W^n and of the stage values U1, U2, U3
   float sc[9], isc[9];
   std::vector<SrcHost> src;
   std::vector<RecvHost> rec;
@@ -189,7 +189,7 @@ struct cgfd_session {
   std::vector<hipEvent_t> ev;
   hipEvent_t ev_run0 = nullptr, ev_run1 = nullptr;
   size_t ev_used = 0;
-  // 8-step periods captured as graphs, one per orientation of the PRE/END slots;
+  // 8-step periods captured as graphs, one per orientation of the W/U1 slots;
   // any change of kernel arguments bumps the epoch and retires them
   bool use_graph = true;
   unsigned long epoch = 1;
@@ -524,8 +524,11 @@ template <typename TC> int build_receivers(cgfd_session *s) {
 }
 
 // ---- stage argument assembly ----
+// Level slots of a stage: input, W^n, output, U1 (written with W^{n+1} by
+// S3) and U2 (read by S3); rhs_mode sends the output to the RHS level.
 template <typename TS, typename TC>
-StageArgs<TS, TC> stage_args(cgfd_session *s, int slot_in, int slot_pre, int slot_out, int slot_end, bool rhs_mode) {
+StageArgs<TS, TC> stage_args(cgfd_session *s, int slot_in, int slot_pre, int slot_out, int slot_end, int slot_u2,
+                             bool rhs_mode) {
   StageArgs<TS, TC> a{};
   a.L = s->L;
   a.met = (const TC *)s->met;
@@ -556,13 +559,17 @@ StageArgs<TS, TC> stage_args(cgfd_session *s, int slot_in, int slot_pre, int slo
     A.a_pre = lv(slot_pre);
     A.a_out = rhs_mode ? lv(4) : lv(slot_out);
     A.a_end = lv(slot_end);
+    A.a_u2 = lv(slot_u2);
   }
   TS *w = (TS *)s->wav;
   a.w_in = w + (long)slot_in * s->level_elems;
   a.w_pre = w + (long)slot_pre * s->level_elems;
   a.w_out = rhs_mode ? (TS *)s->rhs : w + (long)slot_out * s->level_elems;
   a.w_end = w + (long)slot_end * s->level_elems;
+  a.w_u2 = w + (long)slot_u2 * s->level_elems;
   a.shot_stride = rhs_mode ? 0 : s->shot_elems;
+  // S3 weights of the stage values: w_i = b_{i-1} / a_{i-1}
+  for (int q = 0; q < 3; q++) a.rk_w[q] = (TC)(kRkB[q] / kRkA[q]);
   for (int c = 0; c < kNC; c++) { a.sc[c] = s->sc[c]; a.isc[c] = s->isc[c]; }
   a.shot_active = s->shot_active;
   return a;
@@ -663,10 +670,10 @@ int launch_stage(cgfd_session *s, const Shell &sh, StageArgs<TS, TC> &a, int dx,
 
 // Tile autotuner: per (shape, z chunk), time the interior sweeps of one full
 // step's stage sequence (S0, S12, S12, S3 with the operator directions of
-// pair 0) and keep the fastest; the stages differ in traffic (39 to 57 words
+// pair 0) and keep the fastest; the stages differ in traffic (30 to 57 words
 // per cell), so tuning on one mode alone can pick a tile that loses on the
 // others.  The RK coefficients are zero, and every level the sweeps write
-// (TMPA, TMPB, END) is rewritten by the next step before it is read, so
+// (U1, U2, U3) is rewritten by the next step before it is read, so
 // tuning between steps leaves the state intact.  Results are cached per
 // (grid, storage, shots, work split) for the life of the process.
 struct TileKey {
@@ -709,12 +716,12 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
     remember({s->tile_shape, s->tile_zchunk, s->split});
     return CGFD_OK;
   }
-  const int PRE = s->slot[0], TA = s->slot[1], TB = s->slot[2], END = s->slot[3];
-  const int sin[4] = {PRE, TA, TB, TA}, sout[4] = {TA, TB, TA, TB};
+  const int PRE = s->slot[0], U1 = s->slot[1], U2 = s->slot[2], U3 = s->slot[3];
+  const int sin[4] = {PRE, U1, U2, U3}, sout[4] = {U1, U2, U3, U1};
   const int smode[4] = {MODE_S0, MODE_S12, MODE_S12, MODE_S3};
   StageArgs<TS, TC> sa[4];
   for (int q = 0; q < 4; q++) {
-    sa[q] = stage_args<TS, TC>(s, sin[q], PRE, sout[q], END, false);
+    sa[q] = stage_args<TS, TC>(s, sin[q], PRE, sout[q], U1, U2, false);
     sa[q].mode = smode[q];
     sa[q].ca = TC(0);
     sa[q].cb = TC(0);
@@ -764,10 +771,6 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
     float best = 1e30f;
     int ts = 0, tz = 24;
     for (int shape = 0; shape < interior_num_shapes(); shape++) {
-      // split-role and two-column tiles have no PML form (they would rerun
-      // 64 x 8); two-column tiles exist for 16-bit storage only
-      if (sh.inner.pml && shape >= 4) continue;
-      if (shape >= 7 && sizeof(TS) != 2) continue;
       for (int zi = 0; zi < kNZ; zi++) {
         const int zc = zcands[zi];
         if (zi > 0 && zc == zcands[zi - 1]) continue;
@@ -806,7 +809,7 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
 
 // Enqueue steps it0 .. it0+nsteps-1 of the RK4 loop on the session stream:
 // receivers, four stages (interior sweep, PML slabs / free-surface rows,
-// source) and the clock advance per step, then the PRE <-> END swap.
+// source) and the clock advance per step, then the W <-> U1 swap.
 template <typename TS, typename TC>
 int enqueue_steps(cgfd_session *s, const Shell &sh, int it0, int nsteps, TC dt, bool timed) {
   for (int n = 0; n < nsteps; n++) {
@@ -824,17 +827,18 @@ int enqueue_steps(cgfd_session *s, const Shell &sh, int it0, int nsteps, TC dt, 
       if (e != hipSuccess) return fail(CGFD_E_HIP, "receiver kernel: %s", hipGetErrorString(e));
     }
     const int ipair = it % 8;
-    const int PRE = s->slot[0], TA = s->slot[1], TB = s->slot[2], END = s->slot[3];
+    // stage values U1..U3; S3 writes W^{n+1} over U1, which then becomes W
+    const int PRE = s->slot[0], U1 = s->slot[1], U2 = s->slot[2], U3 = s->slot[3];
     for (int istage = 0; istage < 4; istage++) {
       const int dx = (kFlags[ipair][0] + istage) % 2;
       const int dy = (kFlags[ipair][1] + istage) % 2;
       const int dz = (kFlags[ipair][2] + istage) % 2;
       int in, out, mode;
-      if (istage == 0) { in = PRE; out = TA; mode = MODE_S0; }
-      else if (istage == 1) { in = TA; out = TB; mode = MODE_S12; }
-      else if (istage == 2) { in = TB; out = TA; mode = MODE_S12; }
-      else { in = TA; out = TB; mode = MODE_S3; }
-      StageArgs<TS, TC> a = stage_args<TS, TC>(s, in, PRE, out, END, false);
+      if (istage == 0) { in = PRE; out = U1; mode = MODE_S0; }
+      else if (istage == 1) { in = U1; out = U2; mode = MODE_S12; }
+      else if (istage == 2) { in = U2; out = U3; mode = MODE_S12; }
+      else { in = U3; out = U1; mode = MODE_S3; }
+      StageArgs<TS, TC> a = stage_args<TS, TC>(s, in, PRE, out, U1, U2, false);
       a.mode = mode;
       // coef = rk_a[istage]*dt, both float in the reference (drv_rk_curv_col.cu:341-342)
       a.ca = (istage < 3) ? (TC)kRkA[istage] * dt : TC(0);
@@ -864,20 +868,20 @@ int enqueue_steps(cgfd_session *s, const Shell &sh, int it0, int nsteps, TC dt, 
       }
     }
     if (s->peak && s->free_top) {
-      // W^{n+1} sits in END before the swap (drv_rk_curv_col.cu:519-527)
-      const TS *wn = (const TS *)wav_level(s, 0, END), *wo = (const TS *)wav_level(s, 0, PRE);
+      // W^{n+1} sits in U1's slot before the swap (drv_rk_curv_col.cu:519-527)
+      const TS *wn = (const TS *)wav_level(s, 0, U1), *wo = (const TS *)wav_level(s, 0, PRE);
       hipError_t e = launch_peak<TS, TC>(wn, wo, s->shot_elems, s->L, (TC *)s->peak, dt, s->sc, s->shot_active,
                                          s->d.nshots, s->st);
       if (e != hipSuccess) return fail(CGFD_E_HIP, "peak motion kernel: %s", hipGetErrorString(e));
     }
     hipError_t e = launch_clock_advance(s->clock, s->st);
     if (e != hipSuccess) return fail(CGFD_E_HIP, "clock kernel: %s", hipGetErrorString(e));
-    std::swap(s->slot[0], s->slot[3]);  // pre <-> end (drv_rk_curv_col.cu:529-541)
+    std::swap(s->slot[0], s->slot[1]);  // W^{n+1} becomes W (drv_rk_curv_col.cu:529-541)
   }
   return CGFD_OK;
 }
 
-// The graph of one 8-step period (it0 % 8 == 0) for the current PRE/END
+// The graph of one 8-step period (it0 % 8 == 0) for the current W/U1
 // orientation; captured on first use.  Eight swaps bring the slots back, and
 // the step number lives in device memory, so one capture serves every period.
 template <typename TS, typename TC>
@@ -969,7 +973,7 @@ template <typename TS, typename TC> int onestage_t(cgfd_session *s, int shot, in
   const int dy = (kFlags[ipair][1] + istage) % 2;
   const int dz = (kFlags[ipair][2] + istage) % 2;
   const int slot_in = s->slot[in_level];
-  StageArgs<TS, TC> a = stage_args<TS, TC>(s, slot_in, s->slot[0], slot_in, s->slot[3], true);
+  StageArgs<TS, TC> a = stage_args<TS, TC>(s, slot_in, s->slot[0], slot_in, s->slot[1], s->slot[2], true);
   // select the shot by offsetting the bases (RHS level holds a single shot)
   const long so = (long)shot * s->shot_elems;
   a.w_in += so; a.w_pre += so; a.w_end += so;
@@ -1470,6 +1474,12 @@ int cgfd_set_tile(cgfd_session *s, int shape, int zchunk) {
   s->tile_fixed = true;
   s->tile_shape = shape;
   s->tile_zchunk = zchunk;
+  return CGFD_OK;
+}
+
+int cgfd_get_const_metric(cgfd_session *s, int *const_metric) {
+  if (!s || !const_metric) return fail(CGFD_E_ARG, "null argument");
+  *const_metric = s->met_const ? 1 : 0;
   return CGFD_OK;
 }
 

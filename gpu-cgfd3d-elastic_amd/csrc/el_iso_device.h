@@ -14,6 +14,14 @@ constexpr int kNG = 3;    // ghost points per side (src/forward/fd_t.cu:64-66)
 constexpr int kNC = 9;    // wavefield components
 enum Cmp { VX = 0, VY, VZ, TXX, TYY, TZZ, TYZ, TXZ, TXY };
 enum Met { JAC = 0, XIX, XIY, XIZ, ETX, ETY, ETZ, ZTX, ZTY, ZTZ, NMET };
+// RK modes of a stage kernel.  The classical RK4 step is evaluated from the
+// stored stage values U1..U3 (U_{i+1} = W^n + a_i dt K_i):
+//   S0:  U1 = W + a0 dt K0                      (input W^n)
+//   S12: U2 = W + a1 dt K1, U3 = W + a2 dt K2   (input U1 / U2, W^n read at the point)
+//   S3:  W^{n+1} = W + w1 (U1 - W) + w2 (U2 - W) + w3 (U3 - W) + b3 dt K3
+//        with w_i = b_{i-1} / a_{i-1} = 1/3, 2/3, 1/3 (input U3; written over U1)
+// which equals W + dt (b0 K0 + b1 K1 + b2 K2 + b3 K3).  MODE_RHS writes the
+// right-hand side alone (the per-stage operator).
 enum Mode { MODE_RHS = 0, MODE_S0 = 1, MODE_S12 = 2, MODE_S3 = 3 };
 
 // ---- storage traits ---------------------------------------------------------
@@ -76,6 +84,15 @@ template <typename T> __device__ __forceinline__ T &at_b(T *base, unsigned byte_
   return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off);
 }
 
+// ---- the S3 combination (see Mode), one evaluation order for every kernel ----
+// partial = w1 (U1 - W) + w2 (U2 - W); W^{n+1} = W + ((partial + w3 (U3 - W)) + b3 dt K3)
+template <typename TC> __device__ __forceinline__ TC rk_partial(const TC *w, TC y0, TC u1, TC u2) {
+  return w[0] * (u1 - y0) + w[1] * (u2 - y0);
+}
+template <typename TC> __device__ __forceinline__ TC rk_final(TC y0, TC partial, TC w3, TC u3, TC cbh) {
+  return y0 + ((partial + w3 * (u3 - y0)) + cbh);
+}
+
 // ---- operator constants (src/forward/fd_t.cu:76-148) ----------------------
 // DIR 0 = backward {-3,-2,-1,0,1}, DIR 1 = forward {-1,0,1,2,3}.
 template <typename TC, int DIR> struct Drp;
@@ -114,7 +131,9 @@ template <typename TS, typename TM> struct FaceArgs {
   long fsize;                   // points in the box
   long shot_stride;             // elements between shots of this face's buffer
   const TM *A, *B, *D;          // nlay+1 profile values
-  TS *a_in, *a_pre, *a_out, *a_end;  // shot-0 level bases, component stride fsize
+  // shot-0 level bases, component stride fsize: stage input, W^n, output,
+  // U1 (S3 reads it and writes W^{n+1} over it) and U2 (read by S3)
+  TS *a_in, *a_pre, *a_out, *a_end, *a_u2;
 };
 
 // boxes one per-point launch may cover (the boundary shell splits into at most 26)
@@ -129,10 +148,13 @@ template <typename TS, typename TM> struct StageArgs {
   int met_const;   // 1: every metric component is one constant (Cartesian grid)
   TM cmet[9];      // xi_x..zeta_z when met_const
   FaceArgs<TS, TM> f[6];
-  TS *w_in, *w_pre, *w_out, *w_end;  // shot-0 level bases
+  // shot-0 level bases: stage input, W^n, output, U1 (S3: read, then
+  // overwritten by W^{n+1}) and U2 (S3)
+  TS *w_in, *w_pre, *w_out, *w_end, *w_u2;
   long shot_stride;
   int mode;
-  TM ca, cb;
+  TM ca, cb;      // a_i dt and b_i dt of the stage
+  TM rk_w[3];     // w1, w2, w3 of the S3 combination
   float sc[kNC], isc[kNC];  // storage scale (value = stored*sc) and its inverse
   int nbox;
   int box[kMaxBox][6];      // i1,i2,j1,j2,k1,k2 inclusive, physical
@@ -157,7 +179,7 @@ template <typename TS, typename TM> struct SrcArgs {
   long ntab;
   const TM *met;            // for jac
   const TM *med;            // for 1/rho
-  TS *w_out, *w_end;
+  TS *w_out, *w_end;   // stage output (S0..S2) / W^{n+1} (S3) of each shot
   long shot_stride;
   int mode, istage;
   TM ca, cb;

@@ -160,7 +160,9 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
   // after its load (which makes the wave wait there for every load in
   // flight, the next plane's prefetch included).
   constexpr int mode = MODE;
-  constexpr bool need_end = mode == MODE_S12 || mode == MODE_S3, need_pre = mode == MODE_S12;
+  // W^n at the point: S12 and S3; U1 and U2 at the point: S3 (folded into
+  // one partial sum per component as soon as they arrive)
+  constexpr bool need_pre = mode == MODE_S12 || mode == MODE_S3, need_u = mode == MODE_S3;
   // The PML form fetches the RK levels for the current plane (not one plane
   // ahead): the 18 registers it saves are what its face update needs.
   constexpr bool LATE_RK = PML;
@@ -176,9 +178,20 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
     if constexpr (St<TS>::scaled) St<TS>::st(p, v);
     else St<TS>::st_nt(p, v);
   };
-  TC n_lead[kNC], n_hv[NPER], n_met[12], n_pre[kNC], n_end[kNC];
+  TC n_lead[kNC], n_hv[NPER], n_met[12], n_pre[kNC], n_u1[kNC], n_u2[kNC];
 #pragma unroll
-  for (int c = 0; c < kNC; c++) { n_pre[c] = TC(0); n_end[c] = TC(0); }
+  for (int c = 0; c < kNC; c++) { n_pre[c] = TC(0); n_u1[c] = TC(0); n_u2[c] = TC(0); }
+  // this plane's RK levels at the point
+  auto load_rk = [&](unsigned qb, TC *pre, TC *u1, TC *u2) {
+#pragma unroll
+    for (int c = 0; c < kNC; c++) {
+      pre[c] = need_pre ? (TC)ld_rk(&at_b(a.w_pre + so + c * cs, qb)) * sc(c) : TC(0);
+      if constexpr (need_u) {
+        u1[c] = (TC)ld_rk(&at_b(a.w_end + so + c * cs, qb)) * sc(c);
+        u2[c] = (TC)ld_rk(&at_b(a.w_u2 + so + c * cs, qb)) * sc(c);
+      }
+    }
+  };
   auto issue = [&](int kk) {
     const unsigned pk = (unsigned)(kk + kNG) * upz;
     const unsigned o = pk + col;
@@ -195,13 +208,7 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
 #pragma unroll
     for (int m = 0; m < 3; m++) n_met[9 + m] = (at_b(a.med + m * cs, ob));
     const unsigned qb = o * (unsigned)sizeof(TS);
-    if constexpr (!LATE_RK && need_end) {
-#pragma unroll
-      for (int c = 0; c < kNC; c++) {
-        n_end[c] = (TC)ld_rk(&at_b(a.w_end + so + c * cs, qb)) * sc(c);
-        n_pre[c] = need_pre ? (TC)ld_rk(&at_b(a.w_pre + so + c * cs, qb)) * sc(c) : TC(0);
-      }
-    }
+    if constexpr (!LATE_RK && need_pre) load_rk(qb, n_pre, n_u1, n_u2);
   };
   issue(kbeg);
 
@@ -209,27 +216,24 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
     const unsigned pk = (unsigned)(k + kNG) * upz;
     const unsigned o = pk + col;
     const unsigned qb = o * (unsigned)sizeof(TS);
-    TC hv[NPER], mt[12], pre_v[kNC], end_v[kNC];
+    TC hv[NPER], mt[12], pre_v[kNC], part_v[kNC];
 #pragma unroll
     for (int c = 0; c < kNC; c++) {
 #pragma unroll
       for (int s = 0; s < 4; s++) q[c][s] = q[c][s + 1];
       q[c][4] = n_lead[c];
       pre_v[c] = n_pre[c];
-      end_v[c] = n_end[c];
+      part_v[c] = need_u ? rk_partial(a.rk_w, n_pre[c], n_u1[c], n_u2[c]) : TC(0);
     }
 #pragma unroll
     for (int p = 0; p < NPER; p++) hv[p] = n_hv[p];
 #pragma unroll
     for (int m = 0; m < 12; m++) mt[m] = n_met[m];
-    if constexpr (LATE_RK && need_end) {
+    TC u1_v[kNC], u2_v[kNC];
+    if constexpr (LATE_RK && need_pre) {
       // this plane's RK levels, issued before the next plane's prefetch
       // (memory counters complete in order)
-#pragma unroll
-      for (int c = 0; c < kNC; c++) {
-        end_v[c] = (TC)ld_rk(&at_b(a.w_end + so + c * cs, qb)) * sc(c);
-        pre_v[c] = need_pre ? (TC)ld_rk(&at_b(a.w_pre + so + c * cs, qb)) * sc(c) : TC(0);
-      }
+      load_rk(qb, pre_v, u1_v, u2_v);
     }
     if (k < kend) issue(k + 1);
     const TC xix = mt[0], xiy = mt[1], xiz = mt[2], etx = mt[3], ety = mt[4], etz = mt[5];
@@ -340,13 +344,11 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
         const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
         if constexpr (mode == MODE_S12) {
           st_w(&at_b(wout, qb), (pre_v[c] + a.ca * h[c]) * is);
-          st_w(&at_b(wend, qb), (end_v[c] + a.cb * h[c]) * is);
         } else if constexpr (mode == MODE_S0) {
-          const TC v = q[c][zc0];
-          st_w(&at_b(wout, qb), (v + a.ca * h[c]) * is);
-          st_w(&at_b(wend, qb), (v + a.cb * h[c]) * is);
+          st_w(&at_b(wout, qb), (q[c][zc0] + a.ca * h[c]) * is);
         } else if constexpr (mode == MODE_S3) {
-          st_w(&at_b(wend, qb), (end_v[c] + a.cb * h[c]) * is);
+          const TC part = LATE_RK ? rk_partial(a.rk_w, pre_v[c], u1_v[c], u2_v[c]) : part_v[c];
+          st_w(&at_b(wend, qb), rk_final(pre_v[c], part, a.rk_w[2], q[c][zc0], a.cb * h[c]) * is);
         } else {
           st_w(&at_b(wout, qb), h[c] * is);
         }
@@ -355,288 +357,6 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
     // no second barrier: the next plane writes the other LDS buffer, and the
     // one after that is only written once every wave has passed the next barrier
   }
-}
-
-// ---- split-role variant ------------------------------------------------------------
-// A workgroup of 2 x TX x TY threads sweeps one TX x TY tile.  The first half
-// differentiates the six stress components and updates the velocity, the
-// second differentiates the three velocity components and updates the stress.
-// The halves share only the LDS array and the barrier; each carries the
-// register queues, prefetch buffers and coefficients of its own role, which
-// about halves the registers per thread and doubles the waves a CU keeps in
-// flight on this latency-bound sweep.  The arithmetic per point is that of
-// k_march (same derivative sums, same order).
-enum SplitRole { ROLE_V = 0, ROLE_T = 1 };
-
-template <int ROLE, int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, bool CART>
-__device__ __forceinline__ void split_sweep(const StageArgs<TS, TC> &a, TC *ldsf, int gt, int i0, int j0, int bi1,
-                                            int bj1, int kbeg, int kend, int shot) {
-  constexpr int NT = TX * TY;
-  constexpr int NCG = ROLE == ROLE_V ? 6 : 3;   // components this role differentiates
-  constexpr int COFF = ROLE == ROLE_V ? 3 : 0;  // the first of them
-  constexpr int NOUT = ROLE == ROLE_V ? 3 : 6;  // components this role updates
-  constexpr int OOFF = ROLE == ROLE_V ? 0 : 3;
-  constexpr int NMD = ROLE == ROLE_V ? 1 : 2;   // media values: 1/rho | lambda, mu
-  constexpr int HXL = (DX == 0) ? 3 : 1;
-  constexpr int HYL = (DY == 0) ? 3 : 1;
-  constexpr int LW = TX + 4, LH = TY + 4;
-  constexpr int zlo = (DZ == 0) ? -3 : -1;
-  constexpr int zc0 = -zlo;
-  constexpr int NXH = 4 * TY, NYH = 4 * TX, NH = NXH + NYH;
-  constexpr int HPER = (NH + NT - 1) / NT;  // halo cells per thread and component
-  constexpr int LBUF = kNC * LH * LW;
-  const Layout &L = a.L;
-  const long cs = L.cs;
-  const int lx = gt % TX, ly = gt / TX;
-  const int i = i0 + lx, j = j0 + ly;
-  const bool owner = (i <= bi1) && (j <= bj1);
-  const int ic = i <= L.ni + 2 ? i : L.ni + 2, jc = j <= L.nj + 2 ? j : L.nj + 2;
-  const long so = (long)shot * a.shot_stride;
-  const unsigned upz = (unsigned)L.pz, upx = (unsigned)L.px;
-  const unsigned col = (unsigned)(jc + kNG) * upx + (unsigned)(ic + L.xa);
-  auto sc = [&](int c) -> TC { return St<TS>::scaled ? (TC)a.sc[c] : TC(1); };
-  auto gld = [&](const TS *base, int c, unsigned off) -> TC {
-    return (TC)St<TS>::ld(&at_b(base + so + c * cs, off * (unsigned)sizeof(TS))) * sc(c);
-  };
-
-  // the same halo cells for every component: one offset and one LDS slot per cell
-  int h_lds[HPER];
-  unsigned h_off[HPER];
-#pragma unroll
-  for (int p = 0; p < HPER; p++) {
-    const int e = gt + p * NT;
-    const int r = e < NH ? e : NH - 1;
-    int hi, hj, li, lj;
-    if (r < NXH) {
-      const int h = r % 4, row = r / 4;
-      li = (h < HXL) ? h : TX + h;
-      hi = i0 - HXL + li;
-      lj = row + HYL;
-      hj = j0 + row;
-    } else {
-      const int rr = r - NXH;
-      const int h = rr / TX, cc = rr % TX;
-      lj = (h < HYL) ? h : TY + h;
-      hj = j0 - HYL + lj;
-      li = cc + HXL;
-      hi = i0 + cc;
-    }
-    hi = hi <= L.ni + 2 ? hi : L.ni + 2;
-    hj = hj <= L.nj + 2 ? hj : L.nj + 2;
-    h_lds[p] = e < NH ? lj * LW + li : -1;
-    h_off[p] = (unsigned)(hj + kNG) * upx + (unsigned)(hi + L.xa);
-  }
-
-  TC q[NCG][5];
-#pragma unroll
-  for (int c = 0; c < NCG; c++)
-#pragma unroll
-    for (int s = 0; s < 4; s++) q[c][s + 1] = gld(a.w_in, COFF + c, (unsigned)(kbeg + zlo + s + kNG) * upz + col);
-
-  const int mode = a.mode;
-  // S0 reads the base of its outputs from PRE (= the stage input)
-  const bool need_end = mode == MODE_S12 || mode == MODE_S3;
-  const bool need_pre = mode == MODE_S12 || mode == MODE_S0;
-  // wavefield and coefficients of the next plane are fetched one plane ahead;
-  // the RK levels, needed only in the epilogue, are fetched for the current
-  // plane before its barrier
-  TC n_lead[NCG], n_hv[NCG][HPER], n_met[9 + NMD];
-  auto issue = [&](int kk) {
-    const unsigned pk = (unsigned)(kk + kNG) * upz;
-    const unsigned o = pk + col;
-#pragma unroll
-    for (int c = 0; c < NCG; c++) n_lead[c] = gld(a.w_in, COFF + c, (unsigned)(kk + zlo + 4 + kNG) * upz + col);
-#pragma unroll
-    for (int c = 0; c < NCG; c++)
-#pragma unroll
-      for (int p = 0; p < HPER; p++) n_hv[c][p] = gld(a.w_in, COFF + c, pk + h_off[p]);
-    const unsigned ob = o * (unsigned)sizeof(TC);
-#pragma unroll
-    for (int m = 0; m < 9; m++) n_met[m] = CART ? a.cmet[m] : (at_b(a.met + (XIX + m) * cs, ob));
-    if (ROLE == ROLE_V) {
-      n_met[9] = (at_b(a.med + 2 * cs, ob));
-    } else {
-      n_met[9] = (at_b(a.med, ob));
-      n_met[10] = (at_b(a.med + cs, ob));
-    }
-  };
-  issue(kbeg);
-
-  for (int k = kbeg; k <= kend; k++) {
-    const unsigned o = (unsigned)(k + kNG) * upz + col;
-    const unsigned qb = o * (unsigned)sizeof(TS);
-    TC hv[NCG][HPER], mt[9 + NMD], pre_v[NOUT], end_v[NOUT];
-#pragma unroll
-    for (int c = 0; c < NCG; c++) {
-#pragma unroll
-      for (int s = 0; s < 4; s++) q[c][s] = q[c][s + 1];
-      q[c][4] = n_lead[c];
-#pragma unroll
-      for (int p = 0; p < HPER; p++) hv[c][p] = n_hv[c][p];
-    }
-#pragma unroll
-    for (int m = 0; m < 9 + NMD; m++) mt[m] = n_met[m];
-    if (k < kend) issue(k + 1);
-#pragma unroll
-    for (int c = 0; c < NOUT; c++) {
-      pre_v[c] = need_pre ? (TC)St<TS>::ld(&at_b(a.w_pre + so + (OOFF + c) * cs, qb)) * sc(OOFF + c) : TC(0);
-      end_v[c] = need_end ? (TC)St<TS>::ld(&at_b(a.w_end + so + (OOFF + c) * cs, qb)) * sc(OOFF + c) : TC(0);
-    }
-    const TC xix = mt[0], xiy = mt[1], xiz = mt[2], etx = mt[3], ety = mt[4], etz = mt[5];
-    const TC ztx = mt[6], zty = mt[7], ztz = mt[8];
-    const int buf = (k - kbeg) & 1;
-    TC *lb = ldsf + buf * LBUF;
-#pragma unroll
-    for (int c = 0; c < NCG; c++) {
-      lb[((COFF + c) * LH + ly + HYL) * LW + lx + HXL] = q[c][zc0];
-#pragma unroll
-      for (int p = 0; p < HPER; p++)
-        if (h_lds[p] >= 0) lb[(COFF + c) * LH * LW + h_lds[p]] = hv[c][p];
-    }
-    __syncthreads();
-
-    TC Lg[3][3];
-    TC dv[3] = {TC(0), TC(0), TC(0)};
-#pragma unroll
-    for (int cg = 0; cg < NCG; cg++) {
-      const int c = COFF + cg;
-      TC dx = TC(0), dy = TC(0), dz = TC(0);
-#pragma unroll
-      for (int n = 0; n < 5; n++) {
-        dx += Drp<TC, DX>::c(n) * lb[(c * LH + ly + HYL) * LW + lx + n];
-        dy += Drp<TC, DY>::c(n) * lb[(c * LH + ly + n) * LW + lx + HXL];
-        dz += Drp<TC, DZ>::c(n) * q[cg][n];
-      }
-      const TC g0 = xix * dx + etx * dy + ztx * dz;
-      const TC g1 = xiy * dx + ety * dy + zty * dz;
-      const TC g2 = xiz * dx + etz * dy + ztz * dz;
-      if (c < 3) {
-        Lg[c][0] = g0; Lg[c][1] = g1; Lg[c][2] = g2;
-      } else if (c == TXX) {
-        dv[0] += g0;
-      } else if (c == TYY) {
-        dv[1] += g1;
-      } else if (c == TZZ) {
-        dv[2] += g2;
-      } else if (c == TYZ) {
-        dv[1] += g2; dv[2] += g1;
-      } else if (c == TXZ) {
-        dv[0] += g2; dv[2] += g0;
-      } else {
-        dv[0] += g1; dv[1] += g0;
-      }
-    }
-
-    if (owner) {
-      TC h[NOUT];
-      if constexpr (ROLE == ROLE_V) {
-        const TC slw = mt[9];
-        h[0] = slw * dv[0];
-        h[1] = slw * dv[1];
-        h[2] = slw * dv[2];
-      } else {
-        const TC lam = mt[9], mu = mt[10];
-        const TC lam2mu = lam + TC(2) * mu;
-        h[TXX - 3] = lam2mu * Lg[0][0] + lam * (Lg[1][1] + Lg[2][2]);
-        h[TYY - 3] = lam2mu * Lg[1][1] + lam * (Lg[0][0] + Lg[2][2]);
-        h[TZZ - 3] = lam2mu * Lg[2][2] + lam * (Lg[0][0] + Lg[1][1]);
-        h[TXY - 3] = mu * (Lg[0][1] + Lg[1][0]);
-        h[TXZ - 3] = mu * (Lg[0][2] + Lg[2][0]);
-        h[TYZ - 3] = mu * (Lg[1][2] + Lg[2][1]);
-      }
-#pragma unroll
-      for (int cg = 0; cg < NOUT; cg++) {
-        const int c = OOFF + cg;
-        TS *wout = a.w_out + so + c * cs;
-        TS *wend = a.w_end + so + c * cs;
-        const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
-        if (mode == MODE_S12) {
-          St<TS>::st(&at_b(wout, qb), (pre_v[cg] + a.ca * h[cg]) * is);
-          St<TS>::st(&at_b(wend, qb), (end_v[cg] + a.cb * h[cg]) * is);
-        } else if (mode == MODE_S0) {
-          St<TS>::st(&at_b(wout, qb), (pre_v[cg] + a.ca * h[cg]) * is);
-          St<TS>::st(&at_b(wend, qb), (pre_v[cg] + a.cb * h[cg]) * is);
-        } else if (mode == MODE_S3) {
-          St<TS>::st(&at_b(wend, qb), (end_v[cg] + a.cb * h[cg]) * is);
-        } else {
-          St<TS>::st(&at_b(wout, qb), h[cg] * is);
-        }
-      }
-    }
-  }
-}
-
-// MINW: minimum waves per SIMD, which caps the registers per thread (4 waves: 128)
-template <int DX, int DY, int DZ, typename TS, typename TC, int TX, int TY, int MINW, bool CART>
-__global__ __launch_bounds__(2 * TX *TY, MINW) void k_march_split(StageArgs<TS, TC> a, MarchSet b, MarchGrid g) {
-  constexpr int NT = TX * TY;
-  __shared__ TC lds[2][kNC][TY + 4][TX + 4];
-  // XCD-aware remap: consecutive hardware workgroup ids land on different
-  // XCDs (round robin).  Each XCD takes a contiguous eighth of every shot's
-  // tiles, shot after shot: halos of neighbouring tiles meet in its L2, and
-  // any subset of active shots keeps all eight XCDs busy.  The grid holds
-  // 8 x part x nshots workgroups; the few past a shot's last tile exit.
-  const int per_shot = g.wg_beg[b.nbox];
-  const int part = (per_shot + 7) / 8;
-  const int hw = blockIdx.x;
-  const int xcd = hw % 8, m = hw / 8;
-  const int shot = m / part;
-  const int rw = xcd * part + (m - shot * part);
-  if (rw >= per_shot) return;
-  int bx = 0;
-  while (bx < b.nbox - 1 && rw >= g.wg_beg[bx + 1]) bx++;
-  const int lw = rw - g.wg_beg[bx];
-  const int ntx = g.ntx[bx], nty = g.nty[bx];
-  const int tx_t = lw % ntx;
-  const int ty_t = (lw / ntx) % nty;
-  const int zc = lw / (ntx * nty);
-  if (a.shot_active && !a.shot_active[shot]) return;
-  const int i0 = b.box[bx][0] + tx_t * TX, j0 = b.box[bx][2] + ty_t * TY;
-  const int kbeg = b.box[bx][4] + zc * b.zchunk;
-  int kend = kbeg + b.zchunk - 1;
-  if (kend > b.box[bx][5]) kend = b.box[bx][5];
-  const int tid = threadIdx.x;
-  // wave-uniform role: NT is a multiple of the wavefront size
-  if (tid < NT)
-    split_sweep<ROLE_V, DX, DY, DZ, TS, TC, TX, TY, CART>(a, &lds[0][0][0][0], tid, i0, j0, b.box[bx][1],
-                                                           b.box[bx][3], kbeg, kend, shot);
-  else
-    split_sweep<ROLE_T, DX, DY, DZ, TS, TC, TX, TY, CART>(a, &lds[0][0][0][0], tid - NT, i0, j0, b.box[bx][1],
-                                                           b.box[bx][3], kbeg, kend, shot);
-}
-
-template <typename TS, typename TC, int TX, int TY, int MINW, bool CART>
-static hipError_t launch_split_t(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
-                                 hipStream_t st) {
-  static_assert((TX * TY) % 64 == 0, "roles must split at a wavefront boundary");
-  MarchGrid g{};
-  int acc = 0;
-  for (int q = 0; q < b.nbox; q++) {
-    g.wg_beg[q] = acc;
-    g.ntx[q] = (b.box[q][1] - b.box[q][0] + TX) / TX;
-    g.nty[q] = (b.box[q][3] - b.box[q][2] + TY) / TY;
-    g.nzc[q] = (b.box[q][5] - b.box[q][4] + b.zchunk) / b.zchunk;
-    acc += g.ntx[q] * g.nty[q] * g.nzc[q];
-  }
-  g.wg_beg[b.nbox] = acc;
-  const long nwg = 8L * ((acc + 7) / 8) * nshots;  // see the remap at the top of the kernel
-  if (nwg == 0) return hipSuccess;
-  dim3 grid((unsigned)nwg), block(2 * TX * TY);
-  const int sel = dx * 4 + dy * 2 + dz;
-#define CGFD_L(X, Y, Z) \
-  hipLaunchKernelGGL((k_march_split<X, Y, Z, TS, TC, TX, TY, MINW, CART>), grid, block, 0, st, a, b, g)
-  switch (sel) {
-    case 0: CGFD_L(0, 0, 0); break;
-    case 1: CGFD_L(0, 0, 1); break;
-    case 2: CGFD_L(0, 1, 0); break;
-    case 3: CGFD_L(0, 1, 1); break;
-    case 4: CGFD_L(1, 0, 0); break;
-    case 5: CGFD_L(1, 0, 1); break;
-    case 6: CGFD_L(1, 1, 0); break;
-    default: CGFD_L(1, 1, 1); break;
-  }
-#undef CGFD_L
-  return hipGetLastError();
 }
 
 // one RK mode's k_march for the operator directions sel = 4 dx + 2 dy + dz
@@ -684,16 +404,13 @@ static hipError_t launch_march_t(const StageArgs<TS, TC> &a, const MarchSet &b, 
   return hipGetLastError();
 }
 
-// tile shapes the autotuner chooses from (TX x TY threads)
+// tile shapes the autotuner chooses from (TX x TY threads, one per column)
 #if !defined(CGFD_TYPE) || CGFD_TYPE == 0
-// 0..3: one thread per column; 4..6: split roles, two threads per column;
-// 7..8: two columns per thread (16-bit storage; other storage types run
-// shape 2 instead)
-int interior_num_shapes() { return 9; }
+int interior_num_shapes() { return 4; }
 
 void interior_shape(int id, int *tx, int *ty) {
-  static const int T[9][2] = {{64, 4}, {32, 8}, {64, 8}, {32, 16}, {64, 8}, {32, 8}, {64, 4}, {64, 8}, {64, 16}};
-  id = id < 0 ? 0 : id > 8 ? 8 : id;
+  static const int T[4][2] = {{64, 4}, {32, 8}, {64, 8}, {32, 16}};
+  id = id < 0 ? 0 : id > 3 ? 3 : id;
   *tx = T[id][0];
   *ty = T[id][1];
 }
@@ -702,22 +419,6 @@ void interior_shape(int id, int *tx, int *ty) {
 template <typename TS, typename TC, bool PML>
 static hipError_t launch_march_shape(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz,
                                      int nshots, hipStream_t st) {
-  // split roles have no PML form (each role would carry its own PML state
-  // under a 128-register cap): PML sets run the one-role 64 x 8 tile instead
-  if (!PML && b.shape >= 4 && b.shape <= 6) {
-    if (a.met_const) {
-      switch (b.shape) {
-        case 5: return launch_split_t<TS, TC, 32, 8, 4, true>(a, b, dx, dy, dz, nshots, st);
-        case 6: return launch_split_t<TS, TC, 64, 4, 4, true>(a, b, dx, dy, dz, nshots, st);
-        default: return launch_split_t<TS, TC, 64, 8, 1, true>(a, b, dx, dy, dz, nshots, st);
-      }
-    }
-    switch (b.shape) {
-      case 5: return launch_split_t<TS, TC, 32, 8, 4, false>(a, b, dx, dy, dz, nshots, st);
-      case 6: return launch_split_t<TS, TC, 64, 4, 4, false>(a, b, dx, dy, dz, nshots, st);
-      default: return launch_split_t<TS, TC, 64, 8, 1, false>(a, b, dx, dy, dz, nshots, st);
-    }
-  }
   if (a.met_const) {
     switch (b.shape) {
       case 0: return launch_march_t<TS, TC, 64, 4, 2, true, PML>(a, b, dx, dy, dz, nshots, st);
@@ -738,11 +439,7 @@ template <typename TS, typename TC>
 hipError_t launch_interior(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
                            hipStream_t st) {
   if (b.nbox < 1) return hipSuccess;
-  // the two-column sweep has no PML form either
   if (b.pml) return launch_march_shape<TS, TC, true>(a, b, dx, dy, dz, nshots, st);
-  if (b.shape >= 7) {
-    if constexpr (sizeof(TS) == 2) return launch_interior_h2<TS>(a, b, dx, dy, dz, nshots, st);
-  }
   return launch_march_shape<TS, TC, false>(a, b, dx, dy, dz, nshots, st);
 }
 

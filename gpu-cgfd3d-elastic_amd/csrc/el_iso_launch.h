@@ -51,9 +51,4 @@ template <typename TS, typename TC>
 hipError_t launch_interior(const StageArgs<TS, TC> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
                            hipStream_t st);
 
-// two columns per lane, 16-bit storage only (tile shapes 7 and 8)
-template <typename TS>
-hipError_t launch_interior_h2(const StageArgs<TS, float> &a, const MarchSet &b, int dx, int dy, int dz, int nshots,
-                              hipStream_t st);
-
 }  // namespace cgfd

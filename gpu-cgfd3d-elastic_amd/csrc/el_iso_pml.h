@@ -8,7 +8,8 @@
 // along the face's axis, rhs += (B-1) R - B Phi, Phi' = D R - A Phi, plus the
 // free-surface conversion term on the top row of x and y faces (:852-915,
 // :1000-1060).  The memory variables are then advanced with the same RK
-// combination as the wavefield (drv_rk_curv_col.cu:357-477).
+// step as the wavefield (drv_rk_curv_col.cu:357-477), in the stage-value
+// form of Mode (el_iso_device.h).
 #pragma once
 #include "el_iso_device.h"
 
@@ -43,13 +44,16 @@ __device__ __forceinline__ void pml_face(const StageArgs<TS, TC> &a, int fi, int
   R[TYZ] = mu * (e3 * Dd[VY] + e2 * Dd[VZ]);
   const long fso = (long)shot * F.shot_stride;
   const unsigned fs = (unsigned)F.fsize;
-  TC P[kNC], Ph[kNC], pre_a[kNC], end_a[kNC];
+  TC P[kNC], Ph[kNC], pre_a[kNC], u1_a[kNC], u2_a[kNC];
 #pragma unroll
   for (int c = 0; c < kNC; c++) {
     const unsigned qb = (c * fs + ia) * (unsigned)sizeof(TS);
     P[c] = (TC)St<TS>::ld(&at_b(F.a_in + fso, qb)) * S(c);
-    if constexpr (MODE == MODE_S12) pre_a[c] = (TC)St<TS>::ld(&at_b(F.a_pre + fso, qb)) * S(c);
-    if constexpr (MODE == MODE_S12 || MODE == MODE_S3) end_a[c] = (TC)St<TS>::ld(&at_b(F.a_end + fso, qb)) * S(c);
+    if constexpr (MODE == MODE_S12 || MODE == MODE_S3) pre_a[c] = (TC)St<TS>::ld(&at_b(F.a_pre + fso, qb)) * S(c);
+    if constexpr (MODE == MODE_S3) {
+      u1_a[c] = (TC)St<TS>::ld(&at_b(F.a_end + fso, qb)) * S(c);
+      u2_a[c] = (TC)St<TS>::ld(&at_b(F.a_u2 + fso, qb)) * S(c);
+    }
   }
 #pragma unroll
   for (int c = 0; c < kNC; c++) {
@@ -85,12 +89,11 @@ __device__ __forceinline__ void pml_face(const StageArgs<TS, TC> &a, int fi, int
       St<TS>::st(&at_b(F.a_out + fso, qb), Ph[c] * is);
     } else if constexpr (MODE == MODE_S0) {
       St<TS>::st(&at_b(F.a_out + fso, qb), (P[c] + a.ca * Ph[c]) * is);
-      St<TS>::st(&at_b(F.a_end + fso, qb), (P[c] + a.cb * Ph[c]) * is);
     } else if constexpr (MODE == MODE_S12) {
       St<TS>::st(&at_b(F.a_out + fso, qb), (pre_a[c] + a.ca * Ph[c]) * is);
-      St<TS>::st(&at_b(F.a_end + fso, qb), (end_a[c] + a.cb * Ph[c]) * is);
     } else {
-      St<TS>::st(&at_b(F.a_end + fso, qb), (end_a[c] + a.cb * Ph[c]) * is);
+      const TC part = rk_partial(a.rk_w, pre_a[c], u1_a[c], u2_a[c]);
+      St<TS>::st(&at_b(F.a_end + fso, qb), rk_final(pre_a[c], part, a.rk_w[2], P[c], a.cb * Ph[c]) * is);
     }
   }
 }
@@ -114,7 +117,7 @@ __device__ __forceinline__ void pml_face_march(const StageArgs<TS, TC> &a, int f
   const FaceArgs<TS, TC> &F = a.f[fi];
   const int i1 = F.i1, j1 = F.j1, k1 = F.k1, bi = F.bi, bj = F.bj;
   const long fso = (long)shot * F.shot_stride;
-  const TS *pin = F.a_in + fso, *ppre = F.a_pre + fso;
+  const TS *pin = F.a_in + fso, *ppre = F.a_pre + fso, *pu2 = F.a_u2 + fso;
   TS *pout = F.a_out + fso, *pend = F.a_end + fso;
   const unsigned fs = (unsigned)F.fsize;
   const unsigned ia = (unsigned)(((k - k1) * bj + (j - j1)) * bi + (i - i1));
@@ -131,18 +134,18 @@ __device__ __forceinline__ void pml_face_march(const StageArgs<TS, TC> &a, int f
   R[TXY] = mu * (e2 * Dd[VX] + e1 * Dd[VY]);
   R[TXZ] = mu * (e3 * Dd[VX] + e1 * Dd[VZ]);
   R[TYZ] = mu * (e3 * Dd[VY] + e2 * Dd[VZ]);
-  TC P[kNC], pre[kNC], en[kNC];
+  TC P[kNC], pre[kNC], part[kNC];
 #pragma unroll
   for (int c = 0; c < kNC; c++) P[c] = (TC)St<TS>::ld(&at_b(pin, qb(c))) * S(c);
-  if constexpr (MODE == MODE_S12) {
+  if constexpr (MODE == MODE_S12 || MODE == MODE_S3) {
 #pragma unroll
-    for (int c = 0; c < kNC; c++) {
-      pre[c] = (TC)St<TS>::ld(&at_b(ppre, qb(c))) * S(c);
-      en[c] = (TC)St<TS>::ld(&at_b(pend, qb(c))) * S(c);
-    }
-  } else if constexpr (MODE == MODE_S3) {
+    for (int c = 0; c < kNC; c++) pre[c] = (TC)St<TS>::ld(&at_b(ppre, qb(c))) * S(c);
+  }
+  if constexpr (MODE == MODE_S3) {
 #pragma unroll
-    for (int c = 0; c < kNC; c++) en[c] = (TC)St<TS>::ld(&at_b(pend, qb(c))) * S(c);
+    for (int c = 0; c < kNC; c++)
+      part[c] = rk_partial(a.rk_w, pre[c], (TC)St<TS>::ld(&at_b(pend, qb(c))) * S(c),
+                           (TC)St<TS>::ld(&at_b(pu2, qb(c))) * S(c));
   }
   TC cD, cA, cB;
   if constexpr (DIM == 2) {
@@ -166,19 +169,14 @@ __device__ __forceinline__ void pml_face_march(const StageArgs<TS, TC> &a, int f
   }
   if constexpr (MODE == MODE_S12) {
 #pragma unroll
-    for (int c = 0; c < kNC; c++) {
-      St<TS>::st(&at_b(pout, qb(c)), (pre[c] + a.ca * Ph[c]) * IS(c));
-      St<TS>::st(&at_b(pend, qb(c)), (en[c] + a.cb * Ph[c]) * IS(c));
-    }
+    for (int c = 0; c < kNC; c++) St<TS>::st(&at_b(pout, qb(c)), (pre[c] + a.ca * Ph[c]) * IS(c));
   } else if constexpr (MODE == MODE_S0) {
 #pragma unroll
-    for (int c = 0; c < kNC; c++) {
-      St<TS>::st(&at_b(pout, qb(c)), (P[c] + a.ca * Ph[c]) * IS(c));
-      St<TS>::st(&at_b(pend, qb(c)), (P[c] + a.cb * Ph[c]) * IS(c));
-    }
+    for (int c = 0; c < kNC; c++) St<TS>::st(&at_b(pout, qb(c)), (P[c] + a.ca * Ph[c]) * IS(c));
   } else if constexpr (MODE == MODE_S3) {
 #pragma unroll
-    for (int c = 0; c < kNC; c++) St<TS>::st(&at_b(pend, qb(c)), (en[c] + a.cb * Ph[c]) * IS(c));
+    for (int c = 0; c < kNC; c++)
+      St<TS>::st(&at_b(pend, qb(c)), rk_final(pre[c], part[c], a.rk_w[2], P[c], a.cb * Ph[c]) * IS(c));
   } else {
 #pragma unroll
     for (int c = 0; c < kNC; c++) St<TS>::st(&at_b(pout, qb(c)), Ph[c] * IS(c));

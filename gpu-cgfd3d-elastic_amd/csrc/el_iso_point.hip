@@ -4,7 +4,8 @@
 // derivatives, the momentum/Hooke right-hand side (inner form, or the
 // traction-image / vlow forms on the free-surface rows), the CFS-PML ADE
 // corrections of every face box containing the point, then the fused RK
-// update of the wavefield and of the PML memory.  It covers any list of boxes,
+// update of the wavefield and of the PML memory (stage-value form, see Mode
+// in el_iso_device.h).  It covers any list of boxes,
 // so it serves both as the boundary-shell kernel next to the z-marching
 // interior kernel and as the whole-domain kernel of the plain method.
 //
@@ -39,7 +40,7 @@ __device__ __forceinline__ void rk_store(const StageArgs<TS, TC> &a, long so, in
 #pragma unroll
   for (int c = 0; c < kNC; c++) {
     TS *wout = a.w_out + so + c * cs, *wend = a.w_end + so + c * cs;
-    const TS *wpre = a.w_pre + so + c * cs;
+    const TS *wpre = a.w_pre + so + c * cs, *wu2 = a.w_u2 + so + c * cs;
     const TS *win = a.w_in + so + c * cs;
     const TC sc = St<TS>::scaled ? (TC)a.sc[c] : TC(1);
     const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
@@ -48,15 +49,14 @@ __device__ __forceinline__ void rk_store(const StageArgs<TS, TC> &a, long so, in
     } else if constexpr (MODE == MODE_S0) {
       const TC v = (TC)St<TS>::ld(&at_b(win, qb)) * sc;
       St<TS>::st(&at_b(wout, qb), (v + a.ca * h[c]) * is);
-      St<TS>::st(&at_b(wend, qb), (v + a.cb * h[c]) * is);
     } else if constexpr (MODE == MODE_S12) {
       const TC pre = (TC)St<TS>::ld(&at_b(wpre, qb)) * sc;
-      const TC en = (TC)St<TS>::ld(&at_b(wend, qb)) * sc;
       St<TS>::st(&at_b(wout, qb), (pre + a.ca * h[c]) * is);
-      St<TS>::st(&at_b(wend, qb), (en + a.cb * h[c]) * is);
     } else {
-      const TC en = (TC)St<TS>::ld(&at_b(wend, qb)) * sc;
-      St<TS>::st(&at_b(wend, qb), (en + a.cb * h[c]) * is);
+      const TC pre = (TC)St<TS>::ld(&at_b(wpre, qb)) * sc;
+      const TC u3 = (TC)St<TS>::ld(&at_b(win, qb)) * sc;
+      const TC part = rk_partial(a.rk_w, pre, (TC)St<TS>::ld(&at_b(wend, qb)) * sc, (TC)St<TS>::ld(&at_b(wu2, qb)) * sc);
+      St<TS>::st(&at_b(wend, qb), rk_final(pre, part, a.rk_w[2], u3, a.cb * h[c]) * is);
     }
   }
 }
@@ -261,11 +261,12 @@ __global__ __launch_bounds__(256) void k_source(SrcArgs<TS, TC> a) {
     const long q = so + c * cs + o;
     const TC sc = St<TS>::scaled ? TC(1) / (TC)a.isc[c] : TC(1);
     const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
+    // stage values take a_i dt of the source (U_{i+1} = W + a_i dt (K_i + S_i)),
+    // W^{n+1} takes b3 dt of stage 3's; those of stages 0..2 reach it through U1..U3
     if (a.mode == MODE_RHS) {
       St<TS>::st(a.w_out + q, ((TC)St<TS>::ld(a.w_out + q) * sc + v[c]) * is);
     } else if (a.mode == MODE_S0 || a.mode == MODE_S12) {
       St<TS>::st(a.w_out + q, ((TC)St<TS>::ld(a.w_out + q) * sc + a.ca * v[c]) * is);
-      St<TS>::st(a.w_end + q, ((TC)St<TS>::ld(a.w_end + q) * sc + a.cb * v[c]) * is);
     } else {
       St<TS>::st(a.w_end + q, ((TC)St<TS>::ld(a.w_end + q) * sc + a.cb * v[c]) * is);
     }

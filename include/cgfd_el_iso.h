@@ -45,10 +45,15 @@ extern "C" {
 #define CGFD_DT_F64 1
 
 /* wavefield levels of one shot */
+/* Four wavefield levels per shot.  A step stores the stage values
+ * U_{i+1} = W^n + a_i dt K_i and forms W^{n+1} = W^n + w1 (U1 - W^n) +
+ * w2 (U2 - W^n) + w3 (U3 - W^n) + b3 dt K3 (w = 1/3, 2/3, 1/3), the classical
+ * RK4 step of drv_rk_curv_col.cu:193-542 with 117 instead of 144 level
+ * words moved per point and step. */
 #define CGFD_LEVEL_PRE 0   /* W^n: the level receivers and snapshots read */
-#define CGFD_LEVEL_TMPA 1  /* RK intermediate (ping) */
-#define CGFD_LEVEL_TMPB 2  /* RK intermediate (pong) */
-#define CGFD_LEVEL_END 3   /* RK accumulator of W^{n+1} */
+#define CGFD_LEVEL_TMPA 1  /* after a step: W^{n-1}; during it U1 (stage value of stage 0) */
+#define CGFD_LEVEL_TMPB 2  /* U2 of the last step */
+#define CGFD_LEVEL_END 3   /* U3 of the last step */
 #define CGFD_LEVEL_RHS 4   /* right-hand side written by cgfd_el_iso_onestage */
 
 /* PML faces */
@@ -189,9 +194,7 @@ int cgfd_fetch_subvolume(cgfd_session *s, int shot, int level, unsigned cmp_mask
 int cgfd_fetch_pml_aux(cgfd_session *s, int shot, int face, int level, int dtype, void *out);
 
 /* Tile of the z-marching kernel: shape id (0..3: 64x4, 32x8, 64x8, 32x16
- * threads, one per column; 4..6: 64x8, 32x8, 64x4 columns with two threads
- * each (split roles); 7..8: 64x8, 64x16 columns with two columns per thread,
- * 16-bit storage only) and z planes per workgroup.  By default the first
+ * threads, one per column) and z planes per workgroup.  By default the first
  * cgfd_run times every candidate tile under each work split (below) and keeps
  * the fastest pair (cached per grid, storage, shots and flags for the
  * process); shape < 0 returns to that automatic choice.  A pinned tile runs
@@ -203,6 +206,10 @@ int cgfd_get_tile(cgfd_session *s, int *shape, int *tx, int *ty, int *zchunk);
 #define CGFD_SPLIT_SHELL 0  /* sweep inside the PML face boxes; per-point kernel on the slabs and 4 surface rows */
 #define CGFD_SPLIT_PLANES 1 /* sweep whole planes with the PML terms; per-point kernel on the 4 surface rows */
 int cgfd_get_split(cgfd_session *s, int *split);
+
+/* 1 when the uploaded metric runs the constant-metric sweep (a Cartesian
+ * grid: nine metric values as kernel arguments instead of nine streams). */
+int cgfd_get_const_metric(cgfd_session *s, int *const_metric);
 
 /* Enable or disable (default) HIP-event timing of every interior kernel
  * launch.  Timed runs are enqueued launch by launch, without graphs. */
@@ -251,6 +258,14 @@ int cgfd_wait(cgfd_session *s);
  * duration of its z-marching interior kernel launches (events on the
  * session stream). */
 int cgfd_last_run_timing(cgfd_session *s, double *run_ms, double *interior_avg_ms, int *interior_launches);
+
+/* Achievable HBM bandwidth of the current device (no reference counterpart;
+ * the ceiling the sweep kernels are compared with): a streaming read and a
+ * streaming copy of `bytes` bytes (two buffers allocated and freed here),
+ * 16-byte non-temporal accesses over a grid covering every XCD, each timed
+ * over `reps` launches.  read_gbps counts the bytes read, copy_gbps the
+ * bytes read plus written, in 1e9 bytes per second. */
+int cgfd_hbm_stream(long long bytes, int reps, double *read_gbps, double *copy_gbps);
 
 #ifdef __cplusplus
 }

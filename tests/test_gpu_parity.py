@@ -136,12 +136,11 @@ def test_multishot_equals_single_shot():
 
 
 @pytest.mark.parametrize("flags", [0, cgfd.FLAG_PML_POINT], ids=["planes", "shell"])
-@pytest.mark.parametrize("shape,zchunk", [(0, 7), (1, 48), (2, 5), (3, 24), (4, 6), (5, 11), (6, 24)])
+@pytest.mark.parametrize("shape,zchunk", [(0, 7), (1, 48), (2, 5), (3, 24), (0, 96), (1, 33)])
 def test_every_interior_tile_shape(shape, zchunk, flags):
     """Each z-marching tile shape and a ragged z chunk reproduce the oracle,
     under both work splits (a pinned tile runs the whole-plane split unless
-    the per-point shell is asked for; there the split-role shapes 4-6 run as
-    64x8 one-role tiles with the PML terms)."""
+    the per-point shell is asked for)."""
     P = Problem(ni=75, nj=41, nk=37, curvilinear=True, pml=(4, 3, 5, 2, 4, 0), nt_total=12, seed=12, hetero=0.04)
     s = _sess(P, flags=flags)
     s.set_tile(shape, zchunk)
@@ -160,7 +159,7 @@ def test_autotune_picks_a_valid_tile():
     s = _sess(P)
     s.run(0, 2, P.dt)
     t = s.get_tile()
-    assert 0 <= t["shape"] < 9 and t["zchunk"] >= 1
+    assert 0 <= t["shape"] < 4 and t["zchunk"] >= 1
     assert s.get_split() in (cgfd.SPLIT_SHELL, cgfd.SPLIT_PLANES)
     s.close()
 
