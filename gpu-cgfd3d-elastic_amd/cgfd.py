@@ -40,6 +40,7 @@ EXPORTS = [
     "cgfd_storage_scales",
     "cgfd_server_create", "cgfd_server_destroy", "cgfd_server_add_model", "cgfd_server_submit",
     "cgfd_server_wait", "cgfd_server_result", "cgfd_server_get_stats", "cgfd_fetch_subvolume",
+    "cgfd_fetch_subvolume_noalloc",
     "cgfd_sac_write", "cgfd_station_sac", "cgfd_snap_open", "cgfd_snap_append", "cgfd_snap_close",
     "cgfd_set_peak_motion", "cgfd_read_peak_motion", "cgfd_peak_motion_write", "cgfd_hbm_stream",
 ]
@@ -189,6 +190,7 @@ def load(path=LIB_PATH):
                                  vp, vp, vp], i),
         "cgfd_storage_scales": ([i, i, i, i, vp, vp, vp, vp, C.POINTER(SourceDesc), d, vp], i),
         "cgfd_fetch_subvolume": ([vp, i, i, C.c_uint, i, i, i, i, i, i, i, i, i, i, vp], i),
+        "cgfd_fetch_subvolume_noalloc": ([vp, i, i, C.c_uint, i, i, i, i, i, i, i, i, i, i, vp], i),
         "cgfd_sac_write": ([C.c_char_p, vp, i, C.c_float, C.c_float, vp, vp, C.c_char_p, C.c_char_p], i),
         "cgfd_station_sac": ([C.c_char_p, C.c_char_p, vp, i, C.c_float, C.c_float, C.c_float, C.c_float, i,
                               C.c_float, C.c_float], i),
@@ -369,6 +371,19 @@ class Session:
         out = np.zeros((len(comps), nk, nj, ni), dtype=dtype)
         _check(_lib.cgfd_fetch_subvolume(self.h, shot, level, mask, i1, j1, k1, ni, nj, nk, di, dj, dk,
                                          DT_F64 if dtype == np.float64 else DT_F32, _ptr(out)))
+        return out
+
+    def fetch_subvolume_noalloc(self, shot, level, comps, start, count, stride=(1, 1, 1)):
+        """fetch_subvolume through the library's fixed device scratch (no device
+        allocation); output in the session's model type."""
+        mask = 0
+        for c in comps:
+            mask |= 1 << c
+        (i1, j1, k1), (ni, nj, nk), (di, dj, dk) = start, count, stride
+        dtype = np.float64 if self.storage == STORE_F64 else np.float32
+        out = np.zeros((len(comps), nk, nj, ni), dtype=dtype)
+        _check(_lib.cgfd_fetch_subvolume_noalloc(self.h, shot, level, mask, i1, j1, k1, ni, nj, nk, di, dj, dk,
+                                                 DT_F64 if dtype == np.float64 else DT_F32, _ptr(out)))
         return out
 
     def set_peak_motion(self, on):

@@ -1422,9 +1422,91 @@ int subvolume_t(cgfd_session *s, int shot, int level, unsigned mask, int i1, int
   }
   return CGFD_OK;
 }
+// ---- the sub-volume fetch that allocates nothing -----------------------------
+// scratch: device staging of cgfd_fetch_subvolume_noalloc, a file-scope
+// array every GPU the library runs on holds until the process ends.  Host
+// code never uses its name as an address: the address on each device comes
+// from hipGetSymbolAddress, taken once per device with that device current.
+constexpr size_t kScratchBytes = 8u << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(int dev, void **out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(CGFD_E_ARG, "device %d beyond the %d the library tracks", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return fail(CGFD_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return CGFD_OK;
+}
+
+// The selected components one after another through the scratch staging:
+// gather on the session stream, copy straight into the caller's buffer.
+template <typename TS, typename TC>
+int subvolume_noalloc_t(cgfd_session *s, void *scr, int shot, int level, unsigned mask, int i1, int j1, int k1,
+                        int ni, int nj, int nk, int di, int dj, int dk, void *out) {
+  const size_t n = (size_t)ni * nj * nk;
+  size_t q = 0;
+  for (int c = 0; c < kNC; c++) {
+    if (!(mask >> c & 1u)) continue;
+    const TS *w = (const TS *)wav_level(s, shot, s->slot[level]) + (size_t)c * s->L.cs;
+    const TC sc = St<TS>::scaled ? (TC)s->sc[c] : TC(1);
+    hipError_t e = launch_gather<TS, TC>(w, s->L, i1, j1, k1, ni, nj, nk, di, dj, dk, sc, (TC *)scr, s->st);
+    if (e != hipSuccess) return fail(CGFD_E_HIP, "gather kernel: %s", hipGetErrorString(e));
+    HIPCHK(hipMemcpyAsync((TC *)out + q, scr, n * sizeof(TC), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    q += n;
+  }
+  return CGFD_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int cgfd_fetch_subvolume_noalloc(cgfd_session *s, int shot, int level, unsigned cmp_mask, int i1, int j1, int k1,
+                                 int ni, int nj, int nk, int di, int dj, int dk, int dtype, void *out) {
+  if (!s || shot < 0 || shot >= s->d.nshots || level < 0 || level > 3 || !out)
+    return fail(CGFD_E_ARG, "bad fetch_subvolume_noalloc arguments");
+  if (ni < 1 || nj < 1 || nk < 1 || di < 1 || dj < 1 || dk < 1 || i1 < 0 || j1 < 0 || k1 < 0 ||
+      i1 + (long)(ni - 1) * di > s->d.ni - 1 || j1 + (long)(nj - 1) * dj > s->d.nj - 1 ||
+      k1 + (long)(nk - 1) * dk > s->d.nk - 1)
+    return fail(CGFD_E_ARG, "sub-volume outside the grid");
+  if ((cmp_mask & 0x1ffu) == 0 || (cmp_mask & ~0x1ffu)) return fail(CGFD_E_ARG, "component mask %#x", cmp_mask);
+  const int want = s->ms == 8 ? CGFD_DT_F64 : CGFD_DT_F32;
+  if (dtype != want)
+    return fail(CGFD_E_ARG, "fetch_subvolume_noalloc writes the session's model type (%s)",
+                want == CGFD_DT_F64 ? "CGFD_DT_F64" : "CGFD_DT_F32");
+  const unsigned long long bytes = (unsigned long long)ni * nj * nk * s->ms;
+  if (bytes > kScratchBytes)
+    return fail(CGFD_E_ARG, "sub-volume of %llu bytes per component exceeds the %zu-byte scratch of the "
+                            "non-allocating fetch", bytes, kScratchBytes);
+  int cur = 0;
+  HIPCHK(hipGetDevice(&cur));
+  if (cur != s->dev) HIPCHK(hipSetDevice(s->dev));
+  void *scr = nullptr;
+  int r = scratch_address(s->dev, &scr);
+  if (!r) {
+    hipError_t e = hipStreamSynchronize(s->st);
+    if (e != hipSuccess) r = fail(CGFD_E_HIP, "stream sync: %s", hipGetErrorString(e));
+  }
+  if (!r) {
+    switch (s->d.storage) {
+      case CGFD_STORE_F32: r = subvolume_noalloc_t<float, float>(s, scr, shot, level, cmp_mask, i1, j1, k1, ni, nj, nk, di, dj, dk, out); break;
+      case CGFD_STORE_F64: r = subvolume_noalloc_t<double, double>(s, scr, shot, level, cmp_mask, i1, j1, k1, ni, nj, nk, di, dj, dk, out); break;
+      case CGFD_STORE_F16: r = subvolume_noalloc_t<__half, float>(s, scr, shot, level, cmp_mask, i1, j1, k1, ni, nj, nk, di, dj, dk, out); break;
+      default: r = subvolume_noalloc_t<__hip_bfloat16, float>(s, scr, shot, level, cmp_mask, i1, j1, k1, ni, nj, nk, di, dj, dk, out); break;
+    }
+  }
+  if (cur != s->dev) (void)hipSetDevice(cur);
+  return r;
+}
 
 int cgfd_fetch_subvolume(cgfd_session *s, int shot, int level, unsigned cmp_mask, int i1, int j1, int k1, int ni,
                          int nj, int nk, int di, int dj, int dk, int dtype, void *out) {

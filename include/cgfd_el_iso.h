@@ -189,6 +189,18 @@ int cgfd_fetch_level(cgfd_session *s, int shot, int level, int dtype, void *w9);
 int cgfd_fetch_subvolume(cgfd_session *s, int shot, int level, unsigned cmp_mask, int i1, int j1, int k1, int ni,
                          int nj, int nk, int di, int dj, int dk, int dtype, void *out);
 
+/* cgfd_fetch_subvolume for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); it has no counterpart in the reference.  Same selection and
+ * output layout, but the gather stages through a fixed 8 MiB device array
+ * the library holds per GPU (never hipMalloc/hipFree): one component of the
+ * sub-volume (ni*nj*nk values of the model type) must fit in it, and dtype
+ * must be the session's model type (CGFD_DT_F64 for FP64 sessions,
+ * CGFD_DT_F32 otherwise); larger requests fail with CGFD_E_ARG.  Calls
+ * that overlap on one GPU share that array and race. */
+int cgfd_fetch_subvolume_noalloc(cgfd_session *s, int shot, int level, unsigned cmp_mask, int i1, int j1, int k1,
+                                 int ni, int nj, int nk, int di, int dj, int dk, int dtype, void *out);
+
 /* PML memory of one face: level = PRE/TMPA/TMPB/END or RHS; [9][face box]
  * with the face box indexed iz*(nj*ni)+iy*ni+ix (sv_curv_col_el_iso_gpu.cu:775). */
 int cgfd_fetch_pml_aux(cgfd_session *s, int shot, int face, int level, int dtype, void *out);
