@@ -43,6 +43,7 @@ EXPORTS = [
     "cgfd_fetch_subvolume_noalloc",
     "cgfd_sac_write", "cgfd_station_sac", "cgfd_snap_open", "cgfd_snap_append", "cgfd_snap_close",
     "cgfd_set_peak_motion", "cgfd_read_peak_motion", "cgfd_peak_motion_write", "cgfd_hbm_stream",
+    "cgfd_slice_open", "cgfd_slice_append", "cgfd_slice_close", "cgfd_line_sac", "cgfd_line_points",
 ]
 
 
@@ -208,6 +209,11 @@ def load(path=LIB_PATH):
         "cgfd_server_result": ([vp, ll, i, vp, C.POINTER(d), C.POINTER(d)], i),
         "cgfd_server_get_stats": ([vp, C.POINTER(ServerStats)], i),
         "cgfd_hbm_stream": ([ll, i, C.POINTER(d), C.POINTER(d)], i),
+        "cgfd_slice_open": ([C.c_char_p, i, i, i, i, vp, C.POINTER(vp)], i),
+        "cgfd_slice_append": ([vp, C.c_float, vp], i),
+        "cgfd_slice_close": ([vp], i),
+        "cgfd_line_sac": ([C.c_char_p, C.c_char_p, i, vp, i, C.c_float, C.c_float, C.c_float, C.c_float], i),
+        "cgfd_line_points": ([i, i, i, vp, vp, vp, vp, vp, i, vp, vp, vp, vp], i),
     })
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -776,6 +782,73 @@ class Snapshot:
         if self.h:
             _check(_lib.cgfd_snap_close(self.h))
             self.h = C.c_void_p()
+
+
+def slice_path(out_dir, axis, gindex, part="px0_py0_pz0"):
+    """The reference's slice file name (io_slice_locate, io_funcs.cu:457-512):
+    axis 0/1/2 -> slicex_i / slicey_j / slicez_k, 1-based plane number."""
+    return os.path.join(str(out_dir), f"slice{'xyz'[axis]}_{'ijk'[axis]}{gindex + 1}_{part}.nc")
+
+
+class Slice:
+    """NetCDF-3 slice file with the reference's dimensions, names and attributes."""
+
+    def __init__(self, path, axis, gindex, ni, nj, nk, topo=(0, 0, 0)):
+        load()
+        if axis not in (0, 1, 2):
+            raise ValueError("slice axis is 0 (x), 1 (y) or 2 (z)")
+        if not 0 <= gindex < (ni, nj, nk)[axis]:
+            raise ValueError(f"slice plane {gindex} outside the grid's {(ni, nj, nk)[axis]} planes")
+        self.axis = axis
+        self.shape = [(nk, nj), (nk, ni), (nj, ni)][axis]
+        self.h = C.c_void_p()
+        t = (C.c_int * 3)(*topo)
+        _check(_lib.cgfd_slice_open(str(path).encode(), axis, gindex + 3, self.shape[0], self.shape[1], t,
+                                    C.byref(self.h)))
+
+    def append(self, time, w9):
+        """w9: [9][n_slow][n_fast] (e.g. a Session.fetch_subvolume plane, squeezed)."""
+        w = _arr(np.asarray(w9).reshape(9, *self.shape), np.float32)
+        _check(_lib.cgfd_slice_append(self.h, float(time), _ptr(w)))
+
+    def close(self):
+        if self.h:
+            _check(_lib.cgfd_slice_close(self.h))
+            self.h = C.c_void_p()
+
+
+def session_slice(sess, shot, axis, gindex):
+    """The plane gindex (0-based physical) of the nine W^n components, [9][slow][fast], gathered on the device."""
+    ni, nj, nk = sess.ni, sess.nj, sess.nk
+    start = [0, 0, 0]
+    count = [ni, nj, nk]
+    start[axis], count[axis] = gindex, 1
+    w = sess.fetch_subvolume(shot, LEVEL_PRE, list(range(9)), tuple(start), tuple(count))
+    return np.squeeze(w, axis=3 - axis)
+
+
+def line_points(ni, nj, nk, start, incre, count, x=None, y=None, z=None):
+    """Receiver line (io_line_locate): (seq, indx8, frac3, xyz) of its points inside the grid."""
+    load()
+    seq = np.zeros(max(count, 1), dtype=np.int32)
+    ix = np.zeros((max(count, 1), 8), dtype=np.int64)
+    fr = np.zeros((max(count, 1), 3))
+    xyz = np.zeros((max(count, 1), 3))
+    st, inc = np.asarray(start, np.int32), np.asarray(incre, np.int32)
+    xs = [_arr(a, np.float64) if a is not None else None for a in (x, y, z)]
+    n = _lib.cgfd_line_points(ni, nj, nk, *[_ptr(a) for a in xs], _ptr(st), _ptr(inc), int(count), _ptr(seq),
+                              _ptr(ix), _ptr(fr), _ptr(xyz) if x is not None else None)
+    if n < 0:
+        _check(n)
+    return seq[:n], ix[:n], fr[:n], xyz[:n]
+
+
+def line_sac(out_dir, line, seq, seismo, dt, x, y, z):
+    """{out_dir}/{line}.no{seq}.{Vx..Txy}.sac of one line point (io_line_output_sac)."""
+    load()
+    s_ = _arr(seismo, np.float32)
+    _check(_lib.cgfd_line_sac(str(out_dir).encode(), line.encode(), int(seq), _ptr(s_), s_.shape[-1], float(dt),
+                              float(x), float(y), float(z)))
 
 
 PEAK_NAMES = ["PGV", "PGVh", "PGVx", "PGVy", "PGVz", "PGA", "PGAh", "PGAx", "PGAy", "PGAz",

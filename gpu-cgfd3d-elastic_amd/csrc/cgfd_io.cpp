@@ -69,7 +69,120 @@ struct cgfd_snap {
   std::vector<unsigned char> buf;
 };
 
+struct cgfd_slice {
+  FILE *fp = nullptr;
+  long long data_begin = 0;  // offset of record 0
+  long long slab = 0;        // bytes of one component per record
+  long long recsize = 0;
+  uint32_t nrec = 0;
+  long long nvals = 0;       // values per component per record
+};
+
+namespace {
+// component names of the wavefield, in storage order (wav_t.cu:100-140)
+const char *kCmpName[9] = {"Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Tyz", "Txz", "Txy"};
+}  // namespace
+
 extern "C" {
+
+int cgfd_slice_open(const char *path, int axis, int index_with_ghosts, int n_slow, int n_fast, const int *topo,
+                    cgfd_slice **out) {
+  if (!path || !out || axis < 0 || axis > 2 || n_slow < 1 || n_fast < 1 || !topo)
+    return iofail(CGFD_E_ARG, "cgfd_slice_open: bad arguments");
+  *out = nullptr;
+  static const char *dims[3][2] = {{"k", "j"}, {"k", "i"}, {"j", "i"}};
+  static const char *att[3] = {"i_index_with_ghosts_in_this_thread", "j_index_with_ghosts_in_this_thread",
+                               "k_index_with_ghosts_in_this_thread"};
+  auto *sl = new cgfd_slice();
+  sl->nvals = (long long)n_slow * n_fast;
+  sl->slab = 4 * sl->nvals;
+  auto header = [&](uint32_t begin0) {
+    BE h;
+    h.b.insert(h.b.end(), {'C', 'D', 'F', 1});
+    h.u32(sl->nrec);
+    h.u32(NC_DIMENSION);
+    h.u32(3);
+    h.name("time"); h.u32(0);
+    h.name(dims[axis][0]); h.u32((uint32_t)n_slow);
+    h.name(dims[axis][1]); h.u32((uint32_t)n_fast);
+    h.u32(NC_ATTRIBUTE);
+    h.u32(2);
+    h.name(att[axis]); h.u32(NC_INT); h.u32(1); h.u32((uint32_t)index_with_ghosts);
+    h.name("coords_of_mpi_topo"); h.u32(NC_INT); h.u32(3);
+    for (int q = 0; q < 3; q++) h.u32((uint32_t)topo[q]);
+    h.u32(NC_VARIABLE);
+    h.u32(10);
+    uint32_t off = begin0;
+    h.name("time");
+    h.u32(1); h.u32(0);
+    h.u32(0); h.u32(0);
+    h.u32(NC_FLOAT);
+    h.u32(4);
+    h.u32(off);
+    off += 4;
+    for (const char *v : kCmpName) {
+      h.name(v);
+      h.u32(3); h.u32(0); h.u32(1); h.u32(2);
+      h.u32(0); h.u32(0);
+      h.u32(NC_FLOAT);
+      h.u32((uint32_t)sl->slab);
+      h.u32(off);
+      off += (uint32_t)sl->slab;
+    }
+    return h.b;
+  };
+  const std::vector<unsigned char> h0 = header(0);
+  sl->data_begin = (long long)h0.size();
+  sl->recsize = 4 + 9 * sl->slab;
+  if (sl->data_begin + sl->recsize > 0x7fffffffLL) {
+    delete sl;
+    return iofail(CGFD_E_ARG, "slice record too large for the classic format offsets");
+  }
+  const std::vector<unsigned char> h1 = header((uint32_t)sl->data_begin);
+  sl->fp = std::fopen(path, "wb");
+  if (!sl->fp) {
+    delete sl;
+    return iofail(CGFD_E_ARG, std::string("cannot open ") + path + ": " + std::strerror(errno));
+  }
+  if (std::fwrite(h1.data(), 1, h1.size(), sl->fp) != h1.size()) {
+    std::fclose(sl->fp);
+    delete sl;
+    return iofail(CGFD_E_ARG, std::string("write failed: ") + path);
+  }
+  *out = sl;
+  return CGFD_OK;
+}
+
+int cgfd_slice_append(cgfd_slice *sl, float time, const float *w) {
+  if (!sl || !w) return iofail(CGFD_E_ARG, "cgfd_slice_append: bad arguments");
+  BE r;
+  r.b.reserve((size_t)sl->recsize);
+  r.f32(time);
+  for (long long q = 0; q < 9 * sl->nvals; q++) r.f32(w[q]);
+  const long long off = sl->data_begin + (long long)sl->nrec * sl->recsize;
+  if (std::fseek(sl->fp, (long)off, SEEK_SET) != 0 || std::fwrite(r.b.data(), 1, r.b.size(), sl->fp) != r.b.size())
+    return iofail(CGFD_E_ARG, "slice write failed");
+  sl->nrec++;
+  return CGFD_OK;
+}
+
+int cgfd_slice_close(cgfd_slice *sl) {
+  if (!sl) return iofail(CGFD_E_ARG, "null slice");
+  BE n;
+  n.u32(sl->nrec);
+  bool ok = std::fseek(sl->fp, 4, SEEK_SET) == 0 && std::fwrite(n.b.data(), 1, 4, sl->fp) == 4;
+  ok = (std::fclose(sl->fp) == 0) && ok;
+  delete sl;
+  return ok ? CGFD_OK : iofail(CGFD_E_ARG, "slice close failed");
+}
+
+int cgfd_line_sac(const char *dir, const char *line, int seq, const float *seismo, int nt, float dt, float x,
+                  float y, float z) {
+  if (!dir || !line || !seismo || nt < 1 || seq < 0) return iofail(CGFD_E_ARG, "cgfd_line_sac: bad arguments");
+  char name[512];
+  std::snprintf(name, sizeof(name), "%s.no%d", line, seq);
+  return cgfd_station_sac(dir, name, seismo, nt, dt, x, y, z, 0, 0.0f, 0.0f);
+}
 
 int cgfd_sac_write(const char *path, const float *data, int npts, float delta, float b, const float *ev,
                    const float *st, const char *kstnm, const char *kcmpnm) {

@@ -1257,6 +1257,36 @@ int cgfd_receiver_index(int ni, int nj, int nk, double rx, double ry, double rz,
   return CGFD_OK;
 }
 
+// ---- receiver lines (io_line_locate, io_funcs.cu:302-417) ----------------------------
+// Points start + ipt*incre (0-based global physical indices), ipt < count,
+// those inside the grid kept in line order.  Each becomes an on-node
+// receiver: fractions 0, so the trilinear weights give the node value
+// exactly, the value io_line_keep copies (io_funcs.cu:1669-1700).
+int cgfd_line_points(int ni, int nj, int nk, const double *x, const double *y, const double *z, const int *start,
+                     const int *incre, int count, int *seq, long long *indx8, double *frac3, double *xyz) {
+  if (!start || !incre || count < 0 || !seq || !indx8 || !frac3) return CGFD_E_ARG;
+  G3 g(ni, nj, nk);
+  int n = 0;
+  for (int ipt = 0; ipt < count; ipt++) {
+    const long gi = start[0] + (long)ipt * incre[0], gj = start[1] + (long)ipt * incre[1],
+               gk = start[2] + (long)ipt * incre[2];
+    if (gi < 0 || gi > ni - 1 || gj < 0 || gj > nj - 1 || gk < 0 || gk > nk - 1) continue;
+    const long p = g.at((int)gi + NG, (int)gj + NG, (int)gk + NG);
+    long long *ix = indx8 + 8L * n;
+    ix[0] = p; ix[1] = p + 1; ix[2] = p + g.sy; ix[3] = p + 1 + g.sy;
+    ix[4] = p + g.sz; ix[5] = p + 1 + g.sz; ix[6] = p + g.sy + g.sz; ix[7] = p + 1 + g.sy + g.sz;
+    frac3[3L * n] = frac3[3L * n + 1] = frac3[3L * n + 2] = 0.0;
+    seq[n] = ipt;
+    if (xyz && x && y && z) {
+      xyz[3L * n] = x[p];
+      xyz[3L * n + 1] = y[p];
+      xyz[3L * n + 2] = z[p];
+    }
+    n++;
+  }
+  return n;
+}
+
 // ---- points given by coordinates ---------------------------------------------------
 // Restatement of the reference's coordinate location, in float as there
 // (its grid arrays are float): depth below the free surface -> z on the

@@ -57,6 +57,37 @@ int cgfd_snap_append(cgfd_snap *sn, float time, const float *w, const float *lam
 /* Writes the record count and closes the file. */
 int cgfd_snap_close(cgfd_snap *sn);
 
+/* Slice file (io_slice_locate / io_slice_nc_create / io_slice_nc_put,
+ * io_funcs.cu:418-784): one plane of the nine components per record, named
+ * by the caller slicex_i{gi+1}_{part}.nc, slicey_j{gj+1}_{part}.nc or
+ * slicez_k{gk+1}_{part}.nc (gi: 0-based global physical index).  NetCDF
+ * classic; dimensions time (unlimited) and, by axis, (k, j) for an x slice,
+ * (k, i) for y, (j, i) for z; float variables time and Vx Vy Vz Txx Tyy Tzz
+ * Tyz Txz Txy over [time][slow][fast]; global int attributes
+ * {i|j|k}_index_with_ghosts_in_this_thread (the plane's local index, ghosts
+ * counted: gi + 3 on one subdomain) and coords_of_mpi_topo (3). */
+typedef struct cgfd_slice cgfd_slice;
+
+/* axis 0 (x), 1 (y) or 2 (z); n_slow / n_fast: the two in-plane sizes in
+ * file order (x: nk, nj; y: nk, ni; z: nj, ni). */
+int cgfd_slice_open(const char *path, int axis, int index_with_ghosts, int n_slow, int n_fast, const int *topo,
+                    cgfd_slice **out);
+
+/* Append one record: w holds the nine components in the order above, each
+ * [n_slow][n_fast] (a plane from cgfd_fetch_subvolume with count 1 on the
+ * slice axis). */
+int cgfd_slice_append(cgfd_slice *sl, float time, const float *w);
+
+/* Writes the record count and closes the file. */
+int cgfd_slice_close(cgfd_slice *sl);
+
+/* The SAC files of one point of a receiver line as io_line_output_sac writes
+ * them (io_funcs.cu:1941-1980): {dir}/{line}.no{seq}.{Vx..Txy}.sac from
+ * seismo [9][nt] (seq = the point's number along the line), event location
+ * 0, station (x, y, z), the station depth field carrying dt as there. */
+int cgfd_line_sac(const char *dir, const char *line, int seq, const float *seismo, int nt, float dt, float x,
+                  float y, float z);
+
 /* Peak ground motion file PG_V_A_D_{coords}.nc (PG_slice_output,
  * io_funcs.cu:2065-2125): dimensions j = ny, i = nx (with ghosts), float
  * variables PGV PGVh PGVx PGVy PGVz PGA PGAh PGAx PGAy PGAz PGD PGDh PGDx PGDy

@@ -211,6 +211,16 @@ int cgfd_source_point(int ni, int nj, int nk, double sx, double sy, double sz, i
 int cgfd_locate_points(int ni, int nj, int nk, const double *x, const double *y, const double *z, int n,
                        const double *pts, const int *depth_flag, int *idx, double *frac, int *found);
 
+/* Receiver line (io_line_locate, io_funcs.cu:302-417): the points start +
+ * ipt*incre, ipt < count, in 0-based global physical indices; those inside
+ * the grid are kept in line order with their number ipt in seq, as on-node
+ * receivers (8 corner offsets in indx8, fractions 0 in frac3: recording
+ * returns the node value, as io_line_keep does) and, when the grid arrays
+ * are given, their coordinates in xyz (3 per point; may be NULL).  Arrays
+ * hold count entries at most.  Returns the number kept, or CGFD_E_ARG. */
+int cgfd_line_points(int ni, int nj, int nk, const double *x, const double *y, const double *z, const int *start,
+                     const int *incre, int count, int *seq, long long *indx8, double *frac3, double *xyz);
+
 /* Receiver given by grid index as in a station file (io_funcs.cu:162-251):
  * x,y 1-based; z depth below the surface (depth_flag 1) or 1-based index.
  * Fills the 8 trilinear corner indices and the fractions; returns

@@ -34,6 +34,7 @@
  *   src/forward/io_funcs.cu:24-299,1624-1666        receivers
  *   src/forward/blk_t.cu:673-763                    dt estimate
  *   src/forward/md_t.cu:622-637                     rho -> 1/rho
+ *   src/forward/wav_t.cu:281-360                    peak ground motion maps
  */
 #include <cmath>
 #include <cstdint>
@@ -1164,6 +1165,41 @@ void orc_rho_to_slow(size_t n, const double *rho, double *slw) {
     float r = (float)rho[i];
     slw[i] = ((double)r > 1e-10) ? (double)(float)(1.0 / r) : 0.0;
   }
+}
+
+/*
+ * Peak ground motion maps (PG_calcu_gpu, src/forward/wav_t.cu:281-360, called
+ * after every step at drv_rk_curv_col.cu:519-527) from the surface velocity
+ * history vtop[nt+1][3][npts] (W^0 .. W^nt on the top plane): per step n the
+ * acceleration |V^{n+1} - V^n| / dt, the displacement accumulated as
+ * D += 0.5 (V^{n+1} + V^n) dt (the 0.5 literal makes it a double sum, stored
+ * to float), and the running maxima of |V|, |V_h|, |V_x..z|, the same for A
+ * and D.  pg[15][npts]: PGV PGVh PGVx PGVy PGVz PGA PGAh PGAx PGAy PGAz PGD
+ * PGDh PGDx PGDy PGDz, float arithmetic as the reference.
+ */
+void orc_peak_motion(int npts, int nt, const double *vtop, double dt_, double *pg) {
+  const float dt = (float)dt_;
+  std::vector<float> P((size_t)15 * npts, 0.0f), D((size_t)3 * npts, 0.0f);
+  for (int n = 0; n < nt; n++) {
+    const double *v0 = vtop + (size_t)n * 3 * npts, *v1 = vtop + (size_t)(n + 1) * 3 * npts;
+    for (int q = 0; q < npts; q++) {
+      const float x1 = (float)v1[q], y1 = (float)v1[npts + q], z1 = (float)v1[2 * npts + q];
+      const float x0 = (float)v0[q], y0 = (float)v0[npts + q], z0 = (float)v0[2 * npts + q];
+      const float Ax = std::fabs((x1 - x0) / dt), Ay = std::fabs((y1 - y0) / dt), Az = std::fabs((z1 - z0) / dt);
+      float &Dx = D[q], &Dy = D[npts + q], &Dz = D[2 * (size_t)npts + q];
+      Dx = (float)((double)Dx + 0.5 * (double)(x1 + x0) * (double)dt);
+      Dy = (float)((double)Dy + 0.5 * (double)(y1 + y0) * (double)dt);
+      Dz = (float)((double)Dz + 0.5 * (double)(z1 + z0) * (double)dt);
+      const float V = std::sqrt(x1 * x1 + y1 * y1 + z1 * z1), Vh = std::sqrt(x1 * x1 + y1 * y1);
+      const float A = std::sqrt(Ax * Ax + Ay * Ay + Az * Az), Ah = std::sqrt(Ax * Ax + Ay * Ay);
+      const float Dm = std::sqrt(Dx * Dx + Dy * Dy + Dz * Dz), Dh = std::sqrt(Dx * Dx + Dy * Dy);
+      const float val[15] = {V, Vh, std::fabs(x1), std::fabs(y1), std::fabs(z1), A, Ah, Ax, Ay, Az,
+                             Dm, Dh, std::fabs(Dx), std::fabs(Dy), std::fabs(Dz)};
+      for (int m = 0; m < 15; m++)
+        if (P[(size_t)m * npts + q] < val[m]) P[(size_t)m * npts + q] = val[m];
+    }
+  }
+  for (size_t q = 0; q < P.size(); q++) pg[q] = P[q];
 }
 
 }  // extern "C"

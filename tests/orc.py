@@ -65,6 +65,7 @@ def lib():
         L.orc_dt_esti.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, d]
         L.orc_dt_esti.restype = d
         L.orc_rho_to_slow.argtypes = [C.c_size_t, vp, vp]
+        L.orc_peak_motion.argtypes = [i, i, vp, d, vp]
         L.orc_set_threads.argtypes = [i]
         L.orc_max_threads.restype = i
         L.orc_last_loop_seconds.restype = d
@@ -80,8 +81,53 @@ def f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def check_problem(ni, nj, nk, metric, lam, mu, slw, mvx, mvy, pml, src, recv_indx, recv_frac):
+    """Every array the oracle will index, checked against the grid before the
+    ctypes call: a short array or an index past the grid raises here instead
+    of letting the C code read or write past a buffer."""
+    n = (ni + 6) * (nj + 6) * (nk + 6)
+
+    def size(a, want, what):
+        got = np.asarray(a).size
+        if got != want:
+            raise ValueError(f"oracle input {what}: {got} values, the grid needs {want}")
+
+    size(metric, 10 * n, "metric")
+    for a, nm in ((lam, "lam"), (mu, "mu"), (slw, "slw")):
+        size(a, n, nm)
+    for a, nm in ((mvx, "matVx2Vz"), (mvy, "matVy2Vz")):
+        size(a, 9 * (ni + 6) * (nj + 6), nm)
+    for f, (nl, A, B, D) in pml.items():
+        for a, nm in ((A, "A"), (B, "B"), (D, "D")):
+            size(a, nl + 1, f"PML face {f} {nm}")
+    if src is not None and src["nsrc"] > 0:
+        ns, me, mnt = src["nsrc"], src["max_ext"], src["max_nt"]
+        for k in ("it_begin", "it_end", "ext_num"):
+            size(src[k], ns, f"source {k}")
+        size(src["ext_indx"], ns * me, "source ext_indx")
+        size(src["ext_coef"], ns * me, "source ext_coef")
+        if src.get("M") is not None:
+            size(src["M"], 6 * ns * mnt * 4, "source M")
+        if src.get("F") is not None:
+            size(src["F"], 3 * ns * mnt * 4, "source F")
+        ix = np.asarray(src["ext_indx"]).reshape(ns, me)
+        for s_ in range(ns):
+            e = ix[s_, :int(np.asarray(src["ext_num"])[s_])]
+            if e.size and (e.min() < 0 or e.max() >= n):
+                raise ValueError("oracle input: source point outside the grid")
+    nr = len(recv_indx)
+    if nr:
+        size(recv_indx, 8 * nr, "receiver indices")
+        size(recv_frac, 3 * nr, "receiver fractions")
+        r = np.asarray(recv_indx)
+        if r.min() < 0 or r.max() >= n:
+            raise ValueError("oracle input: receiver corner outside the grid")
+
+
 def problem_struct(m):
     """OrcProblem for a workload dict of gpu-cgfd3d-elastic_amd/workloads.py (keeps its arrays alive)."""
+    check_problem(m["ni"], m["nj"], m["nk"], m["metric"], m["lam"], m["mu"], m["slw"], m["mvx"], m["mvy"],
+                  m["pml"], m["src"], m["recv_indx"], m["recv_frac"])
     keep = []
 
     def k(a, dt=np.float64):

@@ -156,6 +156,8 @@ class Problem:
 
     # ---------------------------------------------------------------- oracle
     def orc_struct(self):
+        orc.check_problem(self.ni, self.nj, self.nk, self.metric, self.lam, self.mu, self.slw, self.mvx, self.mvy,
+                          self.pml, self.src, self.recv_indx, self.recv_frac)
         keep = []
 
         def k(a, dt=np.float64):
@@ -202,6 +204,11 @@ class Problem:
     def oracle_run(self, precision=32, nt_total=None, want_aux=False, init_aux=None):
         nt = self.nt_total if nt_total is None else nt_total
         P = self.orc_struct()
+        lev = 9 * self.nx * self.ny * self.nz
+        if self.init is not None and np.asarray(self.init).size != lev:
+            raise ValueError("initial wavefield size does not match the grid")
+        if init_aux is not None and np.asarray(init_aux).size != 9 * int(self.face_sizes().sum()):
+            raise ValueError("initial PML memory size does not match the face boxes")
         seis = np.zeros((max(len(self.recv_indx), 1), 9, nt))
         wave = np.zeros((9, self.nz, self.ny, self.nx))
         aux = np.zeros(9 * int(self.face_sizes().sum())) if want_aux else None
@@ -214,6 +221,12 @@ class Problem:
 
     def oracle_stage(self, w, ipair, istage, it, aux=None, precision=32):
         P = self.orc_struct()
+        if np.asarray(w).size != 9 * self.nx * self.ny * self.nz:
+            raise ValueError("stage input size does not match the grid")
+        if aux is not None and np.asarray(aux).size != 9 * int(self.face_sizes().sum()):
+            raise ValueError("PML memory size does not match the face boxes")
+        if not (0 <= ipair < 8 and 0 <= istage < 4):
+            raise ValueError("pair 0..7 and stage 0..3")
         rhs = np.zeros((9, self.nz, self.ny, self.nx))
         aux_rhs = np.zeros(9 * int(self.face_sizes().sum()))
         w = np.ascontiguousarray(w, dtype=np.float64)

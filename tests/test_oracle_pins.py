@@ -343,3 +343,51 @@ def test_single_force_matches_aki_richards():
         e = rel_l2(seis[r, :3], want[r])
         assert e < 0.01, (st[r], e)         # measured <= 0.22%
         assert rel_l2(-seis[r, :3], want[r]) > 1.5
+
+
+# ---------------------------------------------------------------- peak ground motion
+def _peak(vtop, dt):
+    nt = vtop.shape[0] - 1
+    npts = vtop.shape[2]
+    pg = np.zeros(15 * npts)
+    L.orc_peak_motion(npts, nt, orc.p(np.ascontiguousarray(vtop, dtype=np.float64)), dt, orc.p(pg))
+    return pg.reshape(15, npts)
+
+
+def test_peak_motion_linear_velocity_closed_form():
+    """V(t) = a + b t on every point: |dV|/dt = |b| exactly, the trapezoid rule
+    integrates a linear history exactly (D = a t + b t^2 / 2), and the maxima
+    are those of the sampled values (wav_t.cu:281-360)."""
+    rng = np.random.default_rng(13)
+    npts, nt, dt = 7, 40, 0.01
+    a = rng.normal(size=(3, npts))
+    b = rng.normal(size=(3, npts)) * 20
+    b[:, 0] = 0.0                                   # a constant velocity: zero acceleration
+    t = np.arange(nt + 1) * np.float32(dt)
+    vtop = np.float32(a[None] + b[None] * t[:, None, None])
+    pg = _peak(vtop, dt)
+    V = np.float64(vtop[1:])                         # maxima over V^1 .. V^nt (after each step)
+    D = a[None] * t[1:, None, None] + 0.5 * b[None] * t[1:, None, None] ** 2
+    want = [np.sqrt((V ** 2).sum(1)).max(0), np.sqrt((V[:, :2] ** 2).sum(1)).max(0), *np.abs(V).max(0),
+            np.full(npts, 0.0) + np.sqrt((b ** 2).sum(0)), np.sqrt((b[:2] ** 2).sum(0)), *np.abs(b),
+            np.sqrt((D ** 2).sum(1)).max(0), np.sqrt((D[:, :2] ** 2).sum(1)).max(0), *np.abs(D).max(0)]
+    want = np.array(want)
+    # float arithmetic: differences of V in float give |dV|/dt to ~1e-5 relative
+    assert np.allclose(pg, want, rtol=2e-4, atol=1e-5)
+    assert np.all(pg[5:10, 0] < 1e-4)                # the constant column has no acceleration
+
+
+def test_peak_motion_keeps_running_maxima():
+    """A velocity pulse that rises and falls: the maps keep the peak, not the last value."""
+    npts, nt, dt = 3, 60, 0.005
+    t = np.arange(nt + 1) * dt
+    pulse = np.exp(-((t - 0.15) / 0.03) ** 2)
+    vtop = np.zeros((nt + 1, 3, npts))
+    vtop[:, 2, :] = pulse[:, None] * np.array([1.0, -2.0, 0.5])
+    pg = _peak(vtop, dt)
+    assert np.allclose(pg[4], np.abs(vtop[1:, 2]).max(0), rtol=1e-6)   # PGVz
+    assert np.allclose(pg[0], pg[4], rtol=1e-6)                        # |V| = |Vz|
+    assert np.all(pg[1] == 0) and np.all(pg[2] == 0)                   # no horizontal motion
+    # the displacement of a pulse that ends near zero velocity: its integral, the area of the Gaussian
+    area = 0.03 * np.sqrt(np.pi) * np.array([1.0, 2.0, 0.5])
+    assert np.allclose(pg[14], area, rtol=2e-3)

@@ -121,3 +121,80 @@ def test_peak_motion_file(tmp_path):
         assert f.variables[n].dimensions == ("j", "i")
         assert np.array_equal(f.variables[n][:], pg[q])
     f.close()
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_slice_netcdf_layout(tmp_path, axis):
+    ni, nj, nk = 7, 5, 4
+    gindex = 2
+    path = cgfd.slice_path(tmp_path, axis, gindex)
+    # file name carries the 1-based plane number (io_funcs.cu:466-468)
+    assert path.endswith(f"slice{'xyz'[axis]}_{'ijk'[axis]}3_px0_py0_pz0.nc")
+    sl = cgfd.Slice(path, axis, gindex, ni, nj, nk, topo=(1, 0, 2))
+    rng = np.random.default_rng(10 + axis)
+    frames = [rng.normal(size=(9,) + sl.shape).astype(np.float32) for _ in range(3)]
+    for t, w in enumerate(frames):
+        sl.append(0.25 * (t + 1), w)
+    sl.close()
+    f = netcdf_file(path, "r", mmap=False)
+    dims = [("k", "j"), ("k", "i"), ("j", "i")][axis]
+    sizes = {"i": ni, "j": nj, "k": nk}
+    assert f.dimensions["time"] is None
+    assert {d: f.dimensions[d] for d in dims} == {d: sizes[d] for d in dims}
+    assert "ijk"[axis] not in f.dimensions
+    attr = f"{'ijk'[axis]}_index_with_ghosts_in_this_thread"
+    assert list(np.atleast_1d(f._attributes[attr])) == [gindex + 3]
+    assert list(f._attributes["coords_of_mpi_topo"]) == [1, 0, 2]
+    assert np.array_equal(f.variables["time"][:], np.float32([0.25, 0.5, 0.75]))
+    names = ["Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Tyz", "Txz", "Txy"]
+    assert list(f.variables) == ["time"] + names
+    for c, n in enumerate(names):
+        v = f.variables[n]
+        assert v.dimensions == ("time",) + dims and v.typecode() == "f"
+        for t, w in enumerate(frames):
+            assert np.array_equal(v[t], w[c])
+    f.close()
+
+
+def test_slice_rejects_bad_plane(tmp_path):
+    with pytest.raises(ValueError):
+        cgfd.Slice(tmp_path / "s.nc", 3, 0, 4, 4, 4)
+    with pytest.raises(ValueError):
+        cgfd.Slice(tmp_path / "s.nc", 1, 4, 4, 4, 4)
+    with pytest.raises(cgfd.CgfdError):
+        cgfd.Slice(tmp_path / "missing_dir" / "s.nc", 2, 0, 4, 4, 4)
+
+
+def test_line_points_follow_the_reference_walk():
+    # io_line_locate (io_funcs.cu:302-416): point ipt at start + ipt*incre,
+    # kept when inside the physical grid; seq is ipt, the value is the node's
+    ni, nj, nk = 9, 6, 5
+    g = (ni + 6, nj + 6, nk + 6)
+    x, y, z = np.meshgrid(np.arange(g[0]) * 10.0, np.arange(g[1]) * 20.0, np.arange(g[2]) * 30.0, indexing="ij")
+    x, y, z = (np.ascontiguousarray(a.transpose(2, 1, 0)).ravel() for a in (x, y, z))
+    seq, ix, fr, xyz = cgfd.line_points(ni, nj, nk, (-2, 1, 4), (2, 1, -1), 8, x, y, z)
+    want = [ipt for ipt in range(8)
+            if 0 <= -2 + 2 * ipt < ni and 0 <= 1 + ipt < nj and 0 <= 4 - ipt < nk]
+    assert list(seq) == want == [1, 2, 3, 4]
+    sy, sz = g[0], g[0] * g[1]
+    for n, ipt in enumerate(seq):
+        i, j, k = -2 + 2 * ipt + 3, 1 + ipt + 3, 4 - ipt + 3
+        p = i + j * sy + k * sz
+        assert ix[n, 0] == p and list(ix[n]) == [p, p + 1, p + sy, p + 1 + sy, p + sz, p + 1 + sz,
+                                                 p + sy + sz, p + 1 + sy + sz]
+        assert list(fr[n]) == [0.0, 0.0, 0.0]
+        assert list(xyz[n]) == [10.0 * i, 20.0 * j, 30.0 * k]
+    s2, _, _, _ = cgfd.line_points(ni, nj, nk, (ni, 0, 0), (1, 0, 0), 3)
+    assert s2.size == 0
+
+
+def test_line_sac_names(tmp_path):
+    nt = 12
+    seis = np.arange(9 * nt, dtype=np.float32).reshape(9, nt)
+    cgfd.line_sac(tmp_path, "line_a", 7, seis, 0.01, 1.0, 2.0, 3.0)
+    names = ["Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Tyz", "Txz", "Txy"]
+    for c, n in enumerate(names):
+        # io_line_output_sac (io_funcs.cu:1966-1968): {dir}/{line}.no{seq}.{cmp}.sac
+        hf, hi, _, data = _sac(tmp_path / f"line_a.no7.{n}.sac")
+        assert np.array_equal(data, seis[c]) and hi[9] == nt
+        assert list(hf[31:35]) == [1.0, 2.0, 3.0, np.float32(0.01)]
