@@ -31,7 +31,7 @@ EXPORTS = [
     "cgfd_el_iso_onestage", "cgfd_read_seismo", "cgfd_fetch_level",
     "cgfd_fetch_pml_aux", "cgfd_set_timing", "cgfd_last_run_timing",
     "cgfd_set_tile", "cgfd_get_tile", "cgfd_get_split", "cgfd_get_const_metric", "cgfd_set_graph", "cgfd_set_shot",
-    "cgfd_run_async", "cgfd_poll", "cgfd_wait", "cgfd_reserve_receivers",
+    "cgfd_run_async", "cgfd_poll", "cgfd_wait", "cgfd_reserve_receivers", "cgfd_set_seismo_ring",
     "cgfd_grid_cartesian", "cgfd_grid_topography", "cgfd_metric_calc", "cgfd_dt_estimate",
     "cgfd_media_layered", "cgfd_media_homogeneous", "cgfd_rho_to_slow", "cgfd_pml_len_dh",
     "cgfd_pml_profile", "cgfd_free_surface_matrices", "cgfd_wavelet", "cgfd_angle2moment",
@@ -161,6 +161,7 @@ def load(path=LIB_PATH):
         "cgfd_run_async": ([vp, i, i, C.c_double], i),
         "cgfd_poll": ([vp], i),
         "cgfd_reserve_receivers": ([vp, i, i], i),
+        "cgfd_set_seismo_ring": ([vp, i], i),
         "cgfd_wait": ([vp], i),
         "cgfd_set_tile": ([vp, i, i], i),
         "cgfd_get_tile": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)], i),
@@ -406,6 +407,11 @@ class Session:
 
     def reserve_receivers(self, max_nrecv, max_nt):
         _check(_lib.cgfd_reserve_receivers(self.h, max_nrecv, max_nt))
+
+    def set_seismo_ring(self, ring_steps):
+        """Keep ring_steps samples per trace on the device, streaming chunks
+        of ring_steps/2 to a pinned host record (0: whole record on device)."""
+        _check(_lib.cgfd_set_seismo_ring(self.h, int(ring_steps)))
 
     def set_graph(self, on):
         _check(_lib.cgfd_set_graph(self.h, 1 if on else 0))

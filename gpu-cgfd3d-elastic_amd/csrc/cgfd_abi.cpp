@@ -179,6 +179,19 @@ struct cgfd_session {
   long seismo_stride = 0;        // elements reserved per shot
   size_t r_cap[5] = {0};
   bool rec_dirty = false;
+  // Seismogram ring (ring > 0): the device keeps `ring` samples per trace;
+  // runs are cut into chunks of ring/2 steps and after each chunk its
+  // samples are copied on st_copy into the pinned host record (per shot
+  // [nrecv][9][nt] at host_off), overlapping the next chunk.  A chunk waits
+  // for the copy of the chunk two back, whose ring columns it reuses.
+  int ring = 0;
+  hipStream_t st_copy = nullptr;
+  hipEvent_t ev_chunk = nullptr;
+  hipEvent_t ev_copied[2] = {nullptr, nullptr};
+  long copied_chunk[2] = {-1, -1};
+  void *host_seismo = nullptr;  // hipHostMalloc, nshots * host_stride samples
+  long host_stride = 0;
+  std::vector<int> start_h, active_h;  // host copies of shot_start / shot_active
   int *clock = nullptr, *shot_start = nullptr, *shot_active = nullptr;
   // interior tile choice (-1: not tuned yet)
   int tile_shape = -1, tile_zchunk = 48;
@@ -380,6 +393,64 @@ int fill_zero(cgfd_session *s, void *p, size_t elems, size_t esz) {
   return CGFD_OK;
 }
 
+// ---- seismogram ring: pinned host record and per-chunk copies ----
+// The host record grows like the device region, keeps the samples of shots
+// whose receivers did not change and zeroes the others; pending copies into
+// it are drained first.
+int host_record(cgfd_session *s, long hstride) {
+  if (s->st_copy) HIPCHK(hipStreamSynchronize(s->st_copy));
+  const int ns = s->d.nshots;
+  const size_t ms = s->ms;
+  if (hstride > s->host_stride || !s->host_seismo) {
+    const long nst = std::max(hstride, s->host_stride);
+    void *p = nullptr;
+    if (nst > 0) {
+      hipError_t e = hipHostMalloc(&p, (size_t)ns * nst * ms, hipHostMallocDefault);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(CGFD_E_NOMEM, "pinned seismogram record of %zu bytes: %s", (size_t)ns * nst * ms,
+                    hipGetErrorString(e));
+      }
+      std::memset(p, 0, (size_t)ns * nst * ms);
+      if (s->host_seismo)
+        for (int shot = 0; shot < ns; shot++)
+          if (!s->rec_changed[shot])
+            std::memcpy((char *)p + (size_t)shot * nst * ms, (const char *)s->host_seismo + (size_t)shot * s->host_stride * ms,
+                        (size_t)s->host_stride * ms);
+    }
+    if (s->host_seismo) (void)hipHostFree(s->host_seismo);
+    s->host_seismo = p;
+    s->host_stride = nst;
+  } else {
+    for (int shot = 0; shot < ns; shot++)
+      if (s->rec_changed[shot])
+        std::memset((char *)s->host_seismo + (size_t)shot * s->host_stride * ms, 0, (size_t)s->host_stride * ms);
+  }
+  return CGFD_OK;
+}
+
+// Copy the samples of steps [a, b) of every active shot from the device
+// ring to the host record on st_copy: per shot one strided copy of its
+// nrecv*9 traces (two where the window wraps round the ring).
+int copy_window(cgfd_session *s, int a, int b) {
+  const size_t ms = s->ms;
+  for (int shot = 0; shot < s->d.nshots; shot++) {
+    const RecvHost &h = s->rec[shot];
+    if (h.nrecv == 0 || !s->active_h[shot]) continue;
+    long t0 = std::max<long>((long)a - s->start_h[shot], 0);
+    const long t1 = std::min<long>((long)b - s->start_h[shot], h.nt);
+    while (t0 < t1) {
+      const long col = t0 % s->ring, len = std::min<long>(t1 - t0, s->ring - col);
+      const char *src = (const char *)s->seismo + ((size_t)s->seismo_off[shot] + col) * ms;
+      char *dst = (char *)s->host_seismo + ((size_t)shot * s->host_stride + t0) * ms;
+      HIPCHK(hipMemcpy2DAsync(dst, (size_t)h.nt * ms, src, (size_t)s->ring * ms, (size_t)len * ms,
+                              (size_t)h.nrecv * kNC, hipMemcpyDeviceToHost, s->st_copy));
+      t0 += len;
+    }
+  }
+  return CGFD_OK;
+}
+
 // ---- rebuild device source CSR over all shots ----
 template <typename TM> int build_sources(cgfd_session *s) {
   s->s_npt = 0;
@@ -460,8 +531,14 @@ template <typename TC> int build_receivers(cgfd_session *s) {
   std::vector<int> shot_of, nt_of;
   std::vector<long> off, out;
   std::vector<TC> frac;
-  long stride = 0;
-  for (const RecvHost &h : s->rec) stride = std::max(stride, (long)h.nrecv * kNC * h.nt);
+  long stride = 0, hstride = 0;
+  auto pitch = [&](const RecvHost &h) { return (long)(s->ring > 0 ? s->ring : h.nt); };
+  for (const RecvHost &h : s->rec) {
+    stride = std::max(stride, (long)h.nrecv * kNC * pitch(h));
+    hstride = std::max(hstride, (long)h.nrecv * kNC * h.nt);
+  }
+  int r;
+  if (s->ring > 0 && (r = host_record(s, hstride))) return r;
   const bool grow = stride > s->seismo_stride || !s->seismo;
   const long new_stride = grow ? std::max(stride, s->seismo_stride) : s->seismo_stride;
   for (int shot = 0; shot < (int)s->rec.size(); shot++) {
@@ -473,12 +550,11 @@ template <typename TC> int build_receivers(cgfd_session *s) {
       for (int q = 0; q < 8; q++) off.push_back(ref_to_dev(s->L, h.indx8[(size_t)r * 8 + q]));
       for (int q = 0; q < 3; q++) frac.push_back((TC)h.frac[(size_t)r * 3 + q]);
       out.push_back(pos);
-      pos += (long)kNC * h.nt;
+      pos += (long)kNC * pitch(h);
     }
   }
   s->r_n = (int)shot_of.size();
   invalidate(s);
-  int r;
   if (new_stride > 0 && grow) {
     void *old = s->seismo;
     const long old_stride = s->seismo_stride;
@@ -487,7 +563,7 @@ template <typename TC> int build_receivers(cgfd_session *s) {
     if ((r = fill_zero(s, s->seismo, (size_t)s->d.nshots * new_stride, sizeof(TC)))) return r;
     if (old) {
       for (int shot = 0; shot < (int)s->rec.size(); shot++) {
-        const size_t n = (size_t)s->rec[shot].nrecv * kNC * s->rec[shot].nt;
+        const size_t n = (size_t)s->rec[shot].nrecv * kNC * pitch(s->rec[shot]);
         if (s->rec_changed[shot] || n == 0) continue;
         HIPCHK(hipMemcpyAsync((TC *)s->seismo + (size_t)shot * new_stride, (const TC *)old + (size_t)shot * old_stride,
                               n * sizeof(TC), hipMemcpyDeviceToDevice, s->st));
@@ -818,7 +894,7 @@ int enqueue_steps(cgfd_session *s, const Shell &sh, int it0, int nsteps, TC dt, 
     if (s->r_n > 0) {
       RecvArgs<TS, TC> ra{};
       ra.nrec = s->r_n; ra.r_shot = s->r_shot; ra.r_off = s->r_off; ra.r_frac = (const TC *)s->r_frac;
-      ra.r_out = s->r_out; ra.r_nt = s->r_nt; ra.seismo = (TC *)s->seismo;
+      ra.r_out = s->r_out; ra.r_nt = s->r_nt; ra.seismo = (TC *)s->seismo; ra.ring = s->ring;
       ra.w_pre = (const TS *)s->wav + (long)s->slot[0] * s->level_elems;
       ra.shot_stride = s->shot_elems; ra.cs = s->L.cs;
       ra.clock = s->clock; ra.shot_start = s->shot_start; ra.shot_active = s->shot_active;
@@ -921,21 +997,44 @@ template <typename TS, typename TC> int enqueue_t(cgfd_session *s, int it0, int 
   hipError_t e = launch_fill<int>(s->clock, 1, it0, s->st);
   if (e != hipSuccess) return fail(CGFD_E_HIP, "clock set: %s", hipGetErrorString(e));
   s->ev_used = 0;
-  int it = it0, left = nsteps;
   const bool graphs = s->use_graph && !s->timing;
-  while (left > 0) {
-    if (graphs && it % 8 == 0 && left >= 8) {
-      hipGraphExec_t g = nullptr;
-      int r = period_graph<TS, TC>(s, sh, dt, dt_in, &g);
-      if (r) return r;
-      HIPCHK(hipGraphLaunch(g, s->st));
-      it += 8; left -= 8;
-    } else {
-      const int n = graphs ? std::min(left, 8 - it % 8) : left;
-      int r = enqueue_steps<TS, TC>(s, sh, it, n, dt, s->timing);
-      if (r) return r;
-      it += n; left -= n;
+  auto steps = [&](int it, int left) -> int {
+    while (left > 0) {
+      if (graphs && it % 8 == 0 && left >= 8) {
+        hipGraphExec_t g = nullptr;
+        int r = period_graph<TS, TC>(s, sh, dt, dt_in, &g);
+        if (r) return r;
+        HIPCHK(hipGraphLaunch(g, s->st));
+        it += 8; left -= 8;
+      } else {
+        const int n = graphs ? std::min(left, 8 - it % 8) : left;
+        int r = enqueue_steps<TS, TC>(s, sh, it, n, dt, s->timing);
+        if (r) return r;
+        it += n; left -= n;
+      }
     }
+    return CGFD_OK;
+  };
+  if (s->ring <= 0 || s->r_n == 0) return steps(it0, nsteps);
+  // ring: chunks of ring/2 steps at multiples of ring/2 (whole 8-step
+  // periods, as ring is a multiple of 16); chunk q reuses the ring columns
+  // of chunk q-2 and so waits for that chunk's copy
+  const int C = s->ring / 2;
+  int it = it0, left = nsteps;
+  while (left > 0) {
+    const long q = it / C;
+    const int n = (int)std::min<long>(left, (q + 1) * C - it);
+    const int par = (int)(q & 1);
+    if (s->copied_chunk[par] >= 0 && s->copied_chunk[par] != q)
+      HIPCHK(hipStreamWaitEvent(s->st, s->ev_copied[par], 0));
+    int r = steps(it, n);
+    if (r) return r;
+    HIPCHK(hipEventRecord(s->ev_chunk, s->st));
+    HIPCHK(hipStreamWaitEvent(s->st_copy, s->ev_chunk, 0));
+    if ((r = copy_window(s, it, it + n))) return r;
+    HIPCHK(hipEventRecord(s->ev_copied[par], s->st_copy));
+    s->copied_chunk[par] = q;
+    it += n; left -= n;
   }
   return CGFD_OK;
 }
@@ -947,6 +1046,7 @@ template <typename TS, typename TC> int run_t(cgfd_session *s, int it0, int nste
   if (r) { (void)hipStreamSynchronize(s->st); return r; }
   HIPCHK(hipEventRecord(s->ev_run1, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
+  if (s->st_copy) HIPCHK(hipStreamSynchronize(s->st_copy));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, s->ev_run0, s->ev_run1));
   s->last_run_ms = ms;
@@ -1096,6 +1196,8 @@ int cgfd_create(const cgfd_desc *d, cgfd_session **out) {
   s->src.resize(d->nshots);
   s->rec.resize(d->nshots);
   s->rec_changed.assign(d->nshots, 0);
+  s->start_h.assign(d->nshots, 0);
+  s->active_h.assign(d->nshots, 1);
   if (hipStreamSynchronize(s->st) != hipSuccess) return cleanup(fail(CGFD_E_HIP, "stream sync failed"));
   *out = s;
   return CGFD_OK;
@@ -1104,12 +1206,18 @@ int cgfd_create(const cgfd_desc *d, cgfd_session **out) {
 void cgfd_destroy(cgfd_session *s) {
   if (!s) return;
   if (s->st) (void)hipStreamSynchronize(s->st);
+  if (s->st_copy) (void)hipStreamSynchronize(s->st_copy);
   drop_graphs(s);
   for (void *p : s->allocs) (void)hipFree(p);
   s->allocs.clear();
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
   if (s->ev_run0) (void)hipEventDestroy(s->ev_run0);
   if (s->ev_run1) (void)hipEventDestroy(s->ev_run1);
+  if (s->ev_chunk) (void)hipEventDestroy(s->ev_chunk);
+  for (hipEvent_t e : s->ev_copied)
+    if (e) (void)hipEventDestroy(e);
+  if (s->st_copy) (void)hipStreamDestroy(s->st_copy);
+  if (s->host_seismo) (void)hipHostFree(s->host_seismo);
   if (s->st) (void)hipStreamDestroy(s->st);
   delete s;
 }
@@ -1345,6 +1453,18 @@ int cgfd_read_seismo(cgfd_session *s, int shot, int dtype, void *out) {
   const RecvHost &h = s->rec[shot];
   const size_t n = (size_t)h.nrecv * kNC * h.nt;
   if (n == 0) return CGFD_OK;
+  if (s->ring > 0) {
+    // the host record is complete once the run and its copies are done
+    HIPCHK(hipStreamSynchronize(s->st));
+    HIPCHK(hipStreamSynchronize(s->st_copy));
+    const char *src = (const char *)s->host_seismo + (size_t)shot * s->host_stride * s->ms;
+    for (size_t q = 0; q < n; q++) {
+      const double x = s->ms == 8 ? ((const double *)src)[q] : (double)((const float *)src)[q];
+      if (dtype == CGFD_DT_F64) ((double *)out)[q] = x;
+      else ((float *)out)[q] = (float)x;
+    }
+    return CGFD_OK;
+  }
   const size_t off = (size_t)s->seismo_off[shot];
   if (s->ms == 8) {
     std::vector<double> v(n);
@@ -1632,7 +1752,11 @@ int cgfd_read_peak_motion(cgfd_session *s, int shot, int dtype, void *out) {
 
 int cgfd_reserve_receivers(cgfd_session *s, int max_nrecv, int max_nt) {
   if (!s || max_nrecv < 0 || max_nt < 0) return fail(CGFD_E_ARG, "bad reserve_receivers arguments");
-  const long want = (long)max_nrecv * kNC * max_nt;
+  const long want = (long)max_nrecv * kNC * (s->ring > 0 ? s->ring : max_nt);
+  if (s->ring > 0) {
+    int r = host_record(s, (long)max_nrecv * kNC * max_nt);
+    if (r) return r;
+  }
   if (want <= s->seismo_stride) return CGFD_OK;
   void *old = s->seismo;
   const long old_stride = s->seismo_stride;
@@ -1667,6 +1791,8 @@ int cgfd_set_shot(cgfd_session *s, int shot, int active, int it_start) {
   hipError_t e = launch_fill<int>(s->shot_active + shot, 1, active ? 1 : 0, s->st);
   if (e == hipSuccess && active) e = launch_fill<int>(s->shot_start + shot, 1, it_start, s->st);
   if (e != hipSuccess) return fail(CGFD_E_HIP, "shot state: %s", hipGetErrorString(e));
+  s->active_h[shot] = active ? 1 : 0;
+  if (active) s->start_h[shot] = it_start;
   return CGFD_OK;
 }
 
@@ -1689,6 +1815,7 @@ int cgfd_run_async(cgfd_session *s, int it0, int nsteps, double dt) {
 int cgfd_poll(cgfd_session *s) {
   if (!s) return fail(CGFD_E_ARG, "null session");
   hipError_t e = hipStreamQuery(s->st);
+  if (e == hipSuccess && s->st_copy) e = hipStreamQuery(s->st_copy);
   if (e == hipSuccess) return 0;
   if (e == hipErrorNotReady) return 1;
   return fail(CGFD_E_HIP, "stream query: %s", hipGetErrorString(e));
@@ -1697,6 +1824,32 @@ int cgfd_poll(cgfd_session *s) {
 int cgfd_wait(cgfd_session *s) {
   if (!s) return fail(CGFD_E_ARG, "null session");
   HIPCHK(hipStreamSynchronize(s->st));
+  if (s->st_copy) HIPCHK(hipStreamSynchronize(s->st_copy));
+  return CGFD_OK;
+}
+
+int cgfd_set_seismo_ring(cgfd_session *s, int ring_steps) {
+  if (!s) return fail(CGFD_E_ARG, "null session");
+  if (ring_steps < 0 || (ring_steps > 0 && ring_steps % 16 != 0))
+    return fail(CGFD_E_ARG, "seismogram ring of %d steps: 0 (off) or a positive multiple of 16", ring_steps);
+  if (ring_steps == s->ring) return CGFD_OK;
+  HIPCHK(hipStreamSynchronize(s->st));
+  if (ring_steps > 0 && !s->st_copy) {
+    HIPCHK(hipStreamCreateWithFlags(&s->st_copy, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&s->ev_chunk, hipEventDisableTiming));
+    for (hipEvent_t &e : s->ev_copied) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  if (s->st_copy) HIPCHK(hipStreamSynchronize(s->st_copy));
+  // samples recorded so far are laid out for the old pitch: start over
+  dfree(s, s->seismo);
+  s->seismo_stride = 0;
+  if (s->host_seismo) (void)hipHostFree(s->host_seismo);
+  s->host_seismo = nullptr;
+  s->host_stride = 0;
+  s->ring = ring_steps;
+  s->copied_chunk[0] = s->copied_chunk[1] = -1;
+  std::fill(s->rec_changed.begin(), s->rec_changed.end(), 1);
+  s->rec_dirty = true;
   return CGFD_OK;
 }
 

@@ -196,8 +196,9 @@ template <typename TS, typename TC> struct RecvArgs {
   const int *r_shot;
   const long *r_off;        // [nrec][8] layout offsets
   const TC *r_frac;         // [nrec][3]
-  const long *r_out;        // [nrec] offset of the receiver's [9][nt] block in seismo
+  const long *r_out;        // [nrec] offset of the receiver's [9][pitch] block in seismo
   const int *r_nt;          // [nrec] samples kept
+  int ring;                 // 0: pitch nt, sample it in column it; > 0: pitch ring, column it % ring
   TC *seismo;
   const TS *w_pre;
   long shot_stride, cs;

@@ -295,7 +295,8 @@ __global__ __launch_bounds__(256) void k_recv(RecvArgs<TS, TC> a) {
   const TC Lx1 = TC(1) - Lx2, Ly1 = TC(1) - Ly2, Lz1 = TC(1) - Lz2;
   const TC s = b[0] * Lx1 * Ly1 * Lz1 + b[1] * Lx2 * Ly1 * Lz1 + b[2] * Lx1 * Ly2 * Lz1 + b[3] * Lx2 * Ly2 * Lz1 +
                b[5] * Lx1 * Ly1 * Lz2 + b[5] * Lx2 * Ly1 * Lz2 + b[6] * Lx1 * Ly2 * Lz2 + b[7] * Lx2 * Ly2 * Lz2;
-  a.seismo[a.r_out[r] + (long)c * a.r_nt[r] + it] = s;
+  const long pitch = a.ring > 0 ? a.ring : a.r_nt[r];
+  a.seismo[a.r_out[r] + (long)c * pitch + (a.ring > 0 ? it % a.ring : it)] = s;
 }
 
 #if !defined(CGFD_TYPE) || CGFD_TYPE == 0
@@ -402,7 +403,7 @@ template <typename TS, typename TC>
 hipError_t launch_point(const StageArgs<TS, TC> &a, int dx, int dy, int dz, int nshots, hipStream_t st) {
   const long n = a.nbox ? a.box_end[a.nbox - 1] : 0;
   if (n == 0) return hipSuccess;
-  dim3 grid((unsigned)((n + 255) / 256), (unsigned)nshots), block(256);
+  const dim3 grid((unsigned)((n + 255) / 256), (unsigned)nshots);
   const int sel = dx * 4 + dy * 2 + dz;
   switch (a.mode) {
     case MODE_S0: launch_point_m<MODE_S0>(a, grid, sel, st); break;

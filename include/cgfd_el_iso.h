@@ -250,6 +250,18 @@ int cgfd_set_graph(cgfd_session *s, int on);
  * replaced while others run). */
 int cgfd_reserve_receivers(cgfd_session *s, int max_nrecv, int max_nt);
 
+/* Seismogram ring: ring_steps > 0 (a multiple of 16) keeps only ring_steps
+ * samples per trace on the device; runs are cut into chunks of ring_steps/2
+ * steps and each chunk's samples are copied, on a second stream and
+ * overlapping the next chunk, into a pinned host record from which
+ * cgfd_read_seismo reads.  Device memory for seismograms is then
+ * nrecv*9*ring_steps per shot whatever nt_total is (dense receiver arrays
+ * over long runs).  0 keeps the whole record on the device (default).
+ * Changing the mode discards samples recorded so far.  Replaces the
+ * per-step, per-receiver cudaMemcpy of io_recv_keep / io_line_keep
+ * (src/forward/io_funcs.cu:1625-1698). */
+int cgfd_set_seismo_ring(cgfd_session *s, int ring_steps);
+
 /* Continuous batching of shots (no reference counterpart).  A shot with
  * active = 1 advances with every later run, its own step number being the
  * session step minus it_start; its source and receivers use that number, and
