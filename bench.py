@@ -26,6 +26,7 @@ import argparse
 import json
 import os
 import platform
+import re
 import sys
 import time
 
@@ -94,24 +95,32 @@ def cpu_baseline(m, steps):
 
 
 def measured_traffic(workload, storage, split, tile):
-    """HBM bytes per sweep launch from the committed counter summary, if any.
+    """(HBM bytes per sweep launch, profiled tile) from the committed counter
+    summary, or (None, None).
 
     The summary holds FETCH_SIZE + WRITE_SIZE per dispatch of the z-marching
-    kernel on this workload and work split, scaled by the same counters on a
-    copy kernel of known size; a line keyed by the tile ("[32x8/16]") wins
-    over an unkeyed one."""
+    kernel on this workload and work split, scaled by the same counters on
+    streaming kernels of known size, one line per profiled tile
+    ("traffic_bytes_per_launch[32x8/20] = ...").  The line of the tile this
+    run uses wins; otherwise the line of the same thread shape with the
+    nearest z chunk (the chunk changes only the priming planes), and the
+    profiled tile is reported beside the figure."""
     name = "planes" if split == cgfd.SPLIT_PLANES else "shell"
     path = os.path.join(ROOT, "profiles", f"k_march_{workload}_{storage}_{name}_hbm_counters.txt")
     if not os.path.exists(path):
-        return None
-    key = f"[{tile['tx']}x{tile['ty']}/{tile['zchunk']}]"
-    plain = None
+        return None, None
+    best = None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch" + key):
-            return float(line.split("=")[1])
-        if line.startswith("traffic_bytes_per_launch ") or line.startswith("traffic_bytes_per_launch="):
-            plain = float(line.split("=")[1])
-    return plain
+        mt = re.match(r"traffic_bytes_per_launch\[(\d+)x(\d+)/(\d+)\]\s*=\s*([0-9.eE+]+)", line)
+        if not mt:
+            continue
+        tx, ty, zc, val = int(mt.group(1)), int(mt.group(2)), int(mt.group(3)), float(mt.group(4))
+        if (tx, ty) != (tile["tx"], tile["ty"]):
+            continue
+        d = abs(zc - tile["zchunk"])
+        if best is None or d < best[0]:
+            best = (d, val, f"{tx}x{ty}/{zc}")
+    return (best[1], best[2]) if best else (None, None)
 
 
 STREAM_SIZES = ((64, 64, 64), (128, 128, 96), (256, 256, 128), (400, 400, 200), (512, 512, 256))
@@ -337,6 +346,7 @@ def roofline(m, storage, r, stream):
                 bytes_per_point_stage=unit, swept_points=cells, avg_launch_ms=round(r["int_avg_ms"], 5),
                 algorithmic_bytes_per_launch=unit * cells,
                 tile=f"{tile['tx']}x{tile['ty']} threads, {tile['zchunk']} planes",
+                tile_arg=f"{tile['shape']},{tile['zchunk']}",
                 sweep_share_of_step=round(4 * r["int_avg_ms"] / r["step_ms_timed"], 4) if r["step_ms_timed"] else None,
                 const_metric=r["const_metric"],
                 path_bytes_per_point_stage=path_unit,
@@ -346,23 +356,33 @@ def roofline(m, storage, r, stream):
     if stream:
         roof["stream_read_GBps"] = round(stream[0], 1)
         roof["stream_copy_GBps"] = round(stream[1], 1)
+        roof["stream_read_dword_GBps"] = round(stream[2], 1)
         roof["path_frac_of_stream_copy"] = round(roof["path_GBps"] / stream[1], 4)
-    traffic = measured_traffic(m["name"], storage, split, tile)
+    traffic, ttile = measured_traffic(m["name"], storage, split, tile)
     if traffic:
         roof["traffic"] = traffic
+        roof["traffic_tile"] = ttile
         roof["traffic_GBps"] = round(traffic / sec / 1e9, 1)
     return roof
 
 
+def _rel(a, b):
+    return float(f"{np.linalg.norm(a - b) / np.linalg.norm(b):.4g}")
+
+
 def precision_leg(m, steps):
     """FP64, FP32, FP16 and BF16 storage from the same start for `steps`
-    steps: throughput of each (steps 8.. timed) and the relative L2 error of
-    the seismograms (9 components, all receivers) and of a stride-8 sample of
-    the final wavefield against the FP64 run."""
-    out, ref = {}, None
+    steps: throughput of each (steps 8.. timed) and relative L2 errors
+    against the FP64 run (and against FP32 for the 16-bit forms): the
+    velocity seismograms (Vx, Vy, Vz at every receiver, what the stations
+    record), all nine components, a stride-8 sample of the final wavefield,
+    and the velocity-seismogram error over the first n steps for a growing n
+    (the error's growth with run length)."""
+    out, ref, ref32 = {}, None, None
     cells = workloads.cells(m)
     ni, nj, nk = m["ni"], m["nj"], m["nk"]
     cnt = ((ni - 4) // 8 + 1, (nj - 4) // 8 + 1, (nk - 4) // 8 + 1)
+    marks = [n for n in (80, 160, 320, 640, 1000, 1500, 2000) if n < steps] + [steps]
     for name in ("f64", "f32", "f16", "bf16"):
         st = STORE[name]
         s = cgfd.Session(ni, nj, nk, storage=st)
@@ -373,7 +393,13 @@ def precision_leg(m, steps):
             s.set_storage_scale(ex)
         s.run(0, 8, m["dt"])   # autotune and first graph captures outside the clock
         ta = time.perf_counter()
-        s.run(8, steps - 8, m["dt"])
+        it = 8
+        while it < steps:   # chunks of at most 400 steps, each logged
+            n = min(400, steps - it)
+            s.run(it, n, m["dt"])
+            it += n
+            if it < steps:
+                log(f"{name}: step {it} of {steps}")
         tb = time.perf_counter()
         seis = s.read_seismo(0)
         wave = s.fetch_subvolume(0, cgfd.LEVEL_PRE, list(range(9)), (2, 2, 2), cnt, (8, 8, 8), dtype=np.float64)
@@ -385,12 +411,21 @@ def precision_leg(m, steps):
         if name == "f64":
             ref = (seis, wave)
         else:
-            rec["rel_l2_seismograms_vs_f64"] = float(f"{np.linalg.norm(seis - ref[0]) / np.linalg.norm(ref[0]):.4g}")
-            rec["rel_l2_wavefield_vs_f64"] = float(f"{np.linalg.norm(wave - ref[1]) / np.linalg.norm(ref[1]):.4g}")
+            rec["rel_l2_velocity_seismograms_vs_f64"] = _rel(seis[:, :3], ref[0][:, :3])
+            rec["rel_l2_seismograms_vs_f64"] = _rel(seis, ref[0])
+            rec["rel_l2_wavefield_vs_f64"] = _rel(wave, ref[1])
+            rec["rel_l2_velocity_seismograms_vs_f64_by_steps"] = {
+                str(n): _rel(seis[:, :3, :n], ref[0][:, :3, :n]) for n in marks}
+            if name == "f32":
+                ref32 = (seis, wave)
+            else:
+                rec["rel_l2_velocity_seismograms_vs_f32"] = _rel(seis[:, :3], ref32[0][:, :3])
+                rec["rel_l2_wavefield_vs_f32"] = _rel(wave, ref32[1])
         out[name] = rec
         log(f"{name}: {v:.3f} Gcell-RK-stages/s over {steps - 8} steps" +
-            (f", rel L2 vs FP64 seismograms {rec['rel_l2_seismograms_vs_f64']:.3e} wavefield "
-             f"{rec['rel_l2_wavefield_vs_f64']:.3e}" if name != "f64" else ""))
+            (f", rel L2 vs FP64 velocity seismograms {rec['rel_l2_velocity_seismograms_vs_f64']:.3e}, all nine "
+             f"{rec['rel_l2_seismograms_vs_f64']:.3e}, wavefield {rec['rel_l2_wavefield_vs_f64']:.3e}"
+             if name != "f64" else ""))
     for name in ("f16", "bf16", "f64"):
         out[name]["speedup_vs_f32"] = round(out[name]["value"] / out["f32"]["value"], 3)
     return out
@@ -444,11 +479,20 @@ def main():
     ap.add_argument("--serve-steps", default="40,160", help="step range of the served requests")
     ap.add_argument("--serve-load", type=float, default=4.0, help="arrival rate / sequential plain rate")
     ap.add_argument("--only-headline", action="store_true", help="skip every leg after the headline (profiling)")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="run the streaming read and copy kernels once over MIB MiB and exit (the known-size "
+                         "dispatches that scale the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(the decomposition is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calib:
+        r, c = cgfd.hbm_stream(args.copy_calib << 20, 1)
+        rd = cgfd.hbm_stream_dword(args.copy_calib << 20, 1)
+        print(json.dumps({"copy_calib_mib": args.copy_calib, "read_GBps": round(r, 1), "copy_GBps": round(c, 1),
+                          "read_dword_GBps": round(rd, 1)}))
+        return
     if args.only_headline:
         args.error_steps = args.topo_steps = args.stream_mib = args.serve_shots = args.batch_shots = 0
         args.cpu_steps = 0
@@ -477,8 +521,9 @@ def main():
 
     stream = None
     if args.stream_mib > 0:
-        stream = cgfd.hbm_stream(args.stream_mib << 20, 10)
-        log(f"streaming ceilings: read {stream[0]:.0f} GB/s, copy {stream[1]:.0f} GB/s (read + write)")
+        stream = cgfd.hbm_stream(args.stream_mib << 20, 10) + (cgfd.hbm_stream_dword(args.stream_mib << 20, 10),)
+        log(f"streaming ceilings: read {stream[0]:.0f} GB/s, copy {stream[1]:.0f} GB/s (read + write), "
+            f"4-byte-load read {stream[2]:.0f} GB/s")
     roof = roofline(m, args.storage, r, stream) if not args.plain else None
 
     precision = None

@@ -43,6 +43,7 @@ EXPORTS = [
     "cgfd_fetch_subvolume_noalloc",
     "cgfd_sac_write", "cgfd_station_sac", "cgfd_snap_open", "cgfd_snap_append", "cgfd_snap_close",
     "cgfd_set_peak_motion", "cgfd_read_peak_motion", "cgfd_peak_motion_write", "cgfd_hbm_stream",
+    "cgfd_hbm_stream_dword",
     "cgfd_slice_open", "cgfd_slice_append", "cgfd_slice_close", "cgfd_line_sac", "cgfd_line_points",
 ]
 
@@ -210,6 +211,7 @@ def load(path=LIB_PATH):
         "cgfd_server_result": ([vp, ll, i, vp, C.POINTER(d), C.POINTER(d)], i),
         "cgfd_server_get_stats": ([vp, C.POINTER(ServerStats)], i),
         "cgfd_hbm_stream": ([ll, i, C.POINTER(d), C.POINTER(d)], i),
+        "cgfd_hbm_stream_dword": ([ll, i, C.POINTER(d)], i),
         "cgfd_slice_open": ([C.c_char_p, i, i, i, i, vp, C.POINTER(vp)], i),
         "cgfd_slice_append": ([vp, C.c_float, vp], i),
         "cgfd_slice_close": ([vp], i),
@@ -739,6 +741,14 @@ def hbm_stream(nbytes=2 << 30, reps=10):
     r, c = C.c_double(), C.c_double()
     _check(_lib.cgfd_hbm_stream(int(nbytes), int(reps), C.byref(r), C.byref(c)))
     return r.value, c.value
+
+
+def hbm_stream_dword(nbytes=2 << 30, reps=10):
+    """Read GB/s of the 4-byte-per-lane streaming read (the sweep's access size)."""
+    load()
+    r = C.c_double()
+    _check(_lib.cgfd_hbm_stream_dword(int(nbytes), int(reps), C.byref(r)))
+    return r.value
 
 
 def footprint_bytes(ni, nj, nk, storage=STORE_F32, nshots=1, nrecv=0, nt_total=0):

@@ -291,6 +291,11 @@ int cgfd_last_run_timing(cgfd_session *s, double *run_ms, double *interior_avg_m
  * bytes read plus written, in 1e9 bytes per second. */
 int cgfd_hbm_stream(long long bytes, int reps, double *read_gbps, double *copy_gbps);
 
+/* The same read stream with 4-byte cached loads per lane, the access size of
+ * the sweep kernels (no reference counterpart): the read ceiling at that
+ * granularity, and a known-size dispatch for scaling HBM byte counters. */
+int cgfd_hbm_stream_dword(long long bytes, int reps, double *read_gbps);
+
 #ifdef __cplusplus
 }
 #endif
