@@ -41,12 +41,25 @@ import workloads  # noqa: E402
 
 METRIC = "Gcell-RK-stages/s el_iso FP32 vs FP16-store; served shots/hour, p50/p99"
 HBM_PEAK_GBS = 8000.0
-# algorithmic HBM bytes per interior cell and RK stage of the fused dataflow:
-# mean of 39/57/57/39 words (wavefield in, coefficients, RK levels out)
+# The roofline unit: algorithmic HBM bytes per interior cell and RK stage of
+# the reference's dataflow with the RK update fused, mean of 39/57/57/39
+# words (wavefield in, 12 coefficients, RK levels in and out).
 ALGO_BYTES = {"f32": 192.0, "f16": 120.0, "bf16": 120.0, "f64": 384.0}
-# per (point, PML face) pair in the sweep: nine memory variables, mean of
-# 27/45/45/27 words (memory in, RK levels in and out)
-AUX_BYTES = {"f32": 144.0, "f16": 72.0, "bf16": 72.0, "f64": 288.0}
+# This build's own dataflow (stage values): wavefield words per swept point
+# and stage, S0 9 in + 9 out, S1/S2 9 in + 9 W^n + 9 out, S3 9 in + 27 (W^n,
+# U1, U2) + 9 out, mean 29.25; plus 12 coefficient words (3, the medium,
+# when the metric is constant and passed as kernel arguments).  The nine
+# PML memory variables of a (point, face) pair follow the same pattern.
+WAVE_WORDS = 29.25
+ES_WAVE = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}
+ES_MODEL = {"f32": 4, "f16": 4, "bf16": 4, "f64": 8}
+
+
+def path_bytes(storage, const_metric):
+    return WAVE_WORDS * ES_WAVE[storage] + (3 if const_metric else 12) * ES_MODEL[storage]
+
+
+AUX_BYTES = {k: WAVE_WORDS * v for k, v in ES_WAVE.items()}
 STORE = {"f32": cgfd.STORE_F32, "f16": cgfd.STORE_F16, "bf16": cgfd.STORE_BF16, "f64": cgfd.STORE_F64}
 
 
@@ -335,9 +348,7 @@ def roofline(m, storage, r, stream):
     unit = ALGO_BYTES[storage]
     sec = r["int_avg_ms"] * 1e-3
     ach = unit * cells / sec / 1e9
-    # with the constant-metric sweep 9 of the 12 coefficient words per point and stage are kernel arguments
-    es = {"f32": 4, "f16": 4, "bf16": 4, "f64": 8}[storage]
-    path_unit = unit - (9 * es if r["const_metric"] else 0)
+    path_unit = path_bytes(storage, r["const_metric"])
     tile = r["tile"]
     roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
                 traffic=None,
@@ -352,12 +363,12 @@ def roofline(m, storage, r, stream):
                 path_bytes_per_point_stage=path_unit,
                 path_GBps=round(path_unit * cells / sec / 1e9, 1),
                 pml_face_pairs=pairs,
-                with_pml_GBps=round((unit * cells + AUX_BYTES[storage] * pairs) / sec / 1e9, 1))
+                path_with_pml_GBps=round((path_unit * cells + AUX_BYTES[storage] * pairs) / sec / 1e9, 1))
     if stream:
         roof["stream_read_GBps"] = round(stream[0], 1)
         roof["stream_copy_GBps"] = round(stream[1], 1)
         roof["stream_read_dword_GBps"] = round(stream[2], 1)
-        roof["path_frac_of_stream_copy"] = round(roof["path_GBps"] / stream[1], 4)
+        roof["path_frac_of_stream_read"] = round(roof["path_GBps"] / stream[0], 4)
     traffic, ttile = measured_traffic(m["name"], storage, split, tile)
     if traffic:
         roof["traffic"] = traffic
