@@ -1232,11 +1232,13 @@ int cgfd_upload_metric(cgfd_session *s, int dtype, const void *metric) {
   // A Cartesian grid gives one value per metric component everywhere (ghosts
   // too), up to the rounding of the metric computation: its off-diagonal
   // terms come out as noise of ~1e-14 of the diagonal instead of 0.  The
-  // grid counts as constant when every component stays within 1e-10 of the
-  // largest metric magnitude of its first value; the sweep then takes the
-  // first point's nine values as arguments (a change below 1e-10 relative
-  // in any derivative, far under FP32 rounding).  CGFD_FLAG_FULL_METRIC
-  // keeps the per-point arrays.
+  // grid counts as Cartesian when every component stays within 1e-10 of the
+  // largest metric magnitude of its first value and the off-diagonal ones
+  // within that of zero; the sweep then takes the first point's diagonal
+  // values as arguments and the off-diagonal ones as 0 (a change below
+  // 1e-10 relative in any derivative, far under FP32 rounding).  A uniform
+  // but sheared grid keeps the per-point arrays, as does
+  // CGFD_FLAG_FULL_METRIC.
   const size_t n = (size_t)(s->d.ni + 6) * (s->d.nj + 6) * (s->d.nk + 6);
   double scale = 0.0;
   for (int c = 0; c < 9; c++) scale = std::max(scale, std::fabs(rd(dtype, metric, (size_t)(c + 1) * n)));
@@ -1244,9 +1246,11 @@ int cgfd_upload_metric(cgfd_session *s, int dtype, const void *metric) {
   bool cst = !(s->d.flags & CGFD_FLAG_FULL_METRIC) && scale > 0.0;
   for (int c = 0; c < 9 && cst; c++) {
     const double v0 = rd(dtype, metric, (size_t)(c + 1) * n);
+    const bool diag = c == 0 || c == 4 || c == 8;  // xi_x, eta_y, zeta_z
+    if (!diag && std::fabs(v0) > tol) { cst = false; break; }
     for (size_t q = 1; q < n; q++)
       if (std::fabs(rd(dtype, metric, (size_t)(c + 1) * n + q) - v0) > tol) { cst = false; break; }
-    s->cmet[c] = v0;
+    s->cmet[c] = diag ? v0 : 0.0;
   }
   s->met_const = cst;
   invalidate(s);

@@ -145,8 +145,8 @@ template <typename TS, typename TM> struct StageArgs {
   const TM *med;   // lam, mu, 1/rho, stride L.cs
   const TM *mat;   // 18 components [c][j][i] on the physical top plane
   int free_top;
-  int met_const;   // 1: every metric component is one constant (Cartesian grid)
-  TM cmet[9];      // xi_x..zeta_z when met_const
+  int met_const;   // 1: one constant per metric component, off-diagonal ones zero (Cartesian grid)
+  TM cmet[9];      // xi_x..zeta_z when met_const (off-diagonal entries exactly 0)
   FaceArgs<TS, TM> f[6];
   // shot-0 level bases: stage input, W^n, output, U1 (S3: read, then
   // overwritten by W^{n+1}) and U2 (S3)

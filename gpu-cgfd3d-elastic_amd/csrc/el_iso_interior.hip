@@ -17,8 +17,10 @@
 // is issued before the first barrier, so one memory latency is exposed per
 // plane instead of one per dependent load.
 //
-// On a Cartesian grid every metric component is one constant (CART = true):
-// the nine metric values come in as arguments instead of nine HBM streams.
+// On a Cartesian grid every metric component is one constant and only the
+// diagonal ones (xi_x, eta_y, zeta_z) are non-zero (CART = true): the metric
+// comes in as arguments instead of nine HBM streams, and each derivative
+// maps to one physical direction (6 of the 9 products per component drop).
 //
 // The PML variant (PML = true) sweeps whole x-y planes, PML face slabs
 // included: a point inside a face box recomputes the nine derivatives along
@@ -262,9 +264,9 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
         dy += Drp<TC, DY>::c(n) * lds[buf][c][ly + n][lx + HXL];
         dz += Drp<TC, DZ>::c(n) * q[c][n];
       }
-      const TC g0 = xix * dx + etx * dy + ztx * dz;
-      const TC g1 = xiy * dx + ety * dy + zty * dz;
-      const TC g2 = xiz * dx + etz * dy + ztz * dz;
+      const TC g0 = CART ? xix * dx : xix * dx + etx * dy + ztx * dz;
+      const TC g1 = CART ? ety * dy : xiy * dx + ety * dy + zty * dz;
+      const TC g2 = CART ? ztz * dz : xiz * dx + etz * dy + ztz * dz;
       if (c < 3) {
         Lg[c][0] = g0; Lg[c][1] = g1; Lg[c][2] = g2;
       } else if (c == TXX) {

@@ -220,7 +220,8 @@ int cgfd_get_tile(cgfd_session *s, int *shape, int *tx, int *ty, int *zchunk);
 int cgfd_get_split(cgfd_session *s, int *split);
 
 /* 1 when the uploaded metric runs the constant-metric sweep (a Cartesian
- * grid: nine metric values as kernel arguments instead of nine streams). */
+ * grid: one value per metric component, off-diagonal ones zero; the three
+ * diagonal values are kernel arguments instead of nine streams). */
 int cgfd_get_const_metric(cgfd_session *s, int *const_metric);
 
 /* Enable or disable (default) HIP-event timing of every interior kernel
