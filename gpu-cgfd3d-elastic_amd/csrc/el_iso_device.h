@@ -28,8 +28,14 @@ enum Mode { MODE_RHS = 0, MODE_S0 = 1, MODE_S12 = 2, MODE_S3 = 3 };
 template <typename TS> struct St;
 // ld/st: ordinary accesses; ld_nt/st_nt: non-temporal (streamed once, kept
 // out of the way of the wavefield planes that the stencil re-reads from L2)
+// raw / ld_raw / cvt: the loaded bits kept unconverted (a prefetch buffer
+// converts where the value is used, one plane later)
 template <> struct St<float> {
   static constexpr bool scaled = false;
+  using raw = float;
+  __device__ __forceinline__ static raw ld_raw(const float *p) { return *p; }
+  __device__ __forceinline__ static raw ld_raw_nt(const float *p) { return __builtin_nontemporal_load(p); }
+  __device__ __forceinline__ static float cvt(raw r) { return r; }
   __device__ __forceinline__ static float ld(const float *p) { return *p; }
   __device__ __forceinline__ static void st(float *p, float v) { *p = v; }
   __device__ __forceinline__ static float ld_nt(const float *p) { return __builtin_nontemporal_load(p); }
@@ -37,6 +43,10 @@ template <> struct St<float> {
 };
 template <> struct St<double> {
   static constexpr bool scaled = false;
+  using raw = double;
+  __device__ __forceinline__ static raw ld_raw(const double *p) { return *p; }
+  __device__ __forceinline__ static raw ld_raw_nt(const double *p) { return __builtin_nontemporal_load(p); }
+  __device__ __forceinline__ static double cvt(raw r) { return r; }
   __device__ __forceinline__ static double ld(const double *p) { return *p; }
   __device__ __forceinline__ static void st(double *p, double v) { *p = v; }
   __device__ __forceinline__ static double ld_nt(const double *p) { return __builtin_nontemporal_load(p); }
@@ -44,6 +54,12 @@ template <> struct St<double> {
 };
 template <> struct St<__half> {
   static constexpr bool scaled = true;
+  using raw = unsigned short;
+  __device__ __forceinline__ static raw ld_raw(const __half *p) { return *reinterpret_cast<const unsigned short *>(p); }
+  __device__ __forceinline__ static raw ld_raw_nt(const __half *p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+  }
+  __device__ __forceinline__ static float cvt(raw r) { return __half2float(__ushort_as_half(r)); }
   __device__ __forceinline__ static float ld(const __half *p) { return __half2float(*p); }
   __device__ __forceinline__ static void st(__half *p, float v) { *p = __float2half(v); }
   __device__ __forceinline__ static float ld_nt(const __half *p) {
@@ -56,6 +72,15 @@ template <> struct St<__half> {
 };
 template <> struct St<__hip_bfloat16> {
   static constexpr bool scaled = true;
+  using raw = unsigned short;
+  __device__ __forceinline__ static raw ld_raw(const __hip_bfloat16 *p) {
+    return *reinterpret_cast<const unsigned short *>(p);
+  }
+  __device__ __forceinline__ static raw ld_raw_nt(const __hip_bfloat16 *p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+  }
+  // bfloat16 is the upper half of a float
+  __device__ __forceinline__ static float cvt(raw r) { return __uint_as_float((unsigned)r << 16); }
   __device__ __forceinline__ static float ld(const __hip_bfloat16 *p) { return __bfloat162float(*p); }
   __device__ __forceinline__ static void st(__hip_bfloat16 *p, float v) { *p = __float2bfloat16(v); }
   __device__ __forceinline__ static float ld_nt(const __hip_bfloat16 *p) {

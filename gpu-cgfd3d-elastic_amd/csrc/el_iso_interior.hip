@@ -117,6 +117,12 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
   auto gld = [&](int c, unsigned off) -> TC {
     return (TC)St<TS>::ld(&at_b(wq(c), off * (unsigned)sizeof(TS))) * sc(c);
   };
+  // Prefetched values stay raw (16-bit storage: the loaded bits) until the
+  // plane that uses them: a conversion next to its load would make the wave
+  // wait for the whole prefetch before computing the current plane.
+  using RT = typename St<TS>::raw;
+  auto graw = [&](int c, unsigned off) -> RT { return St<TS>::ld_raw(&at_b(wq(c), off * (unsigned)sizeof(TS))); };
+  auto cv = [&](int c, RT r) -> TC { return (TC)St<TS>::cvt(r) * sc(c); };
 
   // this thread's halo cells (positions fixed over the sweep)
   const int ycb = tid / YH, yr = tid % YH;   // y halo: first component, cell
@@ -148,6 +154,18 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
   TC *ldsf = &lds[0][0][0][0];
   constexpr int LBUF = kNC * LH * LW;  // one LDS plane buffer
 
+  // scale of each halo load's component on this lane, by selects over the
+  // kernel arguments (a per-lane index into them would be a memory load)
+  TC hsc[NPER];
+#pragma unroll
+  for (int p = 0; p < NPER; p++) {
+    const int c = hcomp(p);
+    TC v = sc(0);
+#pragma unroll
+    for (int q = 1; q < kNC; q++) v = c == q ? sc(q) : v;
+    hsc[p] = v;
+  }
+
   // register queues: planes k+zlo .. k+zlo+4
   TC q[kNC][5];
 #pragma unroll
@@ -172,25 +190,26 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
   // each touched once per stage: they bypass L2 (non-temporal) and leave it
   // to the planes the stencil re-reads (sweep 1.93 -> 1.78 ms on the
   // topography workload).  16-bit storage gained nothing (its loads lost).
-  auto ld_rk = [&](const TS *p) -> TC {
-    if constexpr (St<TS>::scaled) return (TC)St<TS>::ld(p);
-    else return (TC)St<TS>::ld_nt(p);
+  auto ld_rk = [&](const TS *p) -> RT {
+    if constexpr (St<TS>::scaled) return St<TS>::ld_raw(p);
+    else return St<TS>::ld_raw_nt(p);
   };
   auto st_w = [&](TS *p, TC v) {
     if constexpr (St<TS>::scaled) St<TS>::st(p, v);
     else St<TS>::st_nt(p, v);
   };
-  TC n_lead[kNC], n_hv[NPER], n_met[12], n_pre[kNC], n_u1[kNC], n_u2[kNC];
+  RT n_lead[kNC], n_hv[NPER], n_pre[kNC], n_u1[kNC], n_u2[kNC];
+  TC n_met[12];
 #pragma unroll
-  for (int c = 0; c < kNC; c++) { n_pre[c] = TC(0); n_u1[c] = TC(0); n_u2[c] = TC(0); }
-  // this plane's RK levels at the point
-  auto load_rk = [&](unsigned qb, TC *pre, TC *u1, TC *u2) {
+  for (int c = 0; c < kNC; c++) { n_pre[c] = RT(0); n_u1[c] = RT(0); n_u2[c] = RT(0); }
+  // this plane's RK levels at the point (raw)
+  auto load_rk = [&](unsigned qb, RT *pre, RT *u1, RT *u2) {
 #pragma unroll
     for (int c = 0; c < kNC; c++) {
-      pre[c] = need_pre ? (TC)ld_rk(&at_b(a.w_pre + so + c * cs, qb)) * sc(c) : TC(0);
+      pre[c] = need_pre ? ld_rk(&at_b(a.w_pre + so + c * cs, qb)) : RT(0);
       if constexpr (need_u) {
-        u1[c] = (TC)ld_rk(&at_b(a.w_end + so + c * cs, qb)) * sc(c);
-        u2[c] = (TC)ld_rk(&at_b(a.w_u2 + so + c * cs, qb)) * sc(c);
+        u1[c] = ld_rk(&at_b(a.w_end + so + c * cs, qb));
+        u2[c] = ld_rk(&at_b(a.w_u2 + so + c * cs, qb));
       }
     }
   };
@@ -198,11 +217,11 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
     const unsigned pk = (unsigned)(kk + kNG) * upz;
     const unsigned o = pk + col;
 #pragma unroll
-    for (int c = 0; c < kNC; c++) n_lead[c] = gld(c, (unsigned)(kk + zlo + 4 + kNG) * upz + col);
+    for (int c = 0; c < kNC; c++) n_lead[c] = graw(c, (unsigned)(kk + zlo + 4 + kNG) * upz + col);
 #pragma unroll
     for (int p = 0; p < NPER; p++) {
       const int c = hcomp(p);
-      n_hv[p] = gld(c < kNC ? c : kNC - 1, pk + (p < NPY ? y_off : x_off));
+      n_hv[p] = graw(c < kNC ? c : kNC - 1, pk + (p < NPY ? y_off : x_off));
     }
     const unsigned ob = o * (unsigned)sizeof(TC);
 #pragma unroll
@@ -212,6 +231,11 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
     const unsigned qb = o * (unsigned)sizeof(TS);
     if constexpr (!LATE_RK && need_pre) load_rk(qb, n_pre, n_u1, n_u2);
   };
+  // Every priming load done before the loop (vmcnt(0); the other counters
+  // untouched): otherwise the wait analysis, merging the loop entry with its
+  // back edge, makes each plane wait for most of the next plane's prefetch
+  // before its arithmetic.
+  __builtin_amdgcn_s_waitcnt(0x0F70);
   issue(kbeg);
 
   for (int k = kbeg; k <= kend; k++) {
@@ -223,20 +247,20 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
     for (int c = 0; c < kNC; c++) {
 #pragma unroll
       for (int s = 0; s < 4; s++) q[c][s] = q[c][s + 1];
-      q[c][4] = n_lead[c];
-      pre_v[c] = n_pre[c];
-      part_v[c] = need_u ? rk_partial(a.rk_w, n_pre[c], n_u1[c], n_u2[c]) : TC(0);
+      q[c][4] = cv(c, n_lead[c]);
+      if constexpr (!LATE_RK) {
+        pre_v[c] = need_pre ? cv(c, n_pre[c]) : TC(0);
+        part_v[c] = need_u ? rk_partial(a.rk_w, pre_v[c], cv(c, n_u1[c]), cv(c, n_u2[c])) : TC(0);
+      }
     }
 #pragma unroll
-    for (int p = 0; p < NPER; p++) hv[p] = n_hv[p];
+    for (int p = 0; p < NPER; p++) hv[p] = (TC)St<TS>::cvt(n_hv[p]) * hsc[p];
 #pragma unroll
     for (int m = 0; m < 12; m++) mt[m] = n_met[m];
-    TC u1_v[kNC], u2_v[kNC];
-    if constexpr (LATE_RK && need_pre) {
-      // this plane's RK levels, issued before the next plane's prefetch
-      // (memory counters complete in order)
-      load_rk(qb, pre_v, u1_v, u2_v);
-    }
+    // LATE_RK: this plane's RK levels, issued before the next plane's
+    // prefetch (memory counters complete in order), converted where used
+    RT pre_r[kNC], u1_r[kNC], u2_r[kNC];
+    if constexpr (LATE_RK && need_pre) load_rk(qb, pre_r, u1_r, u2_r);
     if (k < kend) issue(k + 1);
     const TC xix = mt[0], xiy = mt[1], xiz = mt[2], etx = mt[3], ety = mt[4], etz = mt[5];
     const TC ztx = mt[6], zty = mt[7], ztz = mt[8], lam = mt[9], mu = mt[10], slw = mt[11];
@@ -344,13 +368,17 @@ __global__ __launch_bounds__(TX *TY, MINB) void k_march(StageArgs<TS, TC> a, Mar
         TS *wout = a.w_out + so + c * cs;
         TS *wend = a.w_end + so + c * cs;
         const TC is = St<TS>::scaled ? (TC)a.isc[c] : TC(1);
+        TC pre_c = TC(0);
+        if constexpr (need_pre) pre_c = LATE_RK ? cv(c, pre_r[c]) : pre_v[c];
         if constexpr (mode == MODE_S12) {
-          st_w(&at_b(wout, qb), (pre_v[c] + a.ca * h[c]) * is);
+          st_w(&at_b(wout, qb), (pre_c + a.ca * h[c]) * is);
         } else if constexpr (mode == MODE_S0) {
           st_w(&at_b(wout, qb), (q[c][zc0] + a.ca * h[c]) * is);
         } else if constexpr (mode == MODE_S3) {
-          const TC part = LATE_RK ? rk_partial(a.rk_w, pre_v[c], u1_v[c], u2_v[c]) : part_v[c];
-          st_w(&at_b(wend, qb), rk_final(pre_v[c], part, a.rk_w[2], q[c][zc0], a.cb * h[c]) * is);
+          TC part;
+          if constexpr (LATE_RK) part = rk_partial(a.rk_w, pre_c, cv(c, u1_r[c]), cv(c, u2_r[c]));
+          else part = part_v[c];
+          st_w(&at_b(wend, qb), rk_final(pre_c, part, a.rk_w[2], q[c][zc0], a.cb * h[c]) * is);
         } else {
           st_w(&at_b(wout, qb), h[c] * is);
         }
