@@ -68,15 +68,24 @@ def log(msg):
 
 
 def host_cpu():
-    model, n = "", 0
+    """(model name, logical CPUs, sockets, physical cores) from /proc/cpuinfo."""
+    model, n, phys, cores = "", 0, set(), set()
+    pid = None
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                model = model or line.split(":", 1)[1].strip()
+            key, _, val = line.partition(":")
+            key, val = key.strip(), val.strip()
+            if key == "model name":
+                model = model or val
                 n += 1
+            elif key == "physical id":
+                pid = val
+                phys.add(val)
+            elif key == "core id":
+                cores.add((pid, val))
     except OSError:
         pass
-    return model or platform.processor(), n
+    return model or platform.processor(), n, len(phys), len(cores)
 
 
 def cpu_baseline(m, steps):
@@ -98,13 +107,13 @@ def cpu_baseline(m, steps):
         raise RuntimeError("oracle run failed")
     sec = L.orc_last_loop_seconds()
     cells = workloads.cells(m)
-    model, ncpu = host_cpu()
+    model, ncpu, nsock, ncore = host_cpu()
     return dict(value=round(cells * 4 * steps / sec / 1e9, 5), unit="Gcell-RK-stages/s",
                 cores=int(L.orc_max_threads()), kind="port",
                 sample=f"{steps} full RK4 steps of the same {m['ni']}x{m['nj']}x{m['nk']} model, FP32, unfused "
                        f"reference stage order (RHS then axpys), {sec:.1f} s loop",
-                host=f"{model}, {ncpu} logical CPUs visible; OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} "
-                     f"OMP_PLACES={os.environ.get('OMP_PLACES')}")
+                host=f"{model}; {ncpu} logical CPUs visible, {nsock} socket(s), {ncore} physical cores; "
+                     f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} OMP_PLACES={os.environ.get('OMP_PLACES')}")
 
 
 def measured_traffic(workload, storage, split, tile):
@@ -471,7 +480,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--workload", default="explosion800", choices=["explosion800", "topo", "small"])
     ap.add_argument("--storage", default="f32", choices=sorted(STORE))
-    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=4, help="steps of the CPU baseline sample (0: skip)")
     ap.add_argument("--plain", action="store_true", help="per-point kernel everywhere")
     ap.add_argument("--pml-point", action="store_true",
                     help="PML slabs by the per-point kernel (the sweep keeps to the box inside them)")
@@ -570,6 +579,12 @@ def main():
     if args.cpu_steps > 0:
         log(f"cpu baseline: {args.cpu_steps} steps")
         cpu = cpu_baseline(m, args.cpu_steps)
+        # the 100x100x60 configuration in full (501 executed steps), as the CPU baseline plan has it
+        m1 = workloads.small()
+        c1 = cpu_baseline(m1, m1["nt_total"])
+        cpu["config_100x100x60_full"] = dict(value=c1["value"], unit=c1["unit"], sample=c1["sample"])
+        log(f"cpu baseline {cpu['value']} Gcell-RK-stages/s ({cpu['cores']} threads); 100x100x60 in full "
+            f"{c1['value']}")
 
     desc = {"explosion800": "explosion model at 800x800x400: Cartesian 100 m, homogeneous Vp 6000 / Vs 3464 / "
                             "rho 2670, explosive source 3 km deep, Ricker 2 Hz, 20 surface receivers",
