@@ -801,6 +801,11 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
     sa[q].mode = smode[q];
     sa[q].ca = TC(0);
     sa[q].cb = TC(0);
+    // time every shot slot, active or not: a batched session is tuned for
+    // its full batch even when its first run has one shot active (a server
+    // group warmed by one request).  The stages write only the stage-value
+    // levels of each slot; W^n (and its PML memory) is read, never written.
+    sa[q].shot_active = nullptr;
   }
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
