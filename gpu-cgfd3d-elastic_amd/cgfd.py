@@ -7,6 +7,7 @@ the shared library has not been built; there is no CPU fallback.
 """
 import ctypes as C
 import os
+import re
 
 import numpy as np
 
@@ -43,7 +44,7 @@ EXPORTS = [
     "cgfd_fetch_subvolume_noalloc",
     "cgfd_sac_write", "cgfd_station_sac", "cgfd_snap_open", "cgfd_snap_append", "cgfd_snap_close",
     "cgfd_set_peak_motion", "cgfd_read_peak_motion", "cgfd_peak_motion_write", "cgfd_hbm_stream",
-    "cgfd_hbm_stream_dword",
+    "cgfd_hbm_stream_dword", "cgfd_station_file",
     "cgfd_slice_open", "cgfd_slice_append", "cgfd_slice_close", "cgfd_line_sac", "cgfd_line_points",
 ]
 
@@ -212,6 +213,7 @@ def load(path=LIB_PATH):
         "cgfd_server_get_stats": ([vp, C.POINTER(ServerStats)], i),
         "cgfd_hbm_stream": ([ll, i, C.POINTER(d), C.POINTER(d)], i),
         "cgfd_hbm_stream_dword": ([ll, i, C.POINTER(d)], i),
+        "cgfd_station_file": ([C.c_char_p, i, i, i, vp, vp, vp, i, vp, C.POINTER(i)], i),
         "cgfd_slice_open": ([C.c_char_p, i, i, i, i, vp, C.POINTER(vp)], i),
         "cgfd_slice_append": ([vp, C.c_float, vp], i),
         "cgfd_slice_close": ([vp], i),
@@ -841,6 +843,38 @@ def session_slice(sess, shot, axis, gindex):
     start[axis], count[axis] = gindex, 1
     w = sess.fetch_subvolume(shot, LEVEL_PRE, list(range(9)), tuple(start), tuple(count))
     return np.squeeze(w, axis=3 - axis)
+
+
+class Station(C.Structure):
+    """cgfd_station (include/cgfd_setup.h)."""
+    _fields_ = [("name", C.c_char * 64), ("by_coord", C.c_int), ("depth", C.c_int), ("inp", C.c_double * 3),
+                ("inside", C.c_int), ("indx8", C.c_longlong * 8), ("frac", C.c_double * 3),
+                ("xyz", C.c_double * 3)]
+
+
+def station_file(path, ni, nj, nk, x=None, y=None, z=None, max_n=100000):
+    """Stations of a station file (io_recv_read_locate): list of dicts with
+    name, by_coord, depth, in, inside, indx8, frac, xyz, in file order."""
+    load()
+    xs = [_arr(a, np.float64) if a is not None else None for a in (x, y, z)]
+    xp = [_ptr(a) for a in xs]
+    nf = C.c_int(0)
+    # a first pass with room for none reads and checks the file and gives its count
+    r = _lib.cgfd_station_file(str(path).encode(), ni, nj, nk, *xp, 0, None, C.byref(nf))
+    if r < 0:
+        _check(r)
+    n_out = min(nf.value, max_n)
+    arr = (Station * max(n_out, 1))()
+    r = _lib.cgfd_station_file(str(path).encode(), ni, nj, nk, *xp, n_out, arr, C.byref(nf))
+    if r < 0:
+        _check(r)
+    out = []
+    for q in range(r):
+        s_ = arr[q]
+        out.append(dict(name=s_.name.decode(), by_coord=s_.by_coord, depth=s_.depth, inp=list(s_.inp),
+                        inside=bool(s_.inside), indx8=np.array(s_.indx8[:], dtype=np.int64),
+                        frac=np.array(s_.frac[:]), xyz=np.array(s_.xyz[:])))
+    return out
 
 
 def line_points(ni, nj, nk, start, incre, count, x=None, y=None, z=None):

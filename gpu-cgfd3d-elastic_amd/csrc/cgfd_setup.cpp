@@ -1581,6 +1581,119 @@ int cgfd_locate_points(int ni, int nj, int nk, const double *x, const double *y,
   return CGFD_OK;
 }
 
+// ---- station files (io_recv_read_locate, io_funcs.cu:24-299) ----------------------
+namespace {
+
+// next line that is neither empty nor a comment, newline removed
+// (io_get_nextline, io_funcs.cu:2131-2154)
+bool next_line(FILE *fp, char *buf, int len) {
+  do {
+    if (!std::fgets(buf, len, fp)) return false;
+  } while (buf[0] == '#' || buf[0] == '\n');
+  const size_t n = std::strlen(buf);
+  if (n > 0 && buf[n - 1] == '\n') buf[n - 1] = '\0';
+  return true;
+}
+
+void corners8(const G3 &g, int i, int j, int k, long long *ix) {
+  const long p = g.at(i + NG, j + NG, k + NG);
+  ix[0] = p; ix[1] = p + 1; ix[2] = p + g.sy; ix[3] = p + 1 + g.sy;
+  ix[4] = p + g.sz; ix[5] = p + 1 + g.sz; ix[6] = p + g.sy + g.sz; ix[7] = p + 1 + g.sy + g.sz;
+}
+
+}  // namespace
+
+int cgfd_station_file(const char *path, int ni, int nj, int nk, const double *x, const double *y, const double *z,
+                      int max_n, cgfd_station *out, int *n_file) {
+  if (!path || max_n < 0 || (max_n > 0 && !out) || ni < 2 || nj < 2 || nk < 2) return CGFD_E_ARG;
+  FILE *fp = std::fopen(path, "rt");
+  if (!fp) {
+    cgfd_internal_set_error((std::string("cannot open station file ") + path).c_str());
+    return CGFD_E_ARG;
+  }
+  char line[500];
+  int total = -1;
+  if (!next_line(fp, line, sizeof(line)) || std::sscanf(line, "%d", &total) != 1 || total < 0) {
+    std::fclose(fp);
+    cgfd_internal_set_error("station file: the first line must hold the number of stations");
+    return CGFD_E_ARG;
+  }
+  if (n_file) *n_file = total;
+  const int n = std::min(total, max_n);
+  std::vector<cgfd_station> st((size_t)n);
+  for (int r = 0; r < total; r++) {
+    char name[500];
+    int fc = 0, fd = 0;
+    float v[3] = {0, 0, 0};
+    if (!next_line(fp, line, sizeof(line)) ||
+        std::sscanf(line, "%499s %d %d %g %g %g", name, &fc, &fd, &v[0], &v[1], &v[2]) != 6) {
+      std::fclose(fp);
+      cgfd_internal_set_error(("station file: station line " + std::to_string(r + 1) + " unreadable").c_str());
+      return CGFD_E_ARG;
+    }
+    if (r >= n) continue;
+    cgfd_station &s = st[(size_t)r];
+    std::memset(&s, 0, sizeof(s));
+    std::snprintf(s.name, sizeof(s.name), "%s", name);
+    s.by_coord = fc;
+    s.depth = fd;
+    for (int d = 0; d < 3; d++) s.in[d] = v[d];
+  }
+  std::fclose(fp);
+  const G3 g(ni, nj, nk);
+  // coordinate stations: located together
+  std::vector<int> cq;
+  for (int r = 0; r < n; r++)
+    if (st[(size_t)r].by_coord == 1) cq.push_back(r);
+  if (!cq.empty()) {
+    if (!x || !y || !z) {
+      cgfd_internal_set_error("station file: stations by coordinates need the grid coordinates");
+      return CGFD_E_ARG;
+    }
+    const int m = (int)cq.size();
+    std::vector<double> pts(3 * (size_t)m);
+    std::vector<int> dfl((size_t)m), idx(3 * (size_t)m), found((size_t)m);
+    std::vector<double> fr(3 * (size_t)m);
+    for (int q = 0; q < m; q++) {
+      for (int d = 0; d < 3; d++) pts[3 * (size_t)q + d] = st[(size_t)cq[q]].in[d];
+      dfl[(size_t)q] = st[(size_t)cq[q]].depth == 1 ? 1 : 0;
+    }
+    int r = cgfd_locate_points(ni, nj, nk, x, y, z, m, pts.data(), dfl.data(), idx.data(), fr.data(), found.data());
+    if (r) return r;
+    const FGrid G(ni, nj, nk, x, y, z);
+    for (int q = 0; q < m; q++) {
+      cgfd_station &s = st[(size_t)cq[q]];
+      // the reference keeps the point with its depth turned into z
+      float sx = (float)s.in[0], sy = (float)s.in[1], sz = (float)s.in[2], zs;
+      if (s.depth == 1 && sx >= G.xmin && sx <= G.xmax && sy >= G.ymin && sy <= G.ymax && surface_z(G, sx, sy, &zs))
+        sz = zs - sz;
+      s.xyz[0] = sx; s.xyz[1] = sy; s.xyz[2] = sz;
+      if (!found[(size_t)q]) continue;
+      s.inside = 1;
+      corners8(g, idx[3 * (size_t)q], idx[3 * (size_t)q + 1], idx[3 * (size_t)q + 2], s.indx8);
+      for (int d = 0; d < 3; d++) s.frac[d] = fr[3 * (size_t)q + d];
+    }
+  }
+  // grid-index stations
+  for (int r = 0; r < n; r++) {
+    cgfd_station &s = st[(size_t)r];
+    if (s.by_coord == 1) continue;
+    if (cgfd_receiver_index(ni, nj, nk, s.in[0], s.in[1], s.in[2], s.depth, s.indx8, s.frac) != CGFD_OK) {
+      std::memset(s.indx8, 0, sizeof(s.indx8));
+      s.frac[0] = s.frac[1] = s.frac[2] = 0.0;
+      continue;
+    }
+    s.inside = 1;
+    if (x && y && z) {
+      s.xyz[0] = (float)x[s.indx8[0]];
+      s.xyz[1] = (float)y[s.indx8[0]];
+      s.xyz[2] = (float)z[s.indx8[0]];
+    }
+  }
+  for (int r = 0; r < n; r++) out[r] = st[(size_t)r];
+  return n;
+}
+
 int cgfd_storage_scales(int storage, int ni, int nj, int nk, const double *metric, const double *lam,
                         const double *mu, const double *slw, const cgfd_source_desc *src, double dt, int *exp2) {
   if (!exp2 || ni < 1 || nj < 1 || nk < 1 || !(dt > 0)) return CGFD_E_ARG;

@@ -228,6 +228,31 @@ int cgfd_line_points(int ni, int nj, int nk, const double *x, const double *y, c
 int cgfd_receiver_index(int ni, int nj, int nk, double rx, double ry, double rz, int depth_flag, long long *indx8,
                         double *frac3);
 
+/* One station of a station file, as io_recv_read_locate reads and locates
+ * it (src/forward/io_funcs.cu:24-299). */
+typedef struct {
+  char name[64];      /* first field of the line (longer names are cut) */
+  int by_coord;       /* file flag: 1 physical coordinates, 0 grid index */
+  int depth;          /* file flag: 1 the third value is a depth below the free surface */
+  double in[3];       /* the three values as read (float precision, as %g into float) */
+  int inside;         /* 1: located inside the physical grid (kept by the reference) */
+  long long indx8[8]; /* trilinear corner offsets in the reference host layout */
+  double frac[3];     /* fractional position in the lower cell */
+  double xyz[3];      /* station coordinates: the located point (depth turned into z), or for
+                         grid-index input the grid node at the lower corner; 0 without x, y, z */
+} cgfd_station;
+
+/* Station file (io_recv_read_locate, src/forward/io_funcs.cu:24-299, lines read
+ * as io_get_nextline does, io_funcs.cu:2131-2154: lines starting with '#' and
+ * empty lines skipped): the number of stations, then per station
+ * "name by_coord depth v1 v2 v3".  Grid-index stations are located as
+ * cgfd_receiver_index does, coordinate stations as cgfd_locate_points (x, y,
+ * z required when the file has any).  Fills out[0 .. min(count, max_n)) in
+ * file order; *n_file receives the count the file states.  Returns the
+ * number filled, or a negative CGFD_E_* (missing file, bad line). */
+int cgfd_station_file(const char *path, int ni, int nj, int nk, const double *x, const double *y, const double *z,
+                      int max_n, cgfd_station *out, int *n_file);
+
 /* ---- reduced-precision storage -------------------------------------------------- */
 
 /* Power-of-two scale exponents (for cgfd_set_storage_scale) that keep the
