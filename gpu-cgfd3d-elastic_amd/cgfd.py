@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcgfd_el_iso.so")
 
 STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
+# CGFD_WAVELET_* (include/cgfd_setup.h)
+WAVELET_RICKER, WAVELET_GAUSSIAN, WAVELET_RICKER_DERIV, WAVELET_GAUSSIAN_DERIV, WAVELET_KLAUDER = 0, 1, 2, 3, 4
 DT_F32, DT_F64 = 0, 1
 LEVEL_PRE, LEVEL_TMPA, LEVEL_TMPB, LEVEL_END, LEVEL_RHS = 0, 1, 2, 3, 4
 FLAG_PLAIN = 1
@@ -44,7 +46,7 @@ EXPORTS = [
     "cgfd_fetch_subvolume_noalloc",
     "cgfd_sac_write", "cgfd_station_sac", "cgfd_snap_open", "cgfd_snap_append", "cgfd_snap_close",
     "cgfd_set_peak_motion", "cgfd_read_peak_motion", "cgfd_peak_motion_write", "cgfd_hbm_stream",
-    "cgfd_hbm_stream_dword", "cgfd_station_file",
+    "cgfd_hbm_stream_dword", "cgfd_station_file", "cgfd_source_file",
     "cgfd_slice_open", "cgfd_slice_append", "cgfd_slice_close", "cgfd_line_sac", "cgfd_line_points",
 ]
 
@@ -214,6 +216,8 @@ def load(path=LIB_PATH):
         "cgfd_hbm_stream": ([ll, i, C.POINTER(d), C.POINTER(d)], i),
         "cgfd_hbm_stream_dword": ([ll, i, C.POINTER(d)], i),
         "cgfd_station_file": ([C.c_char_p, i, i, i, vp, vp, vp, i, vp, C.POINTER(i)], i),
+        "cgfd_source_file": ([C.c_char_p, i, i, i, vp, vp, vp, vp, d, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+                             i),
         "cgfd_slice_open": ([C.c_char_p, i, i, i, i, vp, C.POINTER(vp)], i),
         "cgfd_slice_append": ([vp, C.c_float, vp], i),
         "cgfd_slice_close": ([vp], i),
@@ -843,6 +847,49 @@ def session_slice(sess, shot, axis, gindex):
     start[axis], count[axis] = gindex, 1
     w = sess.fetch_subvolume(shot, LEVEL_PRE, list(range(9)), tuple(start), tuple(count))
     return np.squeeze(w, axis=3 - axis)
+
+
+class SourceFileInfo(C.Structure):
+    """cgfd_source_file_info (include/cgfd_setup.h)."""
+    _fields_ = [("event", C.c_char * 256), ("nsrc_file", C.c_int), ("nsrc", C.c_int), ("stf_given", C.c_int),
+                ("stf_length", C.c_double), ("stf_dt", C.c_double), ("stf_nt", C.c_int), ("cmp_type", C.c_int),
+                ("mechanism", C.c_int), ("by_coord", C.c_int), ("depth", C.c_int), ("force_on", C.c_int),
+                ("moment_on", C.c_int), ("max_nt", C.c_int)]
+
+
+def source_file(path, ni, nj, nk, t0, dt, x=None, y=None, z=None, mu=None):
+    """A source file (src_read_locate_file) as (info dict, source dict for
+    Session.upload_source, node [nsrc][3], shift [nsrc][3])."""
+    load()
+    xs = [_arr(a, np.float64) if a is not None else None for a in (x, y, z, mu)]
+    xp = [_ptr(a) for a in xs]
+    info = SourceFileInfo()
+    r = _lib.cgfd_source_file(str(path).encode(), ni, nj, nk, *xp, float(t0), float(dt), C.byref(info),
+                              *([None] * 9))
+    if r < 0:
+        _check(r)
+    ns, mnt = info.nsrc, info.max_nt
+    itb = np.zeros(max(ns, 1), np.int32)
+    ite = np.zeros(max(ns, 1), np.int32)
+    en = np.zeros(max(ns, 1), np.int32)
+    ei = np.zeros((max(ns, 1), 343), np.int64)
+    ec = np.zeros((max(ns, 1), 343))
+    F = np.zeros((3, max(ns, 1), max(mnt, 1), 4)) if info.force_on else None
+    M = np.zeros((6, max(ns, 1), max(mnt, 1), 4)) if info.moment_on else None
+    node = np.zeros((max(ns, 1), 3), np.int32)
+    shift = np.zeros((max(ns, 1), 3))
+    r = _lib.cgfd_source_file(str(path).encode(), ni, nj, nk, *xp, float(t0), float(dt), C.byref(info),
+                              _ptr(itb), _ptr(ite), _ptr(en), _ptr(ei), _ptr(ec), _ptr(F), _ptr(M), _ptr(node),
+                              _ptr(shift))
+    if r < 0:
+        _check(r)
+    d = {k: getattr(info, k) for k, _ in SourceFileInfo._fields_}
+    d["event"] = info.event.decode()
+    src = dict(nsrc=ns, force_on=info.force_on, moment_on=info.moment_on, max_nt=mnt, max_ext=343,
+               it_begin=itb[:ns], it_end=ite[:ns], ext_num=en[:ns], ext_indx=ei[:ns].ravel(),
+               ext_coef=ec[:ns].ravel(), F=None if F is None else F[:, :ns].ravel(),
+               M=None if M is None else M[:, :ns].ravel())
+    return d, src, node[:ns], shift[:ns]
 
 
 class Station(C.Structure):

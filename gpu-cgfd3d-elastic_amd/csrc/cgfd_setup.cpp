@@ -1158,6 +1158,13 @@ double cgfd_wavelet(int kind, double t, double c0, double c1) {
     }
     case CGFD_WAVELET_GAUSSIAN_DERIV:
       return std::exp(-tau * tau / (c0 * c0)) / (std::sqrt(M_PI) * c0) * (-2 * tau / (c0 * c0));
+    case CGFD_WAVELET_KLAUDER: {
+      // linear sweep f1..f2 over T starting t0 = 0.2 s (the reference's fixed
+      // parameters; c0, c1 unused); singular at t = t0 as there
+      const double ts = 0.2, f1 = 1.5, f2 = 41.5, T = 20.0;
+      const double K = (f2 - f1) / T, fm = (f2 + f1) / 2.0;
+      return std::sin(M_PI * K * t * (T - (t - ts))) * std::cos(2 * M_PI * fm * (t - ts)) / (M_PI * K * (t - ts));
+    }
     default:
       return 0.0;
   }
@@ -1183,18 +1190,15 @@ int cgfd_angle2moment(double strike, double dip, double rake, double *m6) {
   return CGFD_OK;
 }
 
-int cgfd_source_point(int ni, int nj, int nk, double sx, double sy, double sz, int depth_flag, double t0, double dt,
-                      double wavelet_tstart, int wavelet, double c0, double c1, double stf_length, int *it_begin,
-                      int *it_end, int *ext_num, long long *ext_indx, double *ext_coef, int *max_nt_out,
-                      double *stf) {
-  if (ni < 1 || nj < 1 || nk < 1 || dt <= 0 || !it_begin || !it_end || !ext_num || !ext_indx || !ext_coef ||
-      !max_nt_out)
-    return CGFD_E_ARG;
-  G3 g(ni, nj, nk);
-  if (depth_flag == 1) sz = (double)(nk - 1) - sz;
-  const int si = (int)(sx + 0.5), sj = (int)(sy + 0.5), sk = (int)(sz + 0.5);
-  if (si < 0 || si > ni - 1 || sj < 0 || sj > nj - 1 || sk < 0 || sk > nk - 1) return CGFD_E_ARG;
-  const double ox = sx - si, oy = sy - sj, oz = sz - sk;
+namespace {
+
+// Gaussian spreading of a point source over the 7^3 points around node
+// (si, sj, sk), the point sitting at (ox, oy, oz) from it: width 1.5 points,
+// weights normalised over all 343 before those outside the physical grid
+// are dropped (src_cal_norm_delt3d and its use, src_t.cu:614-654).
+// Returns the number of points kept.
+int spread_gauss(const G3 &g, int si, int sj, int sk, double ox, double oy, double oz, long long *ext_indx,
+                 double *ext_coef) {
   const int half = 3;
   const double wid = half / 2.0;
   double w[343], sum = 0;
@@ -1211,13 +1215,29 @@ int cgfd_source_point(int ni, int nj, int nk, double sx, double sy, double sz, i
     for (int j = -half; j <= half; j++)
       for (int i = -half; i <= half; i++) {
         const int gi = si + i, gj = sj + j, gk = sk + k;
-        if (gi < 0 || gi > ni - 1 || gj < 0 || gj > nj - 1 || gk < 0 || gk > nk - 1) continue;
+        if (gi < 0 || gi > g.ni - 1 || gj < 0 || gj > g.nj - 1 || gk < 0 || gk > g.nk - 1) continue;
         const int c = (i + half) + 7 * (j + half) + 49 * (k + half);
         ext_indx[ne] = g.at(gi + NG, gj + NG, gk + NG);
         ext_coef[ne] = w[c] / sum;
         ne++;
       }
-  *ext_num = ne;
+  return ne;
+}
+
+}  // namespace
+
+int cgfd_source_point(int ni, int nj, int nk, double sx, double sy, double sz, int depth_flag, double t0, double dt,
+                      double wavelet_tstart, int wavelet, double c0, double c1, double stf_length, int *it_begin,
+                      int *it_end, int *ext_num, long long *ext_indx, double *ext_coef, int *max_nt_out,
+                      double *stf) {
+  if (ni < 1 || nj < 1 || nk < 1 || dt <= 0 || !it_begin || !it_end || !ext_num || !ext_indx || !ext_coef ||
+      !max_nt_out)
+    return CGFD_E_ARG;
+  G3 g(ni, nj, nk);
+  if (depth_flag == 1) sz = (double)(nk - 1) - sz;
+  const int si = (int)(sx + 0.5), sj = (int)(sy + 0.5), sk = (int)(sz + 0.5);
+  if (si < 0 || si > ni - 1 || sj < 0 || sj > nj - 1 || sk < 0 || sk > nk - 1) return CGFD_E_ARG;
+  *ext_num = spread_gauss(g, si, sj, sk, sx - si, sy - sj, sz - sk, ext_indx, ext_coef);
   const int max_nt = (int)(stf_length / dt + 0.5);
   *max_nt_out = max_nt;
   const int ib = (int)((wavelet_tstart - t0) / dt);
@@ -1530,43 +1550,55 @@ bool surface_z(const FGrid &G, float sx, float sy, float *zs) {
 
 }  // namespace
 
+// One point: depth -> z where asked (the z used is returned in *zout), then
+// the nearest physical node m (0-based) and the signed shift of the point
+// from it (gd_curv_coord_to_glob_indx_gpu, gd_t.cu:1190-1392); false when
+// the point is outside the grid.
+static bool locate_raw(const FGrid &G, float sx, float sy, float sz, bool depth, int *m, float *inc, float *zout) {
+  const int ni = G.g.ni, nj = G.g.nj, nk = G.g.nk;
+  // depth -> z; as in the reference, a point under no surface quad keeps
+  // its depth value as z (and is then normally outside the grid)
+  float zs;
+  if (depth && sx >= G.xmin && sx <= G.xmax && sy >= G.ymin && sy <= G.ymax && surface_z(G, sx, sy, &zs))
+    sz = zs - sz;
+  if (zout) *zout = sz;
+  for (int d = 0; d < 3; d++) { m[d] = -1000; inc[d] = 0.0f; }
+  if (sx < G.xmin || sx > G.xmax || sy < G.ymin || sy > G.ymax || sz < G.zmin || sz > G.zmax) return false;
+  const long p = G.nearest(sx, sy, sz);
+  const int mi = (int)(p % G.g.sy), mj = (int)((p / G.g.sy) % G.g.ny), mk = (int)(p / G.g.sz);
+  if (mi < NG || mi > NG + ni - 1 || mj < NG || mj > NG + nj - 1 || mk < NG || mk > NG + nk - 1) return false;
+  for (int c = 0; c < 8; c++) {
+    const int ci = mi - 1 + (c & 1), cj = mj - 1 + ((c >> 1) & 1), ck = mk - 1 + (c >> 2);
+    float vx[8], vy[8], vz[8];
+    cell_corners(G, ci, cj, ck, vx, vy, vz);
+    if (!in_hexahedron(sx, sy, sz, vx, vy, vz)) continue;
+    int s1, s2, s3;
+    cell_sample(sx, sy, sz, vx, vy, vz, &s1, &s2, &s3);
+    inc[0] = ((float)ci + (float)s1 / 100.0f) - (float)mi;
+    inc[1] = ((float)cj + (float)s2 / 100.0f) - (float)mj;
+    inc[2] = ((float)ck + (float)s3 / 100.0f) - (float)mk;
+    break;
+  }
+  m[0] = mi - NG; m[1] = mj - NG; m[2] = mk - NG;
+  return true;
+}
+
 int cgfd_locate_points(int ni, int nj, int nk, const double *x, const double *y, const double *z, int n,
                        const double *pts, const int *depth_flag, int *idx, double *frac, int *found) {
   if (ni < 2 || nj < 2 || nk < 2 || !x || !y || !z || n < 0 || (n > 0 && (!pts || !idx || !frac || !found)))
     return CGFD_E_ARG;
   const FGrid G(ni, nj, nk, x, y, z);
   par_for(n, [&](int q) {
-    float sx = (float)pts[3 * q], sy = (float)pts[3 * q + 1], sz = (float)pts[3 * q + 2];
-    found[q] = 0;
+    int m[3];
+    float inc[3];
+    found[q] = locate_raw(G, (float)pts[3 * q], (float)pts[3 * q + 1], (float)pts[3 * q + 2],
+                          depth_flag && depth_flag[q], m, inc, nullptr) ? 1 : 0;
     for (int d = 0; d < 3; d++) {
       idx[3 * q + d] = -1000;
       frac[3 * q + d] = 0.0;
     }
-    // depth -> z; as in the reference, a point under no surface quad keeps
-    // its depth value as z (and is then normally outside the grid)
-    float zs;
-    if (depth_flag && depth_flag[q] && sx >= G.xmin && sx <= G.xmax && sy >= G.ymin && sy <= G.ymax &&
-        surface_z(G, sx, sy, &zs))
-      sz = zs - sz;
-    if (sx < G.xmin || sx > G.xmax || sy < G.ymin || sy > G.ymax || sz < G.zmin || sz > G.zmax) return;
-    const long p = G.nearest(sx, sy, sz);
-    const int mi = (int)(p % G.g.sy), mj = (int)((p / G.g.sy) % G.g.ny), mk = (int)(p / G.g.sz);
-    if (mi < NG || mi > NG + ni - 1 || mj < NG || mj > NG + nj - 1 || mk < NG || mk > NG + nk - 1) return;
-    found[q] = 1;
-    float inc[3] = {0.0f, 0.0f, 0.0f};
-    for (int c = 0; c < 8; c++) {
-      const int ci = mi - 1 + (c & 1), cj = mj - 1 + ((c >> 1) & 1), ck = mk - 1 + (c >> 2);
-      float vx[8], vy[8], vz[8];
-      cell_corners(G, ci, cj, ck, vx, vy, vz);
-      if (!in_hexahedron(sx, sy, sz, vx, vy, vz)) continue;
-      int s1, s2, s3;
-      cell_sample(sx, sy, sz, vx, vy, vz, &s1, &s2, &s3);
-      inc[0] = ((float)ci + (float)s1 / 100.0f) - (float)mi;
-      inc[1] = ((float)cj + (float)s2 / 100.0f) - (float)mj;
-      inc[2] = ((float)ck + (float)s3 / 100.0f) - (float)mk;
-      break;
-    }
-    const int m[3] = {mi - NG, mj - NG, mk - NG};
+    if (!found[q]) return;
+    // negative shifts move to the lower cell (io_recv_read_locate, io_funcs.cu:156-170)
     for (int d = 0; d < 3; d++) {
       int id = m[d];
       float f = inc[d];
@@ -1692,6 +1724,261 @@ int cgfd_station_file(const char *path, int ni, int nj, int nk, const double *x,
   }
   for (int r = 0; r < n; r++) out[r] = st[(size_t)r];
   return n;
+}
+
+// ---- source files (src_read_locate_file, src_t.cu:143-795) -------------------------
+namespace {
+
+// piecewise Lagrange interpolation of order `order` on the uniform samples
+// t[i] = t_start + i dt (LagInterp_Piecewise_1d, interp.cu:220-298): the
+// order+1 samples around the nearest one, shifted inside at the ends
+double lag_piecewise(const std::vector<double> &t, const std::vector<double> &v, int order, double t_start,
+                     double dt, double ti) {
+  const int n = (int)t.size();
+  int lower, upper;
+  if (order % 2 == 0) { lower = order / 2; upper = order / 2; }
+  else { lower = order / 2; upper = order / 2 + 1; }
+  int indx = (int)((ti - t_start) / dt + 0.5);
+  if (indx > n - 1) indx = n - 1;
+  int a, b;
+  if (indx - lower >= 0 && indx + upper <= n - 1) { a = indx - lower; b = indx + upper; }
+  else if (indx - lower < 0) { a = 0; b = indx + upper + (lower - indx); }
+  else { a = indx - lower - (indx + upper - (n - 1)); b = n - 1; }
+  a = std::max(a, 0);
+  b = std::min(b, n - 1);
+  double li = 0.0;
+  for (int i = a; i <= b; i++) {
+    double l = 1.0;
+    for (int j = a; j <= b; j++)
+      if (j != i) l *= (ti - t[j]) / (t[i] - t[j]);
+    li += l * v[i];
+  }
+  return li;
+}
+
+int wavelet_kind(const char *name) {
+  if (!std::strcmp(name, "ricker")) return CGFD_WAVELET_RICKER;
+  if (!std::strcmp(name, "gaussian")) return CGFD_WAVELET_GAUSSIAN;
+  if (!std::strcmp(name, "ricker_deriv")) return CGFD_WAVELET_RICKER_DERIV;
+  if (!std::strcmp(name, "gaussian_deriv")) return CGFD_WAVELET_GAUSSIAN_DERIV;
+  if (!std::strcmp(name, "klauder")) return CGFD_WAVELET_KLAUDER;
+  return -1;
+}
+
+// strike, dip, rake, mu, D, A -> tensor mu D A * unit double couple (src_muDA_to_moment, src_t.cu:995-1012)
+void muda_to_moment(double strike, double dip, double rake, double mu, double D, double A, double *m6) {
+  double u[6];
+  cgfd_angle2moment(strike, dip, rake, u);
+  const double m0 = mu * D * A;
+  for (int c = 0; c < 6; c++) m6[c] = m0 * u[c];
+}
+
+}  // namespace
+
+int cgfd_source_file(const char *path, int ni, int nj, int nk, const double *x, const double *y, const double *z,
+                     const double *mu, double t0, double dt, cgfd_source_file_info *info, int *it_begin,
+                     int *it_end, int *ext_num, long long *ext_indx, double *ext_coef, double *F, double *M,
+                     int *node, double *shift) {
+  if (!path || !info || ni < 2 || nj < 2 || nk < 2 || !(dt > 0)) return CGFD_E_ARG;
+  auto bad = [&](const std::string &m) {
+    cgfd_internal_set_error(("source file " + std::string(path) + ": " + m).c_str());
+    return CGFD_E_ARG;
+  };
+  FILE *fp = std::fopen(path, "rt");
+  if (!fp) return bad("cannot open");
+  struct Closer { FILE *f; ~Closer() { std::fclose(f); } } closer{fp};
+  char line[500];
+  cgfd_source_file_info I;
+  std::memset(&I, 0, sizeof(I));
+  if (!next_line(fp, line, sizeof(line))) return bad("no event name");
+  std::snprintf(I.event, sizeof(I.event), "%s", line);
+  if (!next_line(fp, line, sizeof(line)) || std::sscanf(line, "%d", &I.nsrc_file) != 1 || I.nsrc_file <= 0)
+    return bad("the source count must be positive");
+  float fa = 0, fb = 0;
+  int na = 1;
+  if (!next_line(fp, line, sizeof(line)) || std::sscanf(line, "%d", &I.stf_given) != 1) return bad("no time-function line");
+  if (I.stf_given == 0) {
+    std::sscanf(line, "%d %f", &I.stf_given, &fa);
+    I.stf_length = fa;
+  } else if (I.stf_given == 1) {
+    std::sscanf(line, "%d %f %d", &I.stf_given, &fb, &na);
+    I.stf_dt = fb;
+    I.stf_nt = na;
+    if (na < 1 || !(fb > 0)) return bad("sampled time function needs dt > 0 and nt >= 1");
+  } else {
+    return bad("time function given by name (0) or values (1)");
+  }
+  if (!next_line(fp, line, sizeof(line)) || std::sscanf(line, "%d %d", &I.cmp_type, &I.mechanism) != 2)
+    return bad("no component line");
+  if (I.cmp_type < 1 || I.cmp_type > 3) return bad("component type 1, 2 or 3");
+  if (!next_line(fp, line, sizeof(line)) || std::sscanf(line, "%d %d", &I.by_coord, &I.depth) != 2)
+    return bad("no location line");
+  const int ns = I.nsrc_file;
+  std::vector<float> loc(3 * (size_t)ns);
+  for (int q = 0; q < ns; q++)
+    if (!next_line(fp, line, sizeof(line)) ||
+        std::sscanf(line, "%f %f %f", &loc[3 * (size_t)q], &loc[3 * (size_t)q + 1], &loc[3 * (size_t)q + 2]) != 3)
+      return bad("location line " + std::to_string(q + 1) + " unreadable");
+  // locate: nearest node and signed shift
+  const G3 g(ni, nj, nk);
+  std::vector<int> nd(3 * (size_t)ns, -1000);
+  std::vector<double> sh(3 * (size_t)ns, 0.0);
+  std::vector<char> here((size_t)ns, 0);
+  if (I.by_coord == 1) {
+    if (!x || !y || !z) return bad("locations by coordinates need the grid coordinates");
+    const FGrid G(ni, nj, nk, x, y, z);
+    for (int q = 0; q < ns; q++) {
+      int m[3];
+      float inc[3];
+      if (locate_raw(G, loc[3 * (size_t)q], loc[3 * (size_t)q + 1], loc[3 * (size_t)q + 2], I.depth == 1, m, inc,
+                     nullptr))
+        for (int d = 0; d < 3; d++) { nd[3 * (size_t)q + d] = m[d]; sh[3 * (size_t)q + d] = inc[d]; }
+    }
+  } else {
+    const int tot[3] = {ni, nj, nk};
+    for (int q = 0; q < ns; q++) {
+      float v[3] = {loc[3 * (size_t)q], loc[3 * (size_t)q + 1], loc[3 * (size_t)q + 2]};
+      if (I.depth == 1) v[2] = (float)(nk - 1) - v[2];
+      for (int d = 0; d < 3; d++) {
+        const int id = (int)(v[d] + 0.5f);
+        sh[3 * (size_t)q + d] = v[d] - (float)id;
+        nd[3 * (size_t)q + d] = (id < 0 || id > tot[d] - 1) ? -1000 : id;
+      }
+    }
+  }
+  // kept: the node inside the grid (the 7^3 spreading then meets it)
+  int kept = 0;
+  for (int q = 0; q < ns; q++) {
+    const int *m = &nd[3 * (size_t)q];
+    here[(size_t)q] = m[0] >= 0 && m[1] >= 0 && m[2] >= 0;
+    kept += here[(size_t)q];
+  }
+  I.nsrc = kept;
+  if (kept > 0) {
+    if (I.cmp_type == 1 || I.cmp_type == 3) I.force_on = 1;
+    else I.moment_on = 1;  // cmp_type 2 (3 never reaches here, as in the reference)
+  }
+  I.max_nt = I.stf_given == 0 ? (int)(I.stf_length / dt + 0.5) : (int)((I.stf_nt - 1) * I.stf_dt / dt + 0.5) + 1;
+  *info = I;
+  if (!it_begin && !it_end && !ext_num && !ext_indx && !ext_coef && !F && !M) return CGFD_OK;
+  if (!it_begin || !it_end || !ext_num || !ext_indx || !ext_coef || (I.force_on && !F) || (I.moment_on && !M))
+    return bad("table arrays missing");
+  const int mnt = I.max_nt;
+  const double stage[4] = {0.0, 0.5, 0.5, 1.0};
+  // values that persist from one source to the next, as in the reference
+  double coefs[10] = {0}, wstart = 0;
+  int wkind = -1;
+  double fc[3] = {0, 0, 0}, mc[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<double> samp[9];
+  for (auto &v : samp) v.assign((size_t)std::max(I.stf_nt, 1), 0.0);
+  int ks = 0;
+  for (int q = 0; q < ns; q++) {
+    double tstart = 0;
+    int kind = -1;
+    const int *m = &nd[3 * (size_t)q];
+    const long long pnode = here[(size_t)q] ? g.at(m[0] + NG, m[1] + NG, m[2] + NG) : 0;
+    auto mu_at = [&](double v) -> double {
+      if (v >= 0.0) return v;
+      if (!mu) return std::nan("");
+      return mu[pnode];
+    };
+    auto read_cmp = [&](const char *ln, double *f3, double *m6) {
+      float v[9];
+      int got;
+      if (I.cmp_type == 1) {
+        got = std::sscanf(ln, "%f %f %f", &v[0], &v[1], &v[2]);
+        for (int c = 0; c < got; c++) f3[c] = v[c];
+      } else if (I.cmp_type == 2) {
+        got = std::sscanf(ln, "%f %f %f %f %f %f", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5]);
+        for (int c = 0; c < got; c++) m6[c] = v[c];
+      } else {
+        got = std::sscanf(ln, "%f %f %f %f %f %f %f %f %f", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7],
+                          &v[8]);
+        for (int c = 0; c < got; c++) (c < 3 ? f3[c] : m6[c - 3]) = v[c];
+      }
+    };
+    if (I.stf_given == 0) {
+      if (!next_line(fp, line, sizeof(line))) return bad("time-function line of source " + std::to_string(q + 1));
+      // parsed only for a kept source: the start time and coefficients
+      // persist from the last kept source, as in the reference
+      if (here[(size_t)q]) {
+        char name[500] = {0};
+        float ts = 0, c[10];
+        const int got = std::sscanf(line, "%f %499s %f %f %f %f %f %f %f %f %f %f", &ts, name, &c[0], &c[1], &c[2],
+                                    &c[3], &c[4], &c[5], &c[6], &c[7], &c[8], &c[9]);
+        if (got < 2) return bad("time-function line of source " + std::to_string(q + 1));
+        for (int i = 0; i + 2 < got; i++) coefs[i] = c[i];
+        wstart = ts;
+        wkind = wavelet_kind(name);
+        if (wkind < 0) return bad(std::string("wavelet ") + name + " not implemented");
+      }
+      tstart = wstart;
+      kind = wkind;
+      if (!next_line(fp, line, sizeof(line))) return bad("component line of source " + std::to_string(q + 1));
+      if (here[(size_t)q]) {
+        read_cmp(line, fc, mc);
+        if (I.moment_on && I.mechanism == 1) muda_to_moment(mc[0], mc[1], mc[2], mu_at(mc[3]), mc[4], mc[5], mc);
+      }
+    } else {
+      if (!next_line(fp, line, sizeof(line))) return bad("start time of source " + std::to_string(q + 1));
+      float ts = 0;
+      std::sscanf(line, "%f", &ts);
+      tstart = ts;
+      for (int it = 0; it < I.stf_nt; it++) {
+        if (!next_line(fp, line, sizeof(line))) return bad("time-function samples of source " + std::to_string(q + 1));
+        if (!here[(size_t)q]) continue;
+        double f3[3], m6[6];
+        for (int c = 0; c < 3; c++) f3[c] = samp[c][(size_t)it];
+        for (int c = 0; c < 6; c++) m6[c] = samp[3 + c][(size_t)it];
+        read_cmp(line, f3, m6);
+        if (I.moment_on && I.mechanism == 1)
+          muda_to_moment(m6[0], m6[1], m6[2], mu_at(m6[3]), m6[4], m6[5], m6);
+        for (int c = 0; c < 3; c++) samp[c][(size_t)it] = f3[c];
+        for (int c = 0; c < 6; c++) samp[3 + c][(size_t)it] = m6[c];
+      }
+    }
+    if (!here[(size_t)q]) continue;
+    // spreading, start step, per-(step, stage) values
+    ext_num[ks] = spread_gauss(g, m[0], m[1], m[2], sh[3 * (size_t)q], sh[3 * (size_t)q + 1], sh[3 * (size_t)q + 2],
+                               ext_indx + 343 * (size_t)ks, ext_coef + 343 * (size_t)ks);
+    const int ib = (int)((tstart - t0) / dt);
+    it_begin[ks] = ib;
+    it_end[ks] = ib + mnt - 1;
+    const double t_shift = tstart - (ib * dt + t0);
+    std::vector<double> tin;
+    if (I.stf_given == 1) {
+      tin.resize((size_t)I.stf_nt);
+      for (int it = 0; it < I.stf_nt; it++) tin[(size_t)it] = tstart + it * I.stf_dt;
+    }
+    for (int it = ib; it <= ib + mnt - 1; it++) {
+      const int r = it - ib;
+      for (int st = 0; st < 4; st++) {
+        const size_t o = ((size_t)ks * mnt + r) * 4 + st;
+        double fv[3], mv[6];
+        if (I.stf_given == 0) {
+          const double t = r * dt + stage[st] * dt - t_shift;
+          const double v = cgfd_wavelet(kind, t, coefs[0], coefs[1]);
+          for (int c = 0; c < 3; c++) fv[c] = v * fc[c];
+          for (int c = 0; c < 6; c++) mv[c] = v * mc[c];
+        } else {
+          // the reference evaluates the samples at the absolute step it here
+          const double t = it * dt + stage[st] * dt - t_shift;
+          for (int c = 0; c < 3; c++) fv[c] = lag_piecewise(tin, samp[c], 3, tstart, I.stf_dt, t);
+          for (int c = 0; c < 6; c++) mv[c] = lag_piecewise(tin, samp[3 + c], 3, tstart, I.stf_dt, t);
+        }
+        if (I.force_on)
+          for (int c = 0; c < 3; c++) F[(size_t)c * kept * mnt * 4 + o] = fv[c];
+        if (I.moment_on)
+          for (int c = 0; c < 6; c++) M[(size_t)c * kept * mnt * 4 + o] = mv[c];
+      }
+    }
+    if (node)
+      for (int d = 0; d < 3; d++) node[3 * ks + d] = m[d];
+    if (shift)
+      for (int d = 0; d < 3; d++) shift[3 * ks + d] = sh[3 * (size_t)q + d];
+    ks++;
+  }
+  return CGFD_OK;
 }
 
 int cgfd_storage_scales(int storage, int ni, int nj, int nk, const double *metric, const double *lam,

@@ -175,6 +175,7 @@ int cgfd_free_surface_matrices(int ni, int nj, int nk, const double *metric, con
 #define CGFD_WAVELET_GAUSSIAN 1
 #define CGFD_WAVELET_RICKER_DERIV 2
 #define CGFD_WAVELET_GAUSSIAN_DERIV 3
+#define CGFD_WAVELET_KLAUDER 4 /* fixed 0.2 s, 1.5-41.5 Hz, 20 s sweep (fun_klauder, src_t.cu:869-876) */
 
 /* Wavelet value (fun_ricker & co, src/forward/src_t.cu:834-878). */
 double cgfd_wavelet(int kind, double t, double c0, double c1);
@@ -194,6 +195,48 @@ int cgfd_source_point(int ni, int nj, int nk, double sx, double sy, double sz, i
                       double wavelet_tstart, int wavelet, double c0, double c1, double stf_length, int *it_begin,
                       int *it_end, int *ext_num, long long *ext_indx, double *ext_coef, int *max_nt,
                       double *stf);
+
+/* What a source file holds (src_read_locate_file, src/forward/src_t.cu:143-795). */
+typedef struct {
+  char event[256];     /* event name: the first line */
+  int nsrc_file;       /* sources the file lists */
+  int nsrc;            /* sources kept: located inside the physical grid */
+  int stf_given;       /* time function by wavelet name (0) or by sampled values (1) */
+  double stf_length;   /* by name: length in seconds */
+  double stf_dt;       /* by values: sample interval */
+  int stf_nt;          /* by values: samples */
+  int cmp_type;        /* 1 force, 2 moment, 3 force and moment (the reference then activates the force only) */
+  int mechanism;       /* moment as a tensor (0) or strike, dip, rake, mu, D, A (1) */
+  int by_coord;        /* locations by grid index (0) or by coordinates (1) */
+  int depth;           /* the third location value is a depth below the free surface (1) */
+  int force_on, moment_on;
+  int max_nt;          /* time-function samples per source (the tables' step count) */
+} cgfd_source_file_info;
+
+/* Source file (src_read_locate_file, src/forward/src_t.cu:143-795; lines read as
+ * io_get_nextline does): event name, source count, time-function line ("0
+ * length" or "1 dt nt"), component line ("cmp_type mechanism"), location
+ * line ("by_coord depth"), one location per source, then per source either
+ * "t_start wavelet_name coef..." and one component line, or "t_start" and
+ * nt component lines.  Locations by index are rounded to the nearest node
+ * (0-based, depth counted down from the surface); by coordinates they are
+ * located as cgfd_locate_points does, keeping the nearest node and a signed
+ * shift.  Component values missing from a line keep the previous source's
+ * values (zero at first), as in the reference; strike/dip/rake/mu/D/A become
+ * the tensor mu D A * cgfd_angle2moment (mu < 0: the model mu at the source
+ * node, from `mu` in the reference layout).  Sampled time functions are
+ * interpolated with the reference's piecewise cubic Lagrange rule
+ * (LagInterp_Piecewise_1d, src/forward/interp.cu:220-298).
+ * Call with all table pointers NULL to get *info (sizes); then with arrays
+ * of info->nsrc entries (it_begin, it_end, ext_num), nsrc*343 (ext_indx,
+ * ext_coef), 3*nsrc*max_nt*4 (F, when force_on) and 6*nsrc*max_nt*4 (M, when
+ * moment_on), the layout of cgfd_source_desc.  node[nsrc][3] (0-based
+ * physical node) and shift[nsrc][3] may be NULL.  x, y, z are needed for
+ * locations by coordinates only. */
+int cgfd_source_file(const char *path, int ni, int nj, int nk, const double *x, const double *y, const double *z,
+                     const double *mu, double t0, double dt, cgfd_source_file_info *info, int *it_begin,
+                     int *it_end, int *ext_num, long long *ext_indx, double *ext_coef, double *F, double *M,
+                     int *node, double *shift);
 
 /* ---- receivers ------------------------------------------------------------------- */
 
