@@ -41,6 +41,16 @@ import workloads  # noqa: E402
 
 METRIC = "Gcell-RK-stages/s el_iso FP32 vs FP16-store; served shots/hour, p50/p99"
 HBM_PEAK_GBS = 8000.0
+# work splits of a stage (cgfd_get_split): profile names, the swept kernel(s) and the path
+SPLIT_NAMES = {cgfd.SPLIT_PLANES: "planes", cgfd.SPLIT_SHELL: "shell", cgfd.SPLIT_CORE: "core"}
+SWEEP_KERNEL = {
+    cgfd.SPLIT_PLANES: "k_march (z-marching sweep, whole planes with the PML slabs)",
+    cgfd.SPLIT_SHELL: "k_march (z-marching sweep, inside the PML slabs)",
+    cgfd.SPLIT_CORE: "k_march (z-marching sweep, whole planes: PML-free core launch + PML-form rim launch, "
+                     "timed together per stage)",
+}
+KERNEL_PATH = {cgfd.SPLIT_PLANES: "zmarch planes", cgfd.SPLIT_SHELL: "zmarch + point shell",
+               cgfd.SPLIT_CORE: "zmarch core + rim"}
 # The roofline unit: algorithmic HBM bytes per interior cell and RK stage of
 # the reference's dataflow with the RK update fused, mean of 39/57/57/39
 # words (wavefield in, 12 coefficients, RK levels in and out).
@@ -127,7 +137,7 @@ def measured_traffic(workload, storage, split, tile):
     run uses wins; otherwise the line of the same thread shape with the
     nearest z chunk (the chunk changes only the priming planes), and the
     profiled tile is reported beside the figure."""
-    name = "planes" if split == cgfd.SPLIT_PLANES else "shell"
+    name = SPLIT_NAMES[split]
     path = os.path.join(ROOT, "profiles", f"k_march_{workload}_{storage}_{name}_hbm_counters.txt")
     if not os.path.exists(path):
         return None, None
@@ -311,7 +321,7 @@ def scales_for(m, st, src=None):
                                m["src"] if src is None else src, m["dt"])
 
 
-def timed_session(m, storage, warmup, steps, launch_steps, flags=0, tile=""):
+def timed_session(m, storage, warmup, steps, launch_steps, flags=0, tile="", split=""):
     """Upload m into a single-shot session, run warmup steps, time `steps`
     steps (8-step periods replayed as captured graphs), then a short pass with
     HIP events around every sweep launch.  Returns a dict of the measurements."""
@@ -321,6 +331,8 @@ def timed_session(m, storage, warmup, steps, launch_steps, flags=0, tile=""):
     workloads.upload(sess, m, nt_total=nt_total)
     if st == cgfd.STORE_F16:
         sess.set_storage_scale(scales_for(m, st))
+    if split:
+        sess.set_split({v: k for k, v in SPLIT_NAMES.items()}[split])
     if tile:
         sh, zc = (int(v) for v in tile.split(","))
         sess.set_tile(sh, zc)
@@ -361,8 +373,7 @@ def roofline(m, storage, r, stream):
     tile = r["tile"]
     roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
                 traffic=None,
-                kernel="k_march (z-marching sweep" + (", inside the PML slabs)" if split == cgfd.SPLIT_SHELL
-                                                      else ", whole planes with the PML slabs)"),
+                kernel=SWEEP_KERNEL[split],
                 bytes_per_point_stage=unit, swept_points=cells, avg_launch_ms=round(r["int_avg_ms"], 5),
                 algorithmic_bytes_per_launch=unit * cells,
                 tile=f"{tile['tx']}x{tile['ty']} threads, {tile['zchunk']} planes",
@@ -462,7 +473,7 @@ def topo_leg(steps, warmup, stream):
         v = cells * 4 * steps / r["wall"] / 1e9
         out[name] = dict(value=round(v, 4), unit="Gcell-RK-stages/s", ms_per_step=round(r["wall"] * 1e3 / steps, 4),
                          steps=steps, seismograms_finite=r["finite"],
-                         split="planes" if r["split"] == cgfd.SPLIT_PLANES else "shell")
+                         split=SPLIT_NAMES[r["split"]])
         if name == "f32":
             out[name]["roofline"] = roofline(m, name, r, stream)
         log(f"topography {name}: {v:.3f} Gcell-RK-stages/s")
@@ -487,6 +498,8 @@ def main():
     ap.add_argument("--full-metric", action="store_true", help="keep the per-point metric streams")
     ap.add_argument("--tile", default="", metavar="SHAPE,ZCHUNK",
                     help="pin the interior tile instead of autotuning (profiling runs)")
+    ap.add_argument("--split", default="", choices=["", "planes", "core", "shell"],
+                    help="pin the work split (default: the autotuner chooses)")
     ap.add_argument("--timed-launch-steps", type=int, default=8,
                     help="steps of the per-launch event timing pass (0: skip)")
     ap.add_argument("--error-steps", type=int, default=320,
@@ -533,7 +546,7 @@ def main():
     flags = (cgfd.FLAG_PLAIN if args.plain else 0) | (cgfd.FLAG_PML_POINT if args.pml_point else 0) | \
         (cgfd.FLAG_FULL_METRIC if args.full_metric else 0)
     log(f"timing {args.steps} steps ({args.warmup} warmup)")
-    r = timed_session(m, args.storage, args.warmup, args.steps, args.timed_launch_steps, flags, args.tile)
+    r = timed_session(m, args.storage, args.warmup, args.steps, args.timed_launch_steps, flags, args.tile, args.split)
     cells = workloads.cells(m)
     value = cells * 4 * args.steps / r["wall"] / 1e9
     log(f"{value:.3f} Gcell-RK-stages/s wall, device {r['device_ms']:.1f} ms, sweep avg {r['int_avg_ms']:.4f} ms "
@@ -599,8 +612,7 @@ def main():
         "config": {"workload": f"{m['name']}: {m['ni']}x{m['nj']}x{m['nk']} el_iso RK4, {desc}, free surface + "
                                f"10-layer CFS-PML, dt {m['dt']}",
                    "cells": cells, "storage": args.storage,
-                   "kernel_path": "plain" if args.plain else ("zmarch planes" if r["split"] == cgfd.SPLIT_PLANES
-                                                              else "zmarch + point shell"),
+                   "kernel_path": "plain" if args.plain else KERNEL_PATH[r["split"]],
                    "seismograms_finite": r["finite"], "device_ms": round(r["device_ms"], 2)},
         "roofline": roof,
         "reduced_storage": precision,

@@ -24,6 +24,7 @@ FLAG_PML_POINT = 2
 FLAG_FULL_METRIC = 4
 SPLIT_SHELL = 0
 SPLIT_PLANES = 1
+SPLIT_CORE = 2
 
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = [
@@ -33,7 +34,7 @@ EXPORTS = [
     "cgfd_upload_receivers", "cgfd_set_level", "cgfd_reset_shot", "cgfd_run",
     "cgfd_el_iso_onestage", "cgfd_read_seismo", "cgfd_fetch_level",
     "cgfd_fetch_pml_aux", "cgfd_set_timing", "cgfd_last_run_timing",
-    "cgfd_set_tile", "cgfd_get_tile", "cgfd_get_split", "cgfd_get_const_metric", "cgfd_set_graph", "cgfd_set_shot",
+    "cgfd_set_tile", "cgfd_get_tile", "cgfd_get_split", "cgfd_set_split", "cgfd_get_const_metric", "cgfd_set_graph", "cgfd_set_shot",
     "cgfd_run_async", "cgfd_poll", "cgfd_wait", "cgfd_reserve_receivers", "cgfd_set_seismo_ring",
     "cgfd_grid_cartesian", "cgfd_grid_topography", "cgfd_metric_calc", "cgfd_dt_estimate",
     "cgfd_media_layered", "cgfd_media_homogeneous", "cgfd_rho_to_slow", "cgfd_pml_len_dh",
@@ -170,6 +171,7 @@ def load(path=LIB_PATH):
         "cgfd_set_tile": ([vp, i, i], i),
         "cgfd_get_tile": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)], i),
         "cgfd_get_split": ([vp, C.POINTER(i)], i),
+        "cgfd_set_split": ([vp, i], i),
         "cgfd_get_const_metric": ([vp, C.POINTER(i)], i),
         "cgfd_last_run_timing": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)], i),
     }
@@ -359,8 +361,12 @@ class Session:
     def set_tile(self, shape, zchunk=48):
         _check(_lib.cgfd_set_tile(self.h, shape, zchunk))
 
+    def set_split(self, split):
+        """Pin the work split (SPLIT_*; -1: automatic)."""
+        _check(_lib.cgfd_set_split(self.h, split))
+
     def get_split(self):
-        """SPLIT_PLANES or SPLIT_SHELL: how stages divide between the kernels."""
+        """SPLIT_PLANES, SPLIT_CORE or SPLIT_SHELL: how stages divide between the kernels."""
         v = C.c_int()
         _check(_lib.cgfd_get_split(self.h, C.byref(v)))
         return v.value

@@ -197,6 +197,7 @@ struct cgfd_session {
   int tile_shape = -1, tile_zchunk = 48;
   bool tile_fixed = false;
   int split = -1;  // CGFD_SPLIT_*, chosen by the autotuner (-1: not yet: planes)
+  bool split_fixed = false;  // cgfd_set_split pinned it
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev;
@@ -651,15 +652,21 @@ StageArgs<TS, TC> stage_args(cgfd_session *s, int slot_in, int slot_pre, int slo
   return a;
 }
 
-// Work split of one stage.  Default: the z-marching kernel sweeps whole x-y
-// planes from the bottom up to the last plane below the free-surface rows,
-// applying the CFS-PML terms inside the face boxes, and one per-point launch
-// takes the 4 free-surface rows.  With CGFD_FLAG_PML_POINT the sweep keeps to
-// the box inside every face box and the per-point launch takes the whole
-// shell (face slabs and surface rows); the per-point launch also covers
-// everything for tiny grids or the plain method.  (Splitting that shell into
-// per-axis single-face launches measured slower: the x slabs are only nlay+1
-// points wide, and four launches add four tails.)
+// Work split of one stage.  CGFD_SPLIT_PLANES: the z-marching kernel sweeps
+// whole x-y planes from the bottom up to the last plane below the
+// free-surface rows, applying the CFS-PML terms inside the face boxes, and one
+// per-point launch takes the 4 free-surface rows.  CGFD_SPLIT_CORE: the same
+// planes in two sweeps, the PML-free form over a core box inside every face
+// box whose x extent starts and ends on tile boundaries (so its rows load
+// whole 128-byte lines), and the PML form over the up to six rim boxes around
+// it; the PML-free form needs far fewer registers (FP32 32x8 tile: 108 / 128
+// / 164 VGPRs against 160 / 168 / 212 in S0 / S1-S2 / S3), so more waves
+// hide its latency.  CGFD_SPLIT_SHELL (or CGFD_FLAG_PML_POINT): the sweep
+// keeps to the box inside every face box and the per-point launch takes the
+// whole shell (face slabs and surface rows); the per-point launch also
+// covers everything for tiny grids or the plain method.  (Splitting that
+// shell into per-axis single-face launches measured slower: the x slabs are
+// only nlay+1 points wide, and four launches add four tails.)
 struct BoxList {
   int n = 0;
   int box[kMaxBox][6];
@@ -674,7 +681,8 @@ struct BoxList {
 };
 
 struct Shell {
-  MarchSet inner;
+  MarchSet inner;  // PML-free sweep (core or shell split) or the whole-plane sweep
+  MarchSet rim;    // core split: PML-form sweep of the boxes around the core
   BoxList point;
 };
 
@@ -700,8 +708,30 @@ Shell make_shell(cgfd_session *s) {
   }
   sh.inner.zchunk = s->tile_zchunk;
   sh.inner.shape = s->tile_shape < 0 ? 0 : s->tile_shape;
+  sh.rim.zchunk = sh.inner.zchunk;
+  sh.rim.shape = sh.inner.shape;
+  sh.rim.pml = 1;
   const bool any_face = ib0 > 0 || ib1 < ni - 1 || jb0 > 0 || jb1 < nj - 1 || kb0 > 0 || kb1 < ktopm;
   if (any_face && !(s->d.flags & CGFD_FLAG_PML_POINT) && s->split != CGFD_SPLIT_SHELL) {
+    if (s->split == CGFD_SPLIT_CORE) {
+      // core: x from the first tile boundary at or past the x1 face box to
+      // the last one before the x2 face box, y likewise on the tile height
+      int tx = 0, ty = 0;
+      interior_shape(sh.inner.shape, &tx, &ty);
+      const int ca0 = (ib0 + tx - 1) / tx * tx, ca1 = ca0 + (ib1 + 1 - ca0) / tx * tx - 1;
+      const int cb0 = (jb0 + ty - 1) / ty * ty, cb1 = cb0 + (jb1 + 1 - cb0) / ty * ty - 1;
+      if (ca1 >= ca0 && cb1 >= cb0) {
+        addm(sh.inner, ca0, ca1, cb0, cb1, kb0, kb1);
+        addm(sh.rim, 0, ni - 1, 0, cb0 - 1, 0, ktopm);          // y1 side, whole x
+        addm(sh.rim, 0, ni - 1, cb1 + 1, nj - 1, 0, ktopm);     // y2 side, whole x
+        addm(sh.rim, 0, ca0 - 1, cb0, cb1, 0, ktopm);           // x1 side
+        addm(sh.rim, ca1 + 1, ni - 1, cb0, cb1, 0, ktopm);      // x2 side
+        addm(sh.rim, ca0, ca1, cb0, cb1, 0, kb0 - 1);           // below the core
+        addm(sh.rim, ca0, ca1, cb0, cb1, kb1 + 1, ktopm);       // above it (no free surface)
+        sh.point.add(0, ni - 1, 0, nj - 1, ktopm + 1, nk - 1);
+        return sh;
+      }
+    }
     sh.inner.pml = 1;
     addm(sh.inner, 0, ni - 1, 0, nj - 1, 0, ktopm);
     sh.point.add(0, ni - 1, 0, nj - 1, ktopm + 1, nk - 1);
@@ -734,6 +764,10 @@ int launch_stage(cgfd_session *s, const Shell &sh, StageArgs<TS, TC> &a, int dx,
     if (ev0) HIPCHK(hipEventRecord(ev0, s->st));
     hipError_t e = launch_interior<TS, TC>(a, sh.inner, dx, dy, dz, nshots, s->st);
     if (e != hipSuccess) return fail(CGFD_E_HIP, "interior kernel: %s", hipGetErrorString(e));
+    if (sh.rim.nbox) {
+      e = launch_interior<TS, TC>(a, sh.rim, dx, dy, dz, nshots, s->st);
+      if (e != hipSuccess) return fail(CGFD_E_HIP, "rim kernel: %s", hipGetErrorString(e));
+    }
     if (ev1) HIPCHK(hipEventRecord(ev1, s->st));
   }
   if (sh.point.n) {
@@ -785,6 +819,7 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
     std::lock_guard<std::mutex> lk(g_tile_mu);
     g_tile_cache[key] = c;
   };
+  const int pinned = s->split_fixed ? s->split : -1;
   s->tile_shape = 0;
   s->split = CGFD_SPLIT_PLANES;
   const Shell planes = make_shell(s);
@@ -829,13 +864,19 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
     return CGFD_OK;
   };
   // Work splits: whole-plane sweep (PML terms inside) and, when the grid has
-  // PML faces, the sweep inside the face boxes with the per-point kernel on
-  // the slabs.  Each is timed with its best tile plus its per-point launch.
-  const bool shell_too = planes.inner.pml && !(s->d.flags & CGFD_FLAG_PML_POINT);
+  // PML faces, the PML-free core with the PML-form rim, and the sweep inside
+  // the face boxes with the per-point kernel on the slabs.  Each is timed
+  // with its best tile plus its per-point launch (a pinned split alone).
+  const bool faces = planes.inner.pml && !(s->d.flags & CGFD_FLAG_PML_POINT);
+  std::vector<int> splits;
+  if (pinned >= 0) splits.push_back(faces ? pinned : CGFD_SPLIT_PLANES);
+  else if (faces) splits = {CGFD_SPLIT_CORE, CGFD_SPLIT_PLANES, CGFD_SPLIT_SHELL};
+  else splits = {CGFD_SPLIT_PLANES};
   float best_total = 1e30f;
-  int bs = 0, bz = 24, bsplit = CGFD_SPLIT_PLANES;
-  for (int split = CGFD_SPLIT_PLANES; split >= (shell_too ? CGFD_SPLIT_SHELL : CGFD_SPLIT_PLANES); split--) {
+  int bs = 0, bz = 24, bsplit = splits[0];
+  for (int split : splits) {
     s->split = split;
+    s->tile_shape = 0;
     const Shell sh = make_shell(s);
     const int nz = sh.inner.box[0][5] - sh.inner.box[0][4] + 1;
     // z chunks near 16..48 planes that split the swept planes evenly: every
@@ -852,15 +893,22 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
     float best = 1e30f;
     int ts = 0, tz = 24;
     for (int shape = 0; shape < interior_num_shapes(); shape++) {
+      s->tile_shape = shape;
+      const Shell shs = make_shell(s);  // the core's bounds follow the tile shape
       for (int zi = 0; zi < kNZ; zi++) {
         const int zc = zcands[zi];
         if (zi > 0 && zc == zcands[zi - 1]) continue;
-        MarchSet ib = sh.inner;
-        ib.shape = shape;
+        MarchSet ib = shs.inner, rb = shs.rim;
         ib.zchunk = zc;
+        rb.zchunk = zc;
         float t;
-        int r = timed([&](int q, int d) { return launch_interior<TS, TC>(sa[q], ib, d, d, d, s->d.nshots, s->st); },
-                      t);
+        int r = timed(
+            [&](int q, int d) {
+              hipError_t e = launch_interior<TS, TC>(sa[q], ib, d, d, d, s->d.nshots, s->st);
+              if (e == hipSuccess && rb.nbox) e = launch_interior<TS, TC>(sa[q], rb, d, d, d, s->d.nshots, s->st);
+              return e;
+            },
+            t);
         if (r) return r;
         if (t < best) { best = t; ts = shape; tz = zc; }
       }
@@ -1685,6 +1733,17 @@ int cgfd_set_tile(cgfd_session *s, int shape, int zchunk) {
   s->tile_fixed = true;
   s->tile_shape = shape;
   s->tile_zchunk = zchunk;
+  if (!s->split_fixed) s->split = CGFD_SPLIT_PLANES;
+  return CGFD_OK;
+}
+
+int cgfd_set_split(cgfd_session *s, int split) {
+  if (!s) return fail(CGFD_E_ARG, "null session");
+  if (split > CGFD_SPLIT_CORE) return fail(CGFD_E_ARG, "work split %d invalid", split);
+  invalidate(s);
+  s->split_fixed = split >= 0;
+  s->split = split >= 0 ? split : CGFD_SPLIT_PLANES;
+  if (!s->tile_fixed) s->tile_shape = -1;  // tune the tile again (under the pinned split)
   return CGFD_OK;
 }
 
@@ -1696,8 +1755,7 @@ int cgfd_get_const_metric(cgfd_session *s, int *const_metric) {
 
 int cgfd_get_split(cgfd_session *s, int *split) {
   if (!s) return fail(CGFD_E_ARG, "null session");
-  if (split) *split = (s->d.flags & CGFD_FLAG_PML_POINT) || s->split == CGFD_SPLIT_SHELL ? CGFD_SPLIT_SHELL
-                                                                                        : CGFD_SPLIT_PLANES;
+  if (split) *split = (s->d.flags & CGFD_FLAG_PML_POINT) ? CGFD_SPLIT_SHELL : s->split < 0 ? CGFD_SPLIT_PLANES : s->split;
   return CGFD_OK;
 }
 int cgfd_get_tile(cgfd_session *s, int *shape, int *tx, int *ty, int *zchunk) {

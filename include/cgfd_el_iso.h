@@ -210,14 +210,23 @@ int cgfd_fetch_pml_aux(cgfd_session *s, int shot, int face, int level, int dtype
  * cgfd_run times every candidate tile under each work split (below) and keeps
  * the fastest pair (cached per grid, storage, shots and flags for the
  * process); shape < 0 returns to that automatic choice.  A pinned tile runs
- * with the whole-plane split unless CGFD_FLAG_PML_POINT is set. */
+ * with the whole-plane split unless cgfd_set_split pinned another or
+ * CGFD_FLAG_PML_POINT is set. */
 int cgfd_set_tile(cgfd_session *s, int shape, int zchunk);
 int cgfd_get_tile(cgfd_session *s, int *shape, int *tx, int *ty, int *zchunk);
 
 /* Work split of a stage between the z-marching and the per-point kernel. */
 #define CGFD_SPLIT_SHELL 0  /* sweep inside the PML face boxes; per-point kernel on the slabs and 4 surface rows */
 #define CGFD_SPLIT_PLANES 1 /* sweep whole planes with the PML terms; per-point kernel on the 4 surface rows */
+#define CGFD_SPLIT_CORE 2   /* whole planes in two sweeps: the PML-free form on a tile-aligned core inside the
+                               face boxes, the PML form on the boxes around it; per-point kernel on the 4
+                               surface rows */
 int cgfd_get_split(cgfd_session *s, int *split);
+/* Pin the work split (split < 0: the autotuner chooses again).  With an
+ * automatic tile the autotuner then tunes the tile under that split only.
+ * A tuning control with no counterpart in the reference, like cgfd_set_tile;
+ * every split gives the same results to rounding. */
+int cgfd_set_split(cgfd_session *s, int split);
 
 /* 1 when the uploaded metric runs the constant-metric sweep (a Cartesian
  * grid: one value per metric component, off-diagonal ones zero; the three

@@ -135,23 +135,50 @@ def test_multishot_equals_single_shot():
     many.close()
 
 
-@pytest.mark.parametrize("flags", [0, cgfd.FLAG_PML_POINT], ids=["planes", "shell"])
+@pytest.mark.parametrize("split", ["planes", "shell", "core"])
 @pytest.mark.parametrize("shape,zchunk", [(0, 7), (1, 48), (2, 5), (3, 24), (0, 96), (1, 33)])
-def test_every_interior_tile_shape(shape, zchunk, flags):
+def test_every_interior_tile_shape(shape, zchunk, split):
     """Each z-marching tile shape and a ragged z chunk reproduce the oracle,
-    under both work splits (a pinned tile runs the whole-plane split unless
-    the per-point shell is asked for)."""
+    under every work split (a pinned tile runs the whole-plane split unless
+    the per-point shell or the core split is asked for)."""
     P = Problem(ni=75, nj=41, nk=37, curvilinear=True, pml=(4, 3, 5, 2, 4, 0), nt_total=12, seed=12, hetero=0.04)
-    s = _sess(P, flags=flags)
+    s = _sess(P, flags=cgfd.FLAG_PML_POINT if split == "shell" else 0)
+    if split == "core":
+        s.set_split(cgfd.SPLIT_CORE)
     s.set_tile(shape, zchunk)
     s.run(0, P.nt_total, P.dt)
     t = s.get_tile()
     assert t["shape"] == shape and t["zchunk"] == zchunk
-    assert s.get_split() == (cgfd.SPLIT_SHELL if flags else cgfd.SPLIT_PLANES)
+    assert s.get_split() == {"planes": cgfd.SPLIT_PLANES, "shell": cgfd.SPLIT_SHELL, "core": cgfd.SPLIT_CORE}[split]
     so, wo, _ = P.oracle_run(32)
     assert rel_l2(s.read_seismo(0), so) <= TOL32
     _cmp_levels(s.fetch_level(0, cgfd.LEVEL_PRE), wo, TOL32, f"shape {shape}")
     s.close()
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+def test_core_split_wide_grid(shape):
+    """The core split on a grid wide enough for a core under every tile
+    shape (64-wide tiles too), with a free surface and unequal faces: the
+    PML-free core and the PML-form rim boxes around it reproduce the oracle
+    and the whole-plane split to rounding."""
+    P = Problem(ni=150, nj=44, nk=30, pml=(6, 9, 5, 7, 3, 0), nt_total=40, seed=15, hetero=0.03,
+                src_ijk=(40.3, 20.6, 14.2))
+    runs = {}
+    for split in (cgfd.SPLIT_CORE, cgfd.SPLIT_PLANES):
+        s = _sess(P)
+        s.set_split(split)
+        s.set_tile(shape, 9)
+        s.run(0, P.nt_total, P.dt)
+        assert s.get_split() == split
+        runs[split] = (s.read_seismo(0), s.fetch_level(0, cgfd.LEVEL_PRE))
+        s.close()
+    so, wo, _ = P.oracle_run(32)
+    errs = {split: rel_l2(seis, so) for split, (seis, _) in runs.items()}
+    for split, (seis, w) in runs.items():
+        assert errs[split] <= TOL32, errs
+        _cmp_levels(w, wo, TOL32, f"split {split}")
+    _cmp_levels(runs[cgfd.SPLIT_CORE][1], runs[cgfd.SPLIT_PLANES][1], 1e-6, "core vs planes")
 
 
 def test_autotune_picks_a_valid_tile():
@@ -160,7 +187,7 @@ def test_autotune_picks_a_valid_tile():
     s.run(0, 2, P.dt)
     t = s.get_tile()
     assert 0 <= t["shape"] < 4 and t["zchunk"] >= 1
-    assert s.get_split() in (cgfd.SPLIT_SHELL, cgfd.SPLIT_PLANES)
+    assert s.get_split() in (cgfd.SPLIT_SHELL, cgfd.SPLIT_PLANES, cgfd.SPLIT_CORE)
     s.close()
 
 
