@@ -156,29 +156,31 @@ def test_every_interior_tile_shape(shape, zchunk, split):
     s.close()
 
 
+@pytest.mark.parametrize("storage", ["f32", "f64"])
 @pytest.mark.parametrize("shape", [0, 1, 2, 3])
-def test_core_split_wide_grid(shape):
+def test_core_split_wide_grid(shape, storage):
     """The core split on a grid wide enough for a core under every tile
     shape (64-wide tiles too), with a free surface and unequal faces: the
     PML-free core and the PML-form rim boxes around it reproduce the oracle
-    and the whole-plane split to rounding."""
+    (FP32 and the FP64 path) and the whole-plane split to rounding."""
     P = Problem(ni=150, nj=44, nk=30, pml=(6, 9, 5, 7, 3, 0), nt_total=40, seed=15, hetero=0.03,
                 src_ijk=(40.3, 20.6, 14.2))
+    st, prec, tol = (cgfd.STORE_F32, 32, TOL32) if storage == "f32" else (cgfd.STORE_F64, 64, TOL64)
     runs = {}
     for split in (cgfd.SPLIT_CORE, cgfd.SPLIT_PLANES):
-        s = _sess(P)
+        s = _sess(P, storage=st)
         s.set_split(split)
         s.set_tile(shape, 9)
         s.run(0, P.nt_total, P.dt)
         assert s.get_split() == split
         runs[split] = (s.read_seismo(0), s.fetch_level(0, cgfd.LEVEL_PRE))
         s.close()
-    so, wo, _ = P.oracle_run(32)
+    so, wo, _ = P.oracle_run(prec)
     errs = {split: rel_l2(seis, so) for split, (seis, _) in runs.items()}
     for split, (seis, w) in runs.items():
-        assert errs[split] <= TOL32, errs
-        _cmp_levels(w, wo, TOL32, f"split {split}")
-    _cmp_levels(runs[cgfd.SPLIT_CORE][1], runs[cgfd.SPLIT_PLANES][1], 1e-6, "core vs planes")
+        assert errs[split] <= tol, errs
+        _cmp_levels(w, wo, tol, f"split {split}")
+    _cmp_levels(runs[cgfd.SPLIT_CORE][1], runs[cgfd.SPLIT_PLANES][1], tol / 10, "core vs planes")
 
 
 def test_autotune_picks_a_valid_tile():
