@@ -789,9 +789,10 @@ int launch_stage(cgfd_session *s, const Shell &sh, StageArgs<TS, TC> &a, int dx,
 struct TileKey {
   int ni, nj, nk, storage, nshots, flags, met_const, free_top;
   std::array<int, 6> nlay;  // PML layers per face (0: none)
+  int pinned;               // split pinned by cgfd_set_split, -1: automatic
   bool operator<(const TileKey &o) const {
-    return std::tie(ni, nj, nk, storage, nshots, flags, met_const, free_top, nlay) <
-           std::tie(o.ni, o.nj, o.nk, o.storage, o.nshots, o.flags, o.met_const, o.free_top, o.nlay);
+    return std::tie(ni, nj, nk, storage, nshots, flags, met_const, free_top, nlay, pinned) <
+           std::tie(o.ni, o.nj, o.nk, o.storage, o.nshots, o.flags, o.met_const, o.free_top, o.nlay, o.pinned);
   }
 };
 struct TileChoice {
@@ -803,7 +804,8 @@ std::map<TileKey, TileChoice> g_tile_cache;
 std::mutex g_tile_mu;
 
 template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
-  TileKey key{s->d.ni, s->d.nj, s->d.nk, s->d.storage, s->d.nshots, s->d.flags, s->met_const ? 1 : 0, s->free_top, {}};
+  TileKey key{s->d.ni, s->d.nj, s->d.nk, s->d.storage, s->d.nshots, s->d.flags, s->met_const ? 1 : 0, s->free_top,
+              {}, s->split_fixed ? s->split : -1};
   for (int f = 0; f < 6; f++) key.nlay[f] = s->face[f].on ? s->face[f].nlay : 0;
   {
     std::lock_guard<std::mutex> lk(g_tile_mu);
@@ -823,7 +825,8 @@ template <typename TS, typename TC> int autotune_t(cgfd_session *s) {
   s->tile_shape = 0;
   s->split = CGFD_SPLIT_PLANES;
   const Shell planes = make_shell(s);
-  if (!planes.inner.nbox) {
+  if (!planes.inner.nbox) {  // no sweep at all (plain method, tiny grid)
+    if (pinned >= 0) s->split = pinned;
     remember({s->tile_shape, s->tile_zchunk, s->split});
     return CGFD_OK;
   }

@@ -183,6 +183,26 @@ def test_core_split_wide_grid(shape, storage):
     _cmp_levels(runs[cgfd.SPLIT_CORE][1], runs[cgfd.SPLIT_PLANES][1], tol / 10, "core vs planes")
 
 
+def test_pinned_split_survives_the_tile_cache():
+    """A split pinned with cgfd_set_split is kept (and its tile tuned under
+    it) even when an unpinned session of the same model filled the process's
+    tile cache first; every pinned split reproduces the oracle."""
+    P = Problem(ni=96, nj=40, nk=30, pml=(5, 5, 5, 5, 5, 0), nt_total=12, seed=16, hetero=0.02)
+    a = _sess(P)
+    a.run(0, 2, P.dt)
+    a.close()
+    so, wo, _ = P.oracle_run(32)
+    for split in (cgfd.SPLIT_CORE, cgfd.SPLIT_PLANES, cgfd.SPLIT_SHELL):
+        s = _sess(P)
+        s.set_split(split)
+        s.run(0, P.nt_total, P.dt)
+        assert s.get_split() == split
+        assert rel_l2(s.read_seismo(0), so) <= TOL32
+        _cmp_levels(s.fetch_level(0, cgfd.LEVEL_PRE), wo, TOL32, f"pinned split {split}")
+        s.set_split(-1)
+        s.close()
+
+
 def test_autotune_picks_a_valid_tile():
     P = Problem(ni=48, nj=40, nk=36, nt_total=4, seed=13, pml=(5, 5, 5, 5, 5, 0))
     s = _sess(P)
