@@ -1260,6 +1260,8 @@ int cgfd_receiver_index(int ni, int nj, int nk, double rx, double ry, double rz,
   ry -= 1;
   if (depth_flag == 1) rz = (double)(nk - 1) - rz;
   else rz -= 1;
+  // far outside (or not a number): no integer cast of an out-of-range value
+  if (!(std::fabs(rx) < 1e9 && std::fabs(ry) < 1e9 && std::fabs(rz) < 1e9)) return CGFD_E_ARG;
   const int ix = (int)std::floor(rx), iy = (int)std::floor(ry), iz = (int)std::floor(rz);
   if (ix < 0 || ix > ni - 1 || iy < 0 || iy > nj - 1 || iz < 0 || iz > nk - 1) return CGFD_E_ARG;
   const long p = g.at(ix + NG, iy + NG, iz + NG);
@@ -1738,8 +1740,12 @@ double lag_piecewise(const std::vector<double> &t, const std::vector<double> &v,
   int lower, upper;
   if (order % 2 == 0) { lower = order / 2; upper = order / 2; }
   else { lower = order / 2; upper = order / 2 + 1; }
-  int indx = (int)((ti - t_start) / dt + 0.5);
-  if (indx > n - 1) indx = n - 1;
+  // nearest sample, clamped to [0, n-1] before the integer cast (an index
+  // below 0 selects the same first order+1 samples as 0 does)
+  double fi = (ti - t_start) / dt + 0.5;
+  if (!(fi >= 0.0)) fi = 0.0;
+  if (fi > n - 1) fi = n - 1;
+  int indx = (int)fi;
   int a, b;
   if (indx - lower >= 0 && indx + upper <= n - 1) { a = indx - lower; b = indx + upper; }
   else if (indx - lower < 0) { a = 0; b = indx + upper + (lower - indx); }
@@ -1814,11 +1820,13 @@ int cgfd_source_file(const char *path, int ni, int nj, int nk, const double *x, 
   if (!next_line(fp, line, sizeof(line)) || std::sscanf(line, "%d %d", &I.by_coord, &I.depth) != 2)
     return bad("no location line");
   const int ns = I.nsrc_file;
-  std::vector<float> loc(3 * (size_t)ns);
-  for (int q = 0; q < ns; q++)
-    if (!next_line(fp, line, sizeof(line)) ||
-        std::sscanf(line, "%f %f %f", &loc[3 * (size_t)q], &loc[3 * (size_t)q + 1], &loc[3 * (size_t)q + 2]) != 3)
+  std::vector<float> loc;  // grown line by line: the count in the file sizes nothing
+  for (int q = 0; q < ns; q++) {
+    float v[3];
+    if (!next_line(fp, line, sizeof(line)) || std::sscanf(line, "%f %f %f", &v[0], &v[1], &v[2]) != 3)
       return bad("location line " + std::to_string(q + 1) + " unreadable");
+    loc.insert(loc.end(), v, v + 3);
+  }
   // locate: nearest node and signed shift
   const G3 g(ni, nj, nk);
   std::vector<int> nd(3 * (size_t)ns, -1000);
@@ -1840,6 +1848,10 @@ int cgfd_source_file(const char *path, int ni, int nj, int nk, const double *x, 
       float v[3] = {loc[3 * (size_t)q], loc[3 * (size_t)q + 1], loc[3 * (size_t)q + 2]};
       if (I.depth == 1) v[2] = (float)(nk - 1) - v[2];
       for (int d = 0; d < 3; d++) {
+        if (!(std::fabs(v[d]) < 1e9f)) {  // far outside or not a number
+          nd[3 * (size_t)q + d] = -1000;
+          continue;
+        }
         const int id = (int)(v[d] + 0.5f);
         sh[3 * (size_t)q + d] = v[d] - (float)id;
         nd[3 * (size_t)q + d] = (id < 0 || id > tot[d] - 1) ? -1000 : id;
@@ -1858,7 +1870,10 @@ int cgfd_source_file(const char *path, int ni, int nj, int nk, const double *x, 
     if (I.cmp_type == 1 || I.cmp_type == 3) I.force_on = 1;
     else I.moment_on = 1;  // cmp_type 2 (3 never reaches here, as in the reference)
   }
-  I.max_nt = I.stf_given == 0 ? (int)(I.stf_length / dt + 0.5) : (int)((I.stf_nt - 1) * I.stf_dt / dt + 0.5) + 1;
+  const double mnt_d = I.stf_given == 0 ? I.stf_length / dt + 0.5 : (I.stf_nt - 1) * I.stf_dt / dt + 0.5;
+  if (!(mnt_d >= 1.0 - (I.stf_given == 1) && mnt_d < 1073741824.0))
+    return bad("the time function must last at least one step and fewer than 2^30");
+  I.max_nt = (int)mnt_d + (I.stf_given == 1);
   *info = I;
   if (!it_begin && !it_end && !ext_num && !ext_indx && !ext_coef && !F && !M) return CGFD_OK;
   if (!it_begin || !it_end || !ext_num || !ext_indx || !ext_coef || (I.force_on && !F) || (I.moment_on && !M))
@@ -1869,8 +1884,7 @@ int cgfd_source_file(const char *path, int ni, int nj, int nk, const double *x, 
   double coefs[10] = {0}, wstart = 0;
   int wkind = -1;
   double fc[3] = {0, 0, 0}, mc[6] = {0, 0, 0, 0, 0, 0};
-  std::vector<double> samp[9];
-  for (auto &v : samp) v.assign((size_t)std::max(I.stf_nt, 1), 0.0);
+  std::vector<double> samp[9];  // grown as sample lines are read (zero before)
   int ks = 0;
   for (int q = 0; q < ns; q++) {
     double tstart = 0;
@@ -1927,6 +1941,8 @@ int cgfd_source_file(const char *path, int ni, int nj, int nk, const double *x, 
       for (int it = 0; it < I.stf_nt; it++) {
         if (!next_line(fp, line, sizeof(line))) return bad("time-function samples of source " + std::to_string(q + 1));
         if (!here[(size_t)q]) continue;
+        if (samp[0].size() <= (size_t)it)
+          for (auto &v : samp) v.resize((size_t)it + 1, 0.0);
         double f3[3], m6[6];
         for (int c = 0; c < 3; c++) f3[c] = samp[c][(size_t)it];
         for (int c = 0; c < 6; c++) m6[c] = samp[3 + c][(size_t)it];
@@ -1941,7 +1957,9 @@ int cgfd_source_file(const char *path, int ni, int nj, int nk, const double *x, 
     // spreading, start step, per-(step, stage) values
     ext_num[ks] = spread_gauss(g, m[0], m[1], m[2], sh[3 * (size_t)q], sh[3 * (size_t)q + 1], sh[3 * (size_t)q + 2],
                                ext_indx + 343 * (size_t)ks, ext_coef + 343 * (size_t)ks);
-    const int ib = (int)((tstart - t0) / dt);
+    const double ib_d = (tstart - t0) / dt;
+    if (!(std::fabs(ib_d) < 1073741824.0)) return bad("start time of source " + std::to_string(q + 1) + " out of range");
+    const int ib = (int)ib_d;
     it_begin[ks] = ib;
     it_end[ks] = ib + mnt - 1;
     const double t_shift = tstart - (ib * dt + t0);

@@ -176,3 +176,30 @@ def test_bad_source_files(tmp_path):
     with pytest.raises(cgfd.CgfdError, match="not implemented"):
         cgfd.source_file(_write(tmp_path, "ev\n1\n0 1\n1 0\n0 0\n5 5 5\n0 sinc 1 1\n1 1 1\n", "b.src"),
                          10, 10, 10, 0.0, 0.01)
+
+
+def test_out_of_range_values_are_rejected_or_dropped(tmp_path):
+    """Counts, lengths and positions far beyond the grid or the int range
+    give an error or an outside source, never a huge allocation or an
+    out-of-range integer conversion."""
+    def w(name, text):
+        return _write(tmp_path, text, name)
+    # a source count far beyond the lines in the file
+    with pytest.raises(cgfd.CgfdError, match="location line 2"):
+        cgfd.source_file(w("n.src", "ev\n2000000000\n0 1\n1 0\n0 0\n5 5 5\n"), 10, 10, 10, 0.0, 0.01)
+    # sampled time function announcing far more samples than the file holds
+    with pytest.raises(cgfd.CgfdError, match="samples of source 1"):
+        cgfd.source_file(w("s.src", "ev\n1\n1 0.01 2000000000\n1 0\n0 0\n5 5 5\n0.0\n1 1 1\n"), 10, 10, 10, 0.0, 1.0)
+    # no step long, or more than 2^30 steps
+    for length in ("0", "-3", "1e30", "nan"):
+        with pytest.raises(cgfd.CgfdError, match="at least one step"):
+            cgfd.source_file(w("l.src", f"ev\n1\n0 {length}\n1 0\n0 0\n5 5 5\n0 ricker 2 0.2\n1 1 1\n"), 10, 10, 10,
+                             0.0, 0.01)
+    # start time beyond 2^30 steps
+    with pytest.raises(cgfd.CgfdError, match="out of range"):
+        cgfd.source_file(w("t.src", "ev\n1\n0 1\n1 0\n0 0\n5 5 5\n1e30 ricker 2 0.2\n1 1 1\n"), 10, 10, 10, 0.0, 0.01)
+    # positions far outside (or not a number): the source is dropped
+    info, src, _, _ = cgfd.source_file(w("p.src", "ev\n3\n0 1\n1 0\n0 0\n1e30 5 5\n5 -1e30 5\n5 5 nan\n"
+                                                  "0 ricker 2 0.2\n1 1 1\n0 ricker 2 0.2\n1 1 1\n0 ricker 2 0.2\n1 1 1\n"),
+                                       10, 10, 10, 0.0, 0.01)
+    assert info["nsrc_file"] == 3 and info["nsrc"] == 0 and src["nsrc"] == 0

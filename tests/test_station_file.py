@@ -90,3 +90,12 @@ def test_bad_station_files(tmp_path):
     g.write_text("1\na 1 0 100 100 -100\n")
     with pytest.raises(cgfd.CgfdError, match="grid coordinates"):
         cgfd.station_file(g, ni, nj, nk)
+
+
+def test_far_out_stations_are_outside(tmp_path):
+    ni, nj, nk = 20, 20, 20
+    x, y, z = _flat(ni, nj, nk)
+    f = tmp_path / "far.list"
+    f.write_text("4\na 0 0 1e30 5 5\nb 0 1 5 -1e30 0\nc 1 1 1e30 100 0\nd 0 0 nan 5 5\n")
+    st = cgfd.station_file(f, ni, nj, nk, x, y, z)
+    assert [s["inside"] for s in st] == [False] * 4
