@@ -7,6 +7,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <random>
 #include <string>
 #include <thread>
@@ -302,6 +304,11 @@ int md3lay_read(const char *path, Md3Lay &L, std::string &why) {
     return CGFD_E_ARG;
   }
   if (L.nx < 2 || L.ny < 2) { why = "interface mesh needs NX, NY >= 2"; return CGFD_E_ARG; }
+  // the header's counts size nothing before the file is known to hold them
+  if (10.0 * L.ni * L.nx * L.ny > (double)(tok.size() - t)) {
+    why = "insufficient data in the interface file";
+    return CGFD_E_ARG;
+  }
   const size_t n = (size_t)L.ni * L.nx * L.ny;
   std::vector<float> *cols[10] = {&L.elev, &L.rho, &L.rho_g, &L.rho_p, &L.vp, &L.vp_g, &L.vp_p, &L.vs, &L.vs_g, &L.vs_p};
   for (auto *c : cols) c->resize(n);
@@ -590,10 +597,13 @@ int md3grd_read(const char *path, float xmin, float xmax, float ymin, float ymax
   }
   auto num_i = [&](int &v) { std::string s; if (!next(s)) return false; v = std::atoi(s.c_str()); return true; };
   auto num_f = [&](float &v) { std::string s; if (!next(s)) return false; v = std::strtof(s.c_str(), nullptr); return true; };
-  if (!num_i(G.nl) || G.nl < 1) { why = "bad number of layers"; return CGFD_E_ARG; }
+  if (!num_i(G.nl) || G.nl < 1 || (size_t)G.nl > tok.size() - t) { why = "bad number of layers"; return CGFD_E_ARG; }
   std::vector<int> ngz(G.nl);
+  double nsurf = 0;
   for (int l = 0; l < G.nl; l++) {
     if (!num_i(ngz[l]) || ngz[l] < 1) { why = "bad number of surfaces in a layer"; return CGFD_E_ARG; }
+    nsurf += ngz[l];
+    if (nsurf > (double)(tok.size() - t)) { why = "insufficient data in the media file"; return CGFD_E_ARG; }
     G.top.push_back(G.ni);
     G.ni += ngz[l];
   }
@@ -606,6 +616,7 @@ int md3grd_read(const char *path, float xmin, float xmax, float ymin, float ymax
     return CGFD_E_ARG;
   }
   if (NX < 2 || NY < 2) { why = "media mesh needs NX, NY >= 2"; return CGFD_E_ARG; }
+  if (4.0 * G.ni * NX * NY > (double)(tok.size() - t)) { why = "insufficient data in the media file"; return CGFD_E_ARG; }
   // the window [ix0, ix1] x [iy0, iy1] of mesh nodes that covers the grid
   const float fx0 = (xmin - MINX) / G.dx, fy0 = (ymin - MINY) / G.dy;
   const float fx1 = std::ceil((xmax - MINX) / G.dx), fy1 = std::ceil((ymax - MINY) / G.dy);
@@ -681,7 +692,26 @@ int grid_point(const Md3Grd &G, const LayerEval &E, float ax, float ay, float az
 
 }  // namespace
 
-int cgfd_media_layer_file(int ni, int nj, int nk, const double *x, const double *y, const double *z, const char *path,
+// The file readers run on input read from disk: an allocation that fails or
+// a library exception ends the call with an error code instead of crossing
+// the C ABI.
+extern "C++" {
+namespace {
+template <typename F> int guarded(F f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc &) {
+    cgfd_internal_set_error("out of host memory");
+    return CGFD_E_NOMEM;
+  } catch (const std::exception &e) {
+    cgfd_internal_set_error(e.what());
+    return CGFD_E_ARG;
+  }
+}
+}  // namespace
+}  // extern "C++"
+
+static int cgfd_media_layer_file_body(int ni, int nj, int nk, const double *x, const double *y, const double *z, const char *path,
                           const char *method, double *lam, double *mu, double *rho) {
   if (ni < 1 || nj < 1 || nk < 1 || !x || !y || !z || !path || !method || !lam || !mu || !rho) return CGFD_E_ARG;
   const bool loc = !std::strcmp(method, "loc"), har = !std::strcmp(method, "har"), ari = !std::strcmp(method, "ari");
@@ -740,7 +770,7 @@ int cgfd_media_layer_file(int ni, int nj, int nk, const double *x, const double 
                            L.ni, lam, mu, rho);
 }
 
-int cgfd_media_grid_file(int ni, int nj, int nk, const double *x, const double *y, const double *z, const char *path,
+static int cgfd_media_grid_file_body(int ni, int nj, int nk, const double *x, const double *y, const double *z, const char *path,
                          const char *method, double *lam, double *mu, double *rho) {
   if (ni < 1 || nj < 1 || nk < 1 || !x || !y || !z || !path || !method || !lam || !mu || !rho) return CGFD_E_ARG;
   const bool loc = !std::strcmp(method, "loc"), har = !std::strcmp(method, "har"), ari = !std::strcmp(method, "ari");
@@ -806,7 +836,7 @@ int cgfd_media_grid_file(int ni, int nj, int nk, const double *x, const double *
                            G.ni, lam, mu, rho);
 }
 
-int cgfd_media_bin_file(int ni, int nj, int nk, const double *x, const double *y, const double *z,
+static int cgfd_media_bin_file_body(int ni, int nj, int nk, const double *x, const double *y, const double *z,
                         const int order[3], const int size[3], const double spacing[3], const double origin[3],
                         const char *rho_path, const char *vp_path, const char *vs_path, double *lam, double *mu,
                         double *rho) {
@@ -1637,7 +1667,7 @@ void corners8(const G3 &g, int i, int j, int k, long long *ix) {
 
 }  // namespace
 
-int cgfd_station_file(const char *path, int ni, int nj, int nk, const double *x, const double *y, const double *z,
+static int cgfd_station_file_body(const char *path, int ni, int nj, int nk, const double *x, const double *y, const double *z,
                       int max_n, cgfd_station *out, int *n_file) {
   if (!path || max_n < 0 || (max_n > 0 && !out) || ni < 2 || nj < 2 || nk < 2) return CGFD_E_ARG;
   FILE *fp = std::fopen(path, "rt");
@@ -1781,7 +1811,7 @@ void muda_to_moment(double strike, double dip, double rake, double mu, double D,
 
 }  // namespace
 
-int cgfd_source_file(const char *path, int ni, int nj, int nk, const double *x, const double *y, const double *z,
+static int cgfd_source_file_body(const char *path, int ni, int nj, int nk, const double *x, const double *y, const double *z,
                      const double *mu, double t0, double dt, cgfd_source_file_info *info, int *it_begin,
                      int *it_end, int *ext_num, long long *ext_indx, double *ext_coef, double *F, double *M,
                      int *node, double *shift) {
@@ -2061,3 +2091,39 @@ int cgfd_storage_scales(int storage, int ni, int nj, int nk, const double *metri
 }
 
 }  // extern "C"
+
+// ---- guarded C entry points of the file readers ----------------------------------
+int cgfd_media_layer_file(int ni, int nj, int nk, const double *x, const double *y, const double *z, const char *path,
+                          const char *method, double *lam, double *mu, double *rho) {
+  return guarded([&] { return cgfd_media_layer_file_body(ni, nj, nk, x, y, z, path, method, lam, mu, rho); });
+}
+
+int cgfd_media_grid_file(int ni, int nj, int nk, const double *x, const double *y, const double *z, const char *path,
+                         const char *method, double *lam, double *mu, double *rho) {
+  return guarded([&] { return cgfd_media_grid_file_body(ni, nj, nk, x, y, z, path, method, lam, mu, rho); });
+}
+
+int cgfd_media_bin_file(int ni, int nj, int nk, const double *x, const double *y, const double *z,
+                        const int order[3], const int size[3], const double spacing[3], const double origin[3],
+                        const char *rho_path, const char *vp_path, const char *vs_path, double *lam, double *mu,
+                        double *rho) {
+  return guarded([&] {
+    return cgfd_media_bin_file_body(ni, nj, nk, x, y, z, order, size, spacing, origin, rho_path, vp_path, vs_path, lam,
+                                    mu, rho);
+  });
+}
+
+int cgfd_station_file(const char *path, int ni, int nj, int nk, const double *x, const double *y, const double *z,
+                      int max_n, cgfd_station *out, int *n_file) {
+  return guarded([&] { return cgfd_station_file_body(path, ni, nj, nk, x, y, z, max_n, out, n_file); });
+}
+
+int cgfd_source_file(const char *path, int ni, int nj, int nk, const double *x, const double *y, const double *z,
+                     const double *mu, double t0, double dt, cgfd_source_file_info *info, int *it_begin,
+                     int *it_end, int *ext_num, long long *ext_indx, double *ext_coef, double *F, double *M,
+                     int *node, double *shift) {
+  return guarded([&] {
+    return cgfd_source_file_body(path, ni, nj, nk, x, y, z, mu, t0, dt, info, it_begin, it_end, ext_num, ext_indx,
+                                 ext_coef, F, M, node, shift);
+  });
+}

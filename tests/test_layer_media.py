@@ -202,3 +202,13 @@ def test_equivalent_medium_product_equals_oracle(tmp_path, method):
     assert (b[2] != loc[2]).sum() > 10  # the interfaces cross the grid
     for u, v in zip(a, b):
         assert np.array_equal(u, v)
+
+
+def test_header_counts_beyond_the_data_are_rejected(tmp_path):
+    """An interface file whose header announces far more mesh nodes than it
+    holds is refused before anything is sized by those counts."""
+    f = tmp_path / "big.md3lay"
+    f.write_text("elastic_isotropic\n100000\n100000 100000 0 0 1 1\n" + "1 " * 40 + "\n")
+    x, y, z = cgfd.grid_cartesian(8, 8, 8, 100.0, 0.0, 100.0, 0.0, 100.0, -700.0)
+    with pytest.raises(cgfd.CgfdError, match="insufficient data"):
+        cgfd.media_layer_file(8, 8, 8, x, y, z, f, "loc")
