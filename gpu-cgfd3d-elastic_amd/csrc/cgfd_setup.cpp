@@ -304,6 +304,11 @@ int md3lay_read(const char *path, Md3Lay &L, std::string &why) {
     return CGFD_E_ARG;
   }
   if (L.nx < 2 || L.ny < 2) { why = "interface mesh needs NX, NY >= 2"; return CGFD_E_ARG; }
+  if (!(L.dx > 0 && L.dy > 0 && std::isfinite(L.dx) && std::isfinite(L.dy) && std::isfinite(L.minx) &&
+        std::isfinite(L.miny))) {
+    why = "interface mesh needs finite MINX, MINY and DX, DY > 0";
+    return CGFD_E_ARG;
+  }
   // the header's counts size nothing before the file is known to hold them
   if (10.0 * L.ni * L.nx * L.ny > (double)(tok.size() - t)) {
     why = "insufficient data in the interface file";
@@ -316,6 +321,9 @@ int md3lay_read(const char *path, Md3Lay &L, std::string &why) {
     for (auto *c : cols)
       if (!num_f((*c)[q])) {
         why = "insufficient data in the interface file";
+        return CGFD_E_ARG;
+      } else if (!std::isfinite((*c)[q])) {
+        why = "the interface file holds a value that is not a finite number";
         return CGFD_E_ARG;
       }
   return CGFD_OK;
@@ -384,7 +392,7 @@ namespace {
 
 // rho, vp, vs of the layered model at one point (AssignLayerMediaPara2Point +
 // CalPointValue_layer for elastic_isotropic); 1: outside the interface mesh,
-// 2: vp <= sqrt(2) vs
+// 2: vp <= sqrt(2) vs, 3: interpolated interfaces not finite
 int layer_point(const Md3Lay &L, const LayerEval &E, float ax, float ay, float az, std::vector<float> &elev, float &r,
                 float &vp, float &vs) {
   const float maxx = L.minx + (L.nx - 1) * L.dx, maxy = L.miny + (L.ny - 1) * L.dy;
@@ -396,6 +404,7 @@ int layer_point(const Md3Lay &L, const LayerEval &E, float ax, float ay, float a
   if (mi == -1) {
     // above the top interface: the values at the top of the first layer
     mi = last_ge(elev[0], elev);
+    if (mi < 0) return 3;  // interpolated elevations not finite (mesh values overflowed)
     r = field(L.rho, mi);
     vp = field(L.vp, mi);
     vs = field(L.vs, mi);
@@ -556,8 +565,9 @@ int equivalent_medium(const G3 &g, const double *x, const double *y, const doubl
   });
   for (int j = 0; j < g.ny; j++)
     if (jerr[j]) {
-      cgfd_internal_set_error(jerr[j] == 1 ? "grid point outside the interface mesh"
-                                           : "vp must be larger than sqrt(2) vs in the media file");
+      cgfd_internal_set_error(jerr[j] == 1   ? "grid point outside the interface mesh"
+                              : jerr[j] == 2 ? "vp must be larger than sqrt(2) vs in the media file"
+                                             : "the media file gives surfaces that are not finite at a grid point");
       return CGFD_E_ARG;
     }
   return CGFD_OK;
@@ -616,6 +626,11 @@ int md3grd_read(const char *path, float xmin, float xmax, float ymin, float ymax
     return CGFD_E_ARG;
   }
   if (NX < 2 || NY < 2) { why = "media mesh needs NX, NY >= 2"; return CGFD_E_ARG; }
+  if (!(G.dx > 0 && G.dy > 0 && std::isfinite(G.dx) && std::isfinite(G.dy) && std::isfinite(MINX) &&
+        std::isfinite(MINY))) {
+    why = "media mesh needs finite MINX, MINY and DX, DY > 0";
+    return CGFD_E_ARG;
+  }
   if (4.0 * G.ni * NX * NY > (double)(tok.size() - t)) { why = "insufficient data in the media file"; return CGFD_E_ARG; }
   // the window [ix0, ix1] x [iy0, iy1] of mesh nodes that covers the grid
   const float fx0 = (xmin - MINX) / G.dx, fy0 = (ymin - MINY) / G.dy;
@@ -638,6 +653,7 @@ int md3grd_read(const char *path, float xmin, float xmax, float ymin, float ymax
         for (auto *c : cols) {
           float v;
           if (!num_f(v)) { why = "insufficient data in the media file"; return CGFD_E_ARG; }
+          if (!std::isfinite(v)) { why = "the media file holds a value that is not a finite number"; return CGFD_E_ARG; }
           if (keep) (*c)[q] = v;
         }
       }
@@ -658,7 +674,8 @@ int md3grd_read(const char *path, float xmin, float xmax, float ymin, float ymax
 // two surfaces around the point; above the first surface the values at the
 // top; below the last those of the last; within 1 m under the top surface of
 // a layer below the first, the mean of that surface and the one above it.
-// 1: outside the media mesh, 2: vp <= sqrt(2) vs.
+// 1: outside the media mesh, 2: vp <= sqrt(2) vs, 3: interpolated surfaces
+// not finite.
 int grid_point(const Md3Grd &G, const LayerEval &E, float ax, float ay, float az, std::vector<float> &elev,
                float &r, float &vp, float &vs, bool &on) {
   const float maxx = G.minx + (G.nx - 1) * G.dx, maxy = G.miny + (G.ny - 1) * G.dy;
@@ -670,6 +687,7 @@ int grid_point(const Md3Grd &G, const LayerEval &E, float ax, float ay, float az
   auto field = [&](const std::vector<float> &v, int s) { return E.bilinear(&v[s * slice], ax, ay); };
   if (mi == -1 || mi == G.ni - 1) {
     if (mi == -1) mi = last_ge(elev[0], elev);
+    if (mi < 0) return 3;  // interpolated elevations not finite (mesh values overflowed)
     r = field(G.rho, mi);
     vp = field(G.vp, mi);
     vs = field(G.vs, mi);
@@ -731,8 +749,9 @@ static int cgfd_media_layer_file_body(int ni, int nj, int nk, const double *x, c
   auto fail = [&]() {
     for (int k = 0; k < g.nz; k++)
       if (err[k]) {
-        cgfd_internal_set_error(err[k] == 1 ? "grid point outside the interface mesh"
-                                            : "vp must be larger than sqrt(2) vs in the layer file");
+        cgfd_internal_set_error(err[k] == 1   ? "grid point outside the interface mesh"
+                                : err[k] == 2 ? "vp must be larger than sqrt(2) vs in the layer file"
+                                              : "the layer file gives interfaces that are not finite at a grid point");
         return true;
       }
     return false;
@@ -813,8 +832,9 @@ static int cgfd_media_grid_file_body(int ni, int nj, int nk, const double *x, co
   });
   for (int k = 0; k < g.nz; k++)
     if (err[k]) {
-      cgfd_internal_set_error(err[k] == 1 ? "grid point outside the media mesh"
-                                          : "vp must be larger than sqrt(2) vs in the media file");
+      cgfd_internal_set_error(err[k] == 1   ? "grid point outside the media mesh"
+                              : err[k] == 2 ? "vp must be larger than sqrt(2) vs in the media file"
+                                            : "the media file gives surfaces that are not finite at a grid point");
       return CGFD_E_ARG;
     }
   // one layer: nothing to average (the reference falls back to "loc")

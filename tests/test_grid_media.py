@@ -274,3 +274,30 @@ def test_product_equals_oracle_on_topography(tmp_path, method):
     else:
         loc = cgfd.media_grid_file(ni, nj, nk, x, y, z, path)
         assert (b[2] != loc[2]).sum() > 10  # the boundaries cross the grid
+
+
+def test_non_finite_and_overflowing_mesh_values_are_errors(tmp_path):
+    """A mesh spacing so large that the bilinear weights overflow float, and
+    values that are not finite numbers, give an error instead of a layer
+    index outside the file (found by fuzzing the reader)."""
+    ni, nj, nk = 20, 18, 16
+    x, y, z = cgfd.grid_cartesian(ni, nj, nk, 100.0, 0.0, 100.0, 0.0, 100.0, -(nk - 1) * 100.0)
+    surf = [(100, 2000, 3000, 1700), (-500, 2100, 3200, 1800), (-500, 2500, 5000, 2900), (-2000, 2600, 5200, 3000)]
+
+    def grd(mesh, bad=None):
+        rows = ["elastic_isotropic", "2", "2 2", mesh]
+        for s_, v in enumerate(surf):
+            for q in range(9):
+                rows.append(" ".join(str(a) for a in v) if (s_, q) != bad else "nan 2000 3000 1700")
+        p = tmp_path / "m.md3grd"
+        p.write_text("\n".join(rows) + "\n")
+        return p
+
+    cgfd.media_grid_file(ni, nj, nk, x, y, z, grd("3 3 -1000 -1000 2000 2000"), "loc")   # the good file reads
+    for method in ("loc", "har", "ari"):
+        with pytest.raises(cgfd.CgfdError, match="not finite"):
+            cgfd.media_grid_file(ni, nj, nk, x, y, z, grd("3 3 -1000 -1000 20001e30 2000"), method)
+    with pytest.raises(cgfd.CgfdError, match="not a finite number"):
+        cgfd.media_grid_file(ni, nj, nk, x, y, z, grd("3 3 -1000 -1000 2000 2000", bad=(1, 4)), "loc")
+    with pytest.raises(cgfd.CgfdError, match="DX, DY > 0"):
+        cgfd.media_grid_file(ni, nj, nk, x, y, z, grd("3 3 -1000 -1000 0 2000"), "loc")
