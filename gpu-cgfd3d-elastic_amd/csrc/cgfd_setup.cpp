@@ -383,8 +383,13 @@ int first_ge(float value, const std::vector<float> &x) {
 }
 
 // The reference's isEqual(a, b) is `abs(a - b) < FLT_EPSILON` with no std::
-// qualification, which binds C's int abs: it holds whenever |a - b| < 1.
-bool ref_is_equal(float a, float b) { return std::abs((int)(a - b)) < FLT_EPSILON; }
+// qualification, which binds C's int abs: it holds whenever |a - b| < 1
+// (the difference truncates to 0).  Written as that range, so a difference
+// beyond the int range (or not a number) is no integer conversion.
+bool ref_is_equal(float a, float b) {
+  const float d = a - b;
+  return d > -1.0f && d < 1.0f;
+}
 
 }  // namespace
 
@@ -873,8 +878,8 @@ static int cgfd_media_bin_file_body(int ni, int nj, int nk, const double *x, con
     dim[order[d]] = d;
   }
   for (int d = 0; d < 3; d++)
-    if (size[d] < 2 || !(spacing[d] != 0)) {
-      cgfd_internal_set_error("bin sizes must be at least 2 with nonzero spacing");
+    if (size[d] < 2 || !(spacing[d] != 0) || !std::isfinite(spacing[d]) || !std::isfinite(origin[d])) {
+      cgfd_internal_set_error("bin sizes must be at least 2 with finite nonzero spacing and a finite origin");
       return CGFD_E_ARG;
     }
   const G3 g(ni, nj, nk);
@@ -895,14 +900,15 @@ static int cgfd_media_bin_file_body(int ni, int nj, int nk, const double *x, con
   const float amin[2] = {xmin, ymin}, amax[2] = {xmax, ymax};
   for (int a = 0; a < 2; a++) {
     const int n = size[dim[a]];
-    lo[a] = amin[a] < o[a] ? 0 : (int)((amin[a] - o[a]) / h[a]);
-    if (lo[a] >= n || amax[a] < o[a]) {
+    const float flo = amin[a] < o[a] ? 0.0f : (amin[a] - o[a]) / h[a];
+    if (!(flo < (float)n) || amax[a] < o[a]) {  // checked before the integer conversion
       cgfd_internal_set_error(a == 0 ? "the bin model does not cover the grid in x"
                                      : "the bin model does not cover the grid in y");
       return CGFD_E_ARG;
     }
-    hi[a] = (int)std::ceil((amax[a] - o[a]) / h[a]);
-    if (hi[a] >= n) hi[a] = n - 1;
+    lo[a] = (int)flo;
+    const float fhi = std::ceil((amax[a] - o[a]) / h[a]);
+    hi[a] = fhi < (float)n ? (int)fhi : n - 1;
     if (hi[a] <= lo[a]) {
       cgfd_internal_set_error("the bin model window covering the grid needs two nodes per axis");
       return CGFD_E_ARG;
