@@ -784,8 +784,10 @@ int launch_stage(cgfd_session *s, const Shell &sh, StageArgs<TS, TC> &a, int dx,
 // per cell), so tuning on one mode alone can pick a tile that loses on the
 // others.  The RK coefficients are zero, and every level the sweeps write
 // (U1, U2, U3) is rewritten by the next step before it is read, so
-// tuning between steps leaves the state intact.  Results are cached per
-// (grid, storage, shots, work split) for the life of the process.
+// tuning between steps leaves the state intact.  Each work split (core,
+// whole planes, shell) is timed with its best tile and the fastest pair is
+// kept.  Results are cached per (grid, storage, shots, flags, metric form,
+// PML layout, pinned split) for the life of the process.
 struct TileKey {
   int ni, nj, nk, storage, nshots, flags, met_const, free_top;
   std::array<int, 6> nlay;  // PML layers per face (0: none)
