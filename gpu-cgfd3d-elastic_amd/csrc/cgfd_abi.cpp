@@ -1767,7 +1767,8 @@ int cgfd_get_const_metric(cgfd_session *s, int *const_metric) {
 
 int cgfd_get_split(cgfd_session *s, int *split) {
   if (!s) return fail(CGFD_E_ARG, "null session");
-  if (split) *split = (s->d.flags & CGFD_FLAG_PML_POINT) ? CGFD_SPLIT_SHELL : s->split < 0 ? CGFD_SPLIT_PLANES : s->split;
+  if (split)
+    *split = (s->d.flags & CGFD_FLAG_PML_POINT) ? CGFD_SPLIT_SHELL : s->split < 0 ? CGFD_SPLIT_PLANES : s->split;
   return CGFD_OK;
 }
 int cgfd_get_tile(cgfd_session *s, int *shape, int *tx, int *ty, int *zchunk) {
