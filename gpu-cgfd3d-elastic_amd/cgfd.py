@@ -863,9 +863,15 @@ class SourceFileInfo(C.Structure):
                 ("moment_on", C.c_int), ("max_nt", C.c_int)]
 
 
-def source_file(path, ni, nj, nk, t0, dt, x=None, y=None, z=None, mu=None):
+def _host_memory_bytes():
+    return os.sysconf("SC_PHYS_PAGES") * os.sysconf("SC_PAGE_SIZE")
+
+
+def source_file(path, ni, nj, nk, t0, dt, x=None, y=None, z=None, mu=None, max_table_bytes=None):
     """A source file (src_read_locate_file) as (info dict, source dict for
-    Session.upload_source, node [nsrc][3], shift [nsrc][3])."""
+    Session.upload_source, node [nsrc][3], shift [nsrc][3]).  Tables larger
+    than max_table_bytes (default: half the host memory) raise CgfdError
+    before anything is allocated."""
     load()
     xs = [_arr(a, np.float64) if a is not None else None for a in (x, y, z, mu)]
     xp = [_ptr(a) for a in xs]
@@ -875,6 +881,11 @@ def source_file(path, ni, nj, nk, t0, dt, x=None, y=None, z=None, mu=None):
     if r < 0:
         _check(r)
     ns, mnt = info.nsrc, info.max_nt
+    nbytes = (3 * info.force_on + 6 * info.moment_on) * max(ns, 1) * max(mnt, 1) * 4 * 8 + max(ns, 1) * 343 * 16
+    limit = _host_memory_bytes() // 2 if max_table_bytes is None else max_table_bytes
+    if nbytes > limit:
+        raise CgfdError(f"source file {path}: tables of {nbytes} bytes ({max(ns, 1)} sources x {mnt} steps) "
+                        f"exceed the {limit}-byte limit")
     itb = np.zeros(max(ns, 1), np.int32)
     ite = np.zeros(max(ns, 1), np.int32)
     en = np.zeros(max(ns, 1), np.int32)

@@ -78,7 +78,8 @@ def test_source_and_station_readers_survive_mutations(tmp_path):
         p.write_text(_mutate(rng, rng.choice(seeds["src" if it % 2 == 0 else "list"])))
         try:
             if it % 2 == 0:
-                info, src, _, _ = cgfd.source_file(p, NI, NJ, NK, 0.0, 0.01, x, y, z, mu=mu)
+                info, src, _, _ = cgfd.source_file(p, NI, NJ, NK, 0.0, 0.01, x, y, z, mu=mu,
+                                                   max_table_bytes=1 << 26)
                 assert info["nsrc"] == src["nsrc"] >= 0
             else:
                 cgfd.station_file(p, NI, NJ, NK, x, y, z, max_n=1000)
