@@ -28,6 +28,16 @@ def test_subvolume_equals_slice_of_full_level(storage):
                        g + start[0]:g + start[0] + count[0] * stride[0]:stride[0]]
     assert sub.shape == want.shape
     assert np.array_equal(sub, want)
+    if storage == cgfd.STORE_F32:
+        # and the oracle's wavefield after the same steps, sampled the same way
+        wo = P.oracle_run(nt_total=12)[1]
+        ref = wo[comps][:, g + start[2]:g + start[2] + count[2] * stride[2]:stride[2],
+                        g + start[1]:g + start[1] + count[1] * stride[1]:stride[1],
+                        g + start[0]:g + start[0] + count[0] * stride[0]:stride[0]]
+        for c in range(len(comps)):
+            scale = np.abs(wo[comps[c]]).max()
+            assert scale > 0
+            assert np.abs(sub[c] - ref[c]).max() <= 1e-5 * scale, comps[c]
     with pytest.raises(cgfd.CgfdError):
         s.fetch_subvolume(0, cgfd.LEVEL_PRE, [0], (28, 0, 0), (2, 1, 1), (2, 1, 1))  # past the last column
     s.close()
