@@ -8,11 +8,12 @@ import numpy as np
 import pytest
 
 import cgfd
-from problem import Problem
+from problem import Problem, rel_l2
 
 pytestmark = pytest.mark.gpu
 
 TILE = (2, 24)
+TOL32 = 1e-5
 
 
 def _model(src_ijk, nt_total):
@@ -38,6 +39,10 @@ def test_graph_periods_equal_eager_launches():
     b.run(3, 24, P.dt)     # eager 3..7, graphs 8..23, eager 24..26
     assert np.array_equal(a.read_seismo(0), b.read_seismo(0))
     assert np.array_equal(a.fetch_level(0, cgfd.LEVEL_PRE), b.fetch_level(0, cgfd.LEVEL_PRE))
+    # and both are the oracle's run
+    so, wo, _ = P.oracle_run()
+    assert rel_l2(b.read_seismo(0), so) <= TOL32
+    assert rel_l2(b.fetch_level(0, cgfd.LEVEL_PRE), wo) <= TOL32
     # a replayed period after more eager work still matches
     a.run(27, 13, P.dt)
     b.run(27, 13, P.dt)
@@ -54,6 +59,9 @@ def test_shot_admitted_mid_run_matches_standalone(reserve):
     sa.run(0, A.nt_total, A.dt)
     sb = _sess(B)
     sb.run(0, B.nt_total, B.dt)
+    # the standalone runs are the oracle's, so the batch below is too
+    for P, st in ((A, sa), (B, sb)):
+        assert rel_l2(st.read_seismo(0), P.oracle_run()[0]) <= TOL32
 
     m = _sess(A, nshots=3)
     if reserve:
