@@ -48,6 +48,9 @@ def _mix():
 def test_served_shots_match_standalone_runs(budget_groups):
     probs = [(_problem(k, ijk, nt), k) for k, ijk, nt in _mix()]
     ref = [_alone(P) for P, _ in probs]
+    # the alone runs are the oracle's (FP32 tolerance), so the served ones are too
+    for (P, k), r in zip(probs[:2], ref[:2]):
+        assert rel_l2(r, P.oracle_run()[0]) <= 1e-5, k
     pa = next(P for P, k in probs if k == "a")
     pb = next(P for P, k in probs if k == "b")
     # memory for about budget_groups groups of two shots
