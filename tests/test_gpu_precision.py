@@ -2,8 +2,10 @@
 power-of-two scales, FP32 arithmetic) against the FP64 path of the same scheme.
 
 The FP32 path must agree with FP64 to FP32 rounding; the 16-bit formats are
-held to the bounds below (relative L2 over all seismogram components), and
-the measured errors are printed for the record.
+held to the bounds below (relative L2 over all seismogram components and
+over the final wavefield), about twice the errors measured on MI355X (FP32
+6.3e-7 / 1.7e-6, FP16 1.9e-3 / 8.0e-3, BF16 2.2e-2 / 6.7e-2 after 320
+steps), and the measured errors are printed for the record.
 """
 import numpy as np
 import pytest
@@ -13,7 +15,8 @@ from problem import Problem, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-BOUND = {cgfd.STORE_F32: 1e-4, cgfd.STORE_F16: 3e-2, cgfd.STORE_BF16: 2e-1}
+# (seismograms, final wavefield)
+BOUND = {cgfd.STORE_F32: (1e-5, 1e-5), cgfd.STORE_F16: (4e-3, 1.6e-2), cgfd.STORE_BF16: (4.5e-2, 1.4e-1)}
 NAME = {cgfd.STORE_F32: "f32", cgfd.STORE_F16: "f16", cgfd.STORE_BF16: "bf16"}
 
 
@@ -47,5 +50,5 @@ def test_storage_error_vs_fp64(fp64_run, storage):
     e_snap = rel_l2(w, wref)
     print(f"\n[precision] {NAME[storage]} scales {ex}: seismogram rel L2 {e_seis:.3e}, "
           f"final wavefield rel L2 {e_snap:.3e}")
-    assert e_seis <= BOUND[storage]
-    assert e_snap <= BOUND[storage]
+    assert e_seis <= BOUND[storage][0]
+    assert e_snap <= BOUND[storage][1]
